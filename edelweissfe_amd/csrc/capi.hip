@@ -1,0 +1,417 @@
+// C ABI entry points (include/efe_gpu.h).
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "efe_internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+// Natural node coordinates (xi, eta, zeta) of the reference's hexa8 / hexa20
+// (_elementcomputationmatrices.py:164-212): corners n1..n8, then mid-side nodes.
+const double kNat20[20][3] = {
+    {-1, -1, -1}, {-1, -1, 1}, {1, -1, 1}, {1, -1, -1}, {-1, 1, -1}, {-1, 1, 1}, {1, 1, 1}, {1, 1, -1},
+    {-1, -1, 0},  {0, -1, 1},  {1, -1, 0}, {0, -1, -1}, {-1, 1, 0},  {0, 1, 1},  {1, 1, 0}, {0, 1, -1},
+    {-1, 0, -1},  {-1, 0, 1},  {1, 0, 1},  {1, 0, -1}};
+
+// Trilinear (8) / serendipity (20) shape functions and their natural derivatives.
+void shape(int nn, const double p[3], double* N, double* dN) {
+    const double x = p[0], y = p[1], z = p[2];
+    for (int a = 0; a < nn; ++a) {
+        const double ax = kNat20[a][0], ay = kNat20[a][1], az = kNat20[a][2];
+        if (nn == 8) {
+            N[a] = (1 + ax * x) * (1 + ay * y) * (1 + az * z) / 8;
+            dN[a * 3 + 0] = ax * (1 + ay * y) * (1 + az * z) / 8;
+            dN[a * 3 + 1] = ay * (1 + ax * x) * (1 + az * z) / 8;
+            dN[a * 3 + 2] = az * (1 + ax * x) * (1 + ay * y) / 8;
+        } else if (a < 8) {
+            const double s = ax * x + ay * y + az * z - 2;
+            N[a] = (1 + ax * x) * (1 + ay * y) * (1 + az * z) * s / 8;
+            dN[a * 3 + 0] = ax * (1 + ay * y) * (1 + az * z) * (2 * ax * x + ay * y + az * z - 1) / 8;
+            dN[a * 3 + 1] = ay * (1 + ax * x) * (1 + az * z) * (ax * x + 2 * ay * y + az * z - 1) / 8;
+            dN[a * 3 + 2] = az * (1 + ax * x) * (1 + ay * y) * (ax * x + ay * y + 2 * az * z - 1) / 8;
+        } else if (ax == 0) {
+            N[a] = (1 - x * x) * (1 + ay * y) * (1 + az * z) / 4;
+            dN[a * 3 + 0] = -2 * x * (1 + ay * y) * (1 + az * z) / 4;
+            dN[a * 3 + 1] = ay * (1 - x * x) * (1 + az * z) / 4;
+            dN[a * 3 + 2] = az * (1 - x * x) * (1 + ay * y) / 4;
+        } else if (ay == 0) {
+            N[a] = (1 + ax * x) * (1 - y * y) * (1 + az * z) / 4;
+            dN[a * 3 + 0] = ax * (1 - y * y) * (1 + az * z) / 4;
+            dN[a * 3 + 1] = -2 * y * (1 + ax * x) * (1 + az * z) / 4;
+            dN[a * 3 + 2] = az * (1 + ax * x) * (1 - y * y) / 4;
+        } else {
+            N[a] = (1 + ax * x) * (1 + ay * y) * (1 - z * z) / 4;
+            dN[a * 3 + 0] = ax * (1 + ay * y) * (1 - z * z) / 4;
+            dN[a * 3 + 1] = ay * (1 + ax * x) * (1 - z * z) / 4;
+            dN[a * 3 + 2] = -2 * z * (1 + ax * x) * (1 + ay * y) / 4;
+        }
+    }
+}
+
+// Quadrature tables of edelweissfe/elements/library.py:33-46,212-291.
+void qp_table(int eltype, std::vector<double>& xi, std::vector<double>& eta, std::vector<double>& zeta,
+              std::vector<double>& w) {
+    if (eltype == EFE_C3D20) {
+        const double s = std::sqrt(0.6);
+        const double w1 = std::pow(5.0 / 9.0, 3.0), w2 = std::pow(5.0 / 9.0, 2.0) * (8.0 / 9.0);
+        const double w3 = (5.0 / 9.0) * std::pow(8.0 / 9.0, 2.0), w4 = std::pow(8.0 / 9.0, 3.0);
+        const double t20[9] = {-1, -1, -1, 0, 0, 0, 1, 1, 1};
+        const double s20[3] = {-1, 0, 1};
+        const double wtab[27] = {w1, w2, w1, w2, w3, w2, w1, w2, w1, w2, w3, w2, w3, w4,
+                                 w3, w2, w3, w2, w1, w2, w1, w2, w3, w2, w1, w2, w1};
+        for (int i = 0; i < 27; ++i) {
+            xi.push_back(s * (i < 9 ? -1.0 : (i < 18 ? 0.0 : 1.0)));
+            eta.push_back(s * t20[i % 9]);
+            zeta.push_back(s * s20[i % 3]);
+            w.push_back(wtab[i]);
+        }
+    } else {
+        const double s = 1.0 / std::sqrt(3.0);
+        const double t8[4] = {-1, -1, 1, 1}, s8[4] = {-1, 1, 1, -1};
+        for (int i = 0; i < 8; ++i) {
+            xi.push_back(s * (i < 4 ? -1.0 : 1.0));
+            eta.push_back(s * t8[i % 4]);
+            zeta.push_back(s * s8[i % 4]);
+            w.push_back(1.0);
+        }
+    }
+}
+
+template <typename T>
+int upload(T** dst, const T* src, size_t n, hipStream_t st) {
+    EFE_HIP(hipMalloc(dst, (n ? n : 1) * sizeof(T)));
+    if (n) EFE_HIP(hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice, st));
+    return EFE_OK;
+}
+
+int check(efe_ctx* c) {
+    if (!c) {
+        efe_set_error("null context");
+        return EFE_ERR_ARG;
+    }
+    return EFE_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+}  // namespace
+
+void efe_set_error(const std::string& msg) { g_err = msg; }
+
+extern "C" {
+
+const char* efe_last_error(void) { return g_err.c_str(); }
+
+int efe_ctx_create(const efe_mesh_desc* d, efe_ctx** out) {
+    if (!d || !out || !d->coords || !d->conn || d->n_nodes <= 0 || d->n_elements <= 0 || !d->props) {
+        efe_set_error("invalid mesh descriptor");
+        return EFE_ERR_ARG;
+    }
+    int NN, Q;
+    switch (d->element_type) {
+        case EFE_C3D8: NN = 8; Q = 8; break;
+        case EFE_C3D20: NN = 20; Q = 27; break;
+        case EFE_C3D20R: NN = 20; Q = 8; break;
+        default: efe_set_error("unknown element type"); return EFE_ERR_ARG;
+    }
+    int np = (d->material == EFE_LINEAR_ELASTIC) ? 2 : 6;
+    if (d->material < 0 || d->material > 2 || d->n_props != np) {
+        efe_set_error("material / n_props mismatch (linear elastic: 2, von Mises and Drucker-Prager: 6)");
+        return EFE_ERR_ARG;
+    }
+    if ((int64_t)d->n_nodes * 3 >= INT32_MAX || (int64_t)d->n_elements * NN * NN >= INT32_MAX) {
+        efe_set_error("mesh too large for 32-bit indices");
+        return EFE_ERR_ARG;
+    }
+    efe_ctx* c = new efe_ctx();
+    c->N = d->n_nodes;
+    c->E = d->n_elements;
+    c->NN = NN;
+    c->Q = Q;
+    c->D = 3 * c->N;
+    c->eltype = d->element_type;
+    c->material = d->material;
+    c->nprops = np;
+    c->props_per_element = d->props_per_element ? 1 : 0;
+    c->nstate = 12 + (d->material == EFE_LINEAR_ELASTIC ? 0 : 1);
+    c->stream = (hipStream_t)d->stream;
+    c->nspec = d->specimen_of_node ? d->n_specimens : 1;
+    std::string err;
+    if (!efe_build_pattern(c->hp, c->N, c->E, NN, d->conn, err)) {
+        efe_set_error(err);
+        delete c;
+        return EFE_ERR_ARG;
+    }
+    HostPattern& hp = c->hp;
+    c->nb = hp.bptr[c->N];
+    c->nnz = c->nb * 9;
+    c->nslices = (int)hp.slice_deg.size();
+    hipStream_t st = c->stream;
+    int rc = EFE_OK;
+    auto fail = [&](int r) {
+        efe_ctx_destroy(c);
+        return r;
+    };
+    {
+        std::vector<double> xyz((size_t)3 * c->N);
+        for (int n = 0; n < c->N; ++n)
+            for (int k = 0; k < 3; ++k) xyz[(size_t)k * c->N + n] = d->coords[(size_t)n * 3 + k];
+        if ((rc = upload(&c->coords, xyz.data(), xyz.size(), st))) return fail(rc);
+        std::vector<int32_t> cn((size_t)NN * c->E);
+        for (int e = 0; e < c->E; ++e)
+            for (int a = 0; a < NN; ++a) cn[(size_t)a * c->E + e] = d->conn[(size_t)e * NN + a];
+        if ((rc = upload(&c->conn, cn.data(), cn.size(), st))) return fail(rc);
+        std::vector<double> pr;
+        if (c->props_per_element) {
+            pr.resize((size_t)np * c->E);
+            for (int e = 0; e < c->E; ++e)
+                for (int k = 0; k < np; ++k) pr[(size_t)k * c->E + e] = d->props[(size_t)e * np + k];
+        } else {
+            pr.assign(d->props, d->props + np);
+        }
+        if ((rc = upload(&c->props, pr.data(), pr.size(), st))) return fail(rc);
+        std::vector<double> xi, eta, zeta, w;
+        qp_table(c->eltype, xi, eta, zeta, w);
+        std::vector<double> dn((size_t)Q * NN * 3), sn((size_t)Q * NN), tmp((size_t)NN * 3), tn(NN);
+        for (int q = 0; q < Q; ++q) {
+            // J and B: the reference evaluates QP q at (eta_q, xi_q, zeta_q)
+            const double pe[3] = {eta[q], xi[q], zeta[q]};
+            shape(NN, pe, tn.data(), dn.data() + (size_t)q * NN * 3);
+            // N for body forces: at the table point (xi_q, eta_q, zeta_q)
+            const double pt[3] = {xi[q], eta[q], zeta[q]};
+            shape(NN, pt, sn.data() + (size_t)q * NN, tmp.data());
+        }
+        if ((rc = upload(&c->dNdxi, dn.data(), dn.size(), st))) return fail(rc);
+        if ((rc = upload(&c->shapeN, sn.data(), sn.size(), st))) return fail(rc);
+        if ((rc = upload(&c->qpw, w.data(), w.size(), st))) return fail(rc);
+        if (d->specimen_of_node) {
+            if ((rc = upload(&c->specOfNode, d->specimen_of_node, (size_t)c->N, st))) return fail(rc);
+        }
+    }
+    if ((rc = upload(&c->bptr, hp.bptr.data(), hp.bptr.size(), st))) return fail(rc);
+    if ((rc = upload(&c->bcol, hp.bcol.data(), hp.bcol.size(), st))) return fail(rc);
+    if ((rc = upload(&c->coff, hp.coff.data(), hp.coff.size(), st))) return fail(rc);
+    if ((rc = upload(&c->clist, hp.clist.data(), hp.clist.size(), st))) return fail(rc);
+    if ((rc = upload(&c->nptr, hp.nptr.data(), hp.nptr.size(), st))) return fail(rc);
+    if ((rc = upload(&c->nlist, hp.nlist.data(), hp.nlist.size(), st))) return fail(rc);
+    if ((rc = upload(&c->diagk, hp.diagk.data(), hp.diagk.size(), st))) return fail(rc);
+    if ((rc = upload(&c->slice_off, hp.slice_off.data(), hp.slice_off.size(), st))) return fail(rc);
+    if ((rc = upload(&c->scol, hp.scol.data(), hp.scol.size(), st))) return fail(rc);
+    // no host copies of the large gather lists are needed after upload
+    std::vector<int32_t>().swap(hp.clist);
+    std::vector<int32_t>().swap(hp.coff);
+    std::vector<int32_t>().swap(hp.scol);
+    std::vector<int32_t>().swap(hp.nlist);
+    const size_t sbytes = (size_t)c->nstate * c->E * Q * sizeof(double);
+    auto dalloc = [&](void** p, size_t bytes) -> int {
+        EFE_HIP(hipMalloc(p, bytes ? bytes : 1));
+        EFE_HIP(hipMemsetAsync(*p, 0, bytes ? bytes : 1, st));
+        return EFE_OK;
+    };
+    if ((rc = dalloc((void**)&c->stateRef, sbytes))) return fail(rc);
+    if ((rc = dalloc((void**)&c->stateTmp, sbytes))) return fail(rc);
+    if ((rc = dalloc((void**)&c->Pscr, (size_t)c->E * NN * 3 * sizeof(double)))) return fail(rc);
+    if ((rc = dalloc((void**)&c->dirflag, (size_t)c->D))) return fail(rc);
+    if ((rc = dalloc((void**)&c->status, 4 * sizeof(int)))) return fail(rc);
+    c->redn = 8 + 5 * 1024;
+    if ((rc = dalloc((void**)&c->red, (size_t)c->redn * sizeof(double)))) return fail(rc);
+    for (auto& ev : c->ev) EFE_HIP(hipEventCreate(&ev));
+    EFE_HIP(hipStreamSynchronize(st));
+    *out = c;
+    return EFE_OK;
+}
+
+void efe_ctx_destroy(efe_ctx* c) {
+    if (!c) return;
+    hipDeviceSynchronize();
+    efe_krylov_free(c);
+    void* bufs[] = {c->coords, c->conn,  c->props,  c->dNdxi,  c->shapeN, c->qpw,    c->stateRef, c->stateTmp,
+                    c->Kscr,   c->Pscr,  c->bptr,   c->bcol,   c->coff,   c->clist,  c->nptr,     c->nlist,
+                    c->diagk,  c->slice_off, c->scol, c->K64,  c->K32,    c->Minv64, c->Minv32,   c->dirflag,
+                    c->status, c->red,   c->specOfNode};
+    for (void* p : bufs)
+        if (p) hipFree(p);
+    for (auto& ev : c->ev)
+        if (ev) hipEventDestroy(ev);
+    delete c;
+}
+
+int efe_get_sizes(efe_ctx* c, int64_t out[4]) {
+    if (int rc = check(c)) return rc;
+    out[0] = c->D;
+    out[1] = c->nnz;
+    out[2] = c->Q;
+    out[3] = c->nstate;
+    return EFE_OK;
+}
+
+int efe_get_csr_pattern(efe_ctx* c, int32_t* indptr, int32_t* indices) {
+    if (int rc = check(c)) return rc;
+    const HostPattern& hp = c->hp;
+    indptr[0] = 0;
+    for (int n = 0; n < c->N; ++n) {
+        const int deg = hp.bptr[n + 1] - hp.bptr[n];
+        for (int i = 0; i < 3; ++i) {
+            const int64_t row = 3 * (int64_t)n + i;
+            const int64_t r0 = 9 * (int64_t)hp.bptr[n] + 3 * (int64_t)i * deg;
+            indptr[row + 1] = (int32_t)(r0 + 3 * deg);
+            for (int k = 0; k < deg; ++k)
+                for (int j = 0; j < 3; ++j) indices[r0 + 3 * k + j] = 3 * hp.bcol[hp.bptr[n] + k] + j;
+        }
+    }
+    return EFE_OK;
+}
+
+int efe_set_dirichlet(efe_ctx* c, const int32_t* dofs, int32_t n) {
+    if (int rc = check(c)) return rc;
+    std::vector<uint8_t> f((size_t)c->D, 0);
+    for (int32_t i = 0; i < n; ++i) {
+        if (dofs[i] < 0 || dofs[i] >= c->D) {
+            efe_set_error("Dirichlet DOF out of range");
+            return EFE_ERR_ARG;
+        }
+        f[dofs[i]] = 1;
+    }
+    EFE_HIP(hipMemcpyAsync(c->dirflag, f.data(), f.size(), hipMemcpyHostToDevice, c->stream));
+    EFE_HIP(hipStreamSynchronize(c->stream));
+    c->ndir = n;
+    return EFE_OK;
+}
+
+int efe_compute_elements(efe_ctx* c, const double* U, const double* dU, const double* time, double dT, double* P,
+                         double* F, const double* dir_values, double* dir_correction, int32_t precision,
+                         double* pNewDT) {
+    (void)U;
+    (void)time;
+    (void)dT;
+    if (int rc = check(c)) return rc;
+    if (!dU || !P || !F || (dir_values && !dir_correction) || (precision != EFE_FP64 && precision != EFE_FP32)) {
+        efe_set_error("efe_compute_elements: invalid arguments");
+        return EFE_ERR_ARG;
+    }
+    const size_t nblk = (size_t)c->E * c->NN * c->NN * 9;
+    if (!c->Kscr) EFE_HIP(hipMalloc(&c->Kscr, nblk * sizeof(double)));
+    const size_t nval = (size_t)c->hp.padded_blocks * 9;
+    if (precision == EFE_FP64 && !c->K64) {
+        EFE_HIP(hipMalloc(&c->K64, nval * sizeof(double)));
+        EFE_HIP(hipMemsetAsync(c->K64, 0, nval * sizeof(double), c->stream));
+        EFE_HIP(hipMalloc(&c->Minv64, (size_t)c->N * 9 * sizeof(double)));
+    }
+    if (precision == EFE_FP32 && !c->K32) {
+        EFE_HIP(hipMalloc(&c->K32, nval * sizeof(float)));
+        EFE_HIP(hipMemsetAsync(c->K32, 0, nval * sizeof(float), c->stream));
+        EFE_HIP(hipMalloc(&c->Minv32, (size_t)c->N * 9 * sizeof(float)));
+    }
+    EFE_HIP(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
+    EFE_HIP(hipEventRecord(c->ev[0], c->stream));
+    if (int rc = efe_launch_elements(c, dU, precision)) return rc;
+    EFE_HIP(hipEventRecord(c->ev[1], c->stream));
+    if (int rc = efe_launch_assembly(c, P, F, dir_values, dir_correction, precision)) return rc;
+    EFE_HIP(hipEventRecord(c->ev[2], c->stream));
+    int hs = 0;
+    EFE_HIP(hipMemcpyAsync(&hs, c->status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    EFE_HIP(hipStreamSynchronize(c->stream));
+    c->t_elem = elapsed(c->ev[0], c->ev[1]);
+    c->t_asm = elapsed(c->ev[1], c->ev[2]);
+    if (pNewDT) *pNewDT = hs ? 0.5 : 1.0;
+    return hs ? EFE_CUTBACK : EFE_OK;
+}
+
+int efe_body_force(efe_ctx* c, const double* force3, double* PExt) {
+    if (int rc = check(c)) return rc;
+    return efe_launch_body_force(c, force3, PExt);
+}
+
+int efe_export_csr_values(efe_ctx* c, double* values) {
+    if (int rc = check(c)) return rc;
+    if (c->Kprec < 0) {
+        efe_set_error("no assembled operator");
+        return EFE_ERR_STATE;
+    }
+    return efe_launch_export_csr(c, values);
+}
+
+int efe_spmv(efe_ctx* c, const double* x, double* y) {
+    if (int rc = check(c)) return rc;
+    return efe_launch_spmv64(c, x, y);
+}
+
+int efe_linsolve(efe_ctx* c, const double* b, double* x, const efe_solve_opts* o, efe_solve_info* info) {
+    if (int rc = check(c)) return rc;
+    if (!b || !x || !o || !info) {
+        efe_set_error("efe_linsolve: invalid arguments");
+        return EFE_ERR_ARG;
+    }
+    return efe_krylov_solve(c, b, x, o, info);
+}
+
+int efe_convergence_norms(efe_ctx* c, const double* R, const double* ddU, const double* F, double out[5]) {
+    if (int rc = check(c)) return rc;
+    EFE_HIP(hipEventRecord(c->ev[3], c->stream));
+    int rc = efe_launch_norms(c, R, ddU, F, out);
+    EFE_HIP(hipEventRecord(c->ev[4], c->stream));
+    EFE_HIP(hipEventSynchronize(c->ev[4]));
+    c->t_norm = elapsed(c->ev[3], c->ev[4]);
+    return rc;
+}
+
+int efe_convergence_norms_batched(efe_ctx* c, const double* R, const double* ddU, const double* F, double* out) {
+    if (int rc = check(c)) return rc;
+    return efe_launch_norms_batched(c, R, ddU, F, out);
+}
+
+int efe_accept_state(efe_ctx* c) {
+    if (int rc = check(c)) return rc;
+    std::swap(c->stateRef, c->stateTmp);
+    return EFE_OK;
+}
+
+int efe_get_qp_state(efe_ctx* c, int32_t which, double* out) {
+    if (int rc = check(c)) return rc;
+    const size_t EQ = (size_t)c->E * c->Q, ns = c->nstate;
+    std::vector<double> buf(EQ * ns);
+    EFE_HIP(hipMemcpyAsync(buf.data(), which ? c->stateTmp : c->stateRef, buf.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+    EFE_HIP(hipStreamSynchronize(c->stream));
+    for (size_t k = 0; k < ns; ++k)
+        for (size_t i = 0; i < EQ; ++i) out[i * ns + k] = buf[k * EQ + i];
+    return EFE_OK;
+}
+
+int efe_set_qp_state(efe_ctx* c, const double* in) {
+    if (int rc = check(c)) return rc;
+    const size_t EQ = (size_t)c->E * c->Q, ns = c->nstate;
+    std::vector<double> buf(EQ * ns);
+    for (size_t k = 0; k < ns; ++k)
+        for (size_t i = 0; i < EQ; ++i) buf[k * EQ + i] = in[i * ns + k];
+    EFE_HIP(hipMemcpyAsync(c->stateRef, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    EFE_HIP(hipStreamSynchronize(c->stream));
+    return EFE_OK;
+}
+
+int efe_set_launch(efe_ctx* c, int32_t epw, int32_t kblock) {
+    if (int rc = check(c)) return rc;
+    if (epw > 0) c->epw = epw;
+    if (kblock > 0) c->kblock = kblock;
+    return EFE_OK;
+}
+
+int efe_get_phase_times(efe_ctx* c, double out[4]) {
+    if (int rc = check(c)) return rc;
+    out[0] = c->t_elem;
+    out[1] = c->t_asm;
+    out[2] = c->t_kry;
+    out[3] = c->t_norm;
+    return EFE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,109 @@
+// Internal declarations of the gfx950 NIST-step library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/efe_gpu.h"
+
+#define EFE_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            efe_set_error(std::string(#call) + ": " + hipGetErrorString(e_));           \
+            return EFE_ERR_HIP;                                                         \
+        }                                                                               \
+    } while (0)
+
+void efe_set_error(const std::string& msg);
+
+// Slice height of the sliced-ELL 3x3 block layout: one wavefront of row blocks.
+constexpr int kSlice = 64;
+
+struct HostPattern {
+    int N = 0, E = 0, NN = 0;
+    std::vector<int32_t> bptr, bcol;      // node adjacency (block CSR), columns ascending
+    std::vector<int32_t> diagk;           // position of the diagonal block in each row
+    std::vector<int32_t> coff, clist;     // per block: contributing element blocks, element order
+    std::vector<int32_t> nptr, nlist;     // per node: (e*NN + a) incidences, element order
+    std::vector<int32_t> slice_off;       // [nslices+1] first padded block of each slice
+    std::vector<int32_t> slice_deg;       // [nslices] padded row length of each slice
+    std::vector<int32_t> scol;            // [padded blocks] block column (padding: the row itself)
+    int64_t padded_blocks = 0;
+};
+
+// Builds the adjacency, gather maps and sliced layout from the element connectivity
+// (host, conn [E][NN] node positions).  Returns false on inconsistent input.
+bool efe_build_pattern(HostPattern& hp, int N, int E, int NN, const int32_t* conn, std::string& err);
+
+struct KrylovWork;
+
+struct efe_ctx {
+    int N = 0, E = 0, NN = 0, Q = 0, D = 0;
+    int eltype = 0, material = 0, nprops = 0, props_per_element = 0;
+    int nstate = 0;  // per QP: 12 + material state
+    int64_t nnz = 0; // CSR entries
+    int64_t nb = 0;  // 3x3 blocks
+    int nspec = 1;
+    hipStream_t stream = nullptr;
+    HostPattern hp;
+
+    // device mesh and element data
+    double* coords = nullptr;    // [3][N]
+    int32_t* conn = nullptr;     // [NN][E]
+    double* props = nullptr;     // [nprops][E] or [nprops]
+    double* dNdxi = nullptr;     // [Q][NN][3] at the evaluation points
+    double* shapeN = nullptr;    // [Q][NN] at the table points (body force)
+    double* qpw = nullptr;       // [Q]
+    double* stateRef = nullptr;  // [nstate][E][Q]
+    double* stateTmp = nullptr;
+    void* Kscr = nullptr;        // [E][NN][NN][9] in the assembly precision
+    int KscrPrec = -1;
+    double* Pscr = nullptr;      // [E][NN][3]
+    int32_t* specOfNode = nullptr;
+
+    // device pattern
+    int32_t *bptr = nullptr, *bcol = nullptr, *coff = nullptr, *clist = nullptr;
+    int32_t *nptr = nullptr, *nlist = nullptr, *diagk = nullptr;
+    int32_t *slice_off = nullptr, *scol = nullptr;
+    int nslices = 0;
+
+    // assembled operator
+    double* K64 = nullptr;       // sliced layout, [padded*9]
+    float* K32 = nullptr;
+    double* Minv64 = nullptr;    // [N][9] inverse diagonal blocks
+    float* Minv32 = nullptr;
+    int Kprec = -1;              // precision of the current K
+    uint8_t* dirflag = nullptr;  // [D]
+    int ndir = 0;
+    int* status = nullptr;       // device status word (cutback / NaN)
+    double* red = nullptr;       // reduction scratch
+    int redn = 0;
+
+    KrylovWork* kw = nullptr;
+
+    // launch configuration
+    int epw = 0;       // elements per workgroup of the element kernel (0 = default)
+    int kblock = 256;  // threads per workgroup of Krylov kernels
+
+    // phase timing
+    hipEvent_t ev[8] = {};
+    double t_elem = 0, t_asm = 0, t_kry = 0, t_norm = 0;
+};
+
+// element kernels (elements.hip)
+int efe_launch_elements(efe_ctx* c, const double* dU, int precision);
+int efe_launch_assembly(efe_ctx* c, double* P, double* F, const double* dir_values, double* dir_correction,
+                        int precision);
+int efe_launch_body_force(efe_ctx* c, const double* f3, double* PExt);
+int efe_launch_export_csr(efe_ctx* c, double* values);
+
+// Krylov (krylov.hip)
+int efe_krylov_solve(efe_ctx* c, const double* b, double* x, const efe_solve_opts* o, efe_solve_info* info);
+int efe_launch_spmv64(efe_ctx* c, const double* x, double* y);
+int efe_launch_norms(efe_ctx* c, const double* R, const double* ddU, const double* F, double* out);
+int efe_launch_norms_batched(efe_ctx* c, const double* R, const double* ddU, const double* F, double* out);
+void efe_krylov_free(efe_ctx* c);

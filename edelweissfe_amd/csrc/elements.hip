@@ -1,0 +1,460 @@
+// Element, material and assembly kernels for gfx950.
+//
+// element_kernel: one 64-lane wavefront owns EPW elements.
+//   phase 1, one lane per (element, QP): Jacobian from the gathered node
+//            coordinates and the shape-function derivatives at the reference's
+//            evaluation point, dN/dx, strain increment from the gathered dU,
+//            FP64 material update of the reference state into the temporary
+//            state, and (w detJ) C and (w detJ) sigma staged in LDS;
+//   phase 2, one lane per (element, node a, node b): the 3x3 block
+//            K_ab = sum_q B_a^T (w detJ C_q) B_b in KT precision, written to the
+//            element scratch [e][a][b][9];
+//   phase 3, one lane per (element, node a): Pe_a = -sum_q B_a^T (w detJ sigma_q).
+//   Follows DisplacementElement.computeYourself
+//   (edelweissfe/elements/displacementelement/element.py:290-346).
+//
+// assemble_kernel: one thread per node row-block.  For every block slot of its
+//   row it sums the element blocks listed for that slot in element order (the
+//   order of updateCSR, csrgenerator.pyx:112-113), eliminates Dirichlet rows and
+//   columns symmetrically, writes the sliced-ELL value layout, inverts its
+//   diagonal block for block Jacobi, and gathers P = sum Pe and F = sum |Pe|
+//   (nonlinearimplicitstatic.py:843-844) in element order.  No atomics: the
+//   result is bitwise reproducible.
+#include "efe_internal.h"
+#include "materials.h"
+
+namespace {
+
+template <int MAT>
+struct MatState {
+    static constexpr int n = (MAT == EFE_LINEAR_ELASTIC) ? 0 : 1;
+};
+
+template <int NN, int Q, int MAT, typename KT, int EPW>
+__global__ __launch_bounds__(64) void element_kernel(int E, int N, const double* __restrict__ coords,
+                                                     const int32_t* __restrict__ conn,
+                                                     const double* __restrict__ props, int propStride,
+                                                     const double* __restrict__ dNdxi,
+                                                     const double* __restrict__ qpw,
+                                                     const double* __restrict__ dU,
+                                                     const double* __restrict__ sRef, double* __restrict__ sTmp,
+                                                     KT* __restrict__ Kscr, double* __restrict__ Pscr,
+                                                     int* __restrict__ status) {
+    constexpr int NS = 12 + MatState<MAT>::n;
+    constexpr int NP = (MAT == EFE_LINEAR_ELASTIC) ? 2 : 6;
+    __shared__ double g_s[EPW][Q][NN][3];
+    __shared__ KT fC_s[EPW][Q][36];
+    __shared__ double fs_s[EPW][Q][6];
+    const int lane = threadIdx.x;
+    const int e0 = blockIdx.x * EPW;
+    const int64_t EQ = (int64_t)E * Q;
+
+    for (int t = lane; t < EPW * Q; t += 64) {
+        const int el = t / Q, q = t - (t / Q) * Q;
+        const int e = e0 + el;
+        if (e >= E) continue;
+        const double* dn = dNdxi + (size_t)q * NN * 3;
+        double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll 4
+        for (int a = 0; a < NN; ++a) {
+            const int nd = conn[(int64_t)a * E + e];
+            const double x = coords[nd], y = coords[N + nd], z = coords[2 * N + nd];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const double d = dn[a * 3 + r];
+                J[r * 3 + 0] += d * x;
+                J[r * 3 + 1] += d * y;
+                J[r * 3 + 2] += d * z;
+            }
+        }
+        const double det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
+                           J[2] * (J[3] * J[7] - J[4] * J[6]);
+        const double id = 1.0 / det;
+        double iJ[9];
+        iJ[0] = (J[4] * J[8] - J[5] * J[7]) * id;
+        iJ[1] = (J[2] * J[7] - J[1] * J[8]) * id;
+        iJ[2] = (J[1] * J[5] - J[2] * J[4]) * id;
+        iJ[3] = (J[5] * J[6] - J[3] * J[8]) * id;
+        iJ[4] = (J[0] * J[8] - J[2] * J[6]) * id;
+        iJ[5] = (J[2] * J[3] - J[0] * J[5]) * id;
+        iJ[6] = (J[3] * J[7] - J[4] * J[6]) * id;
+        iJ[7] = (J[1] * J[6] - J[0] * J[7]) * id;
+        iJ[8] = (J[0] * J[4] - J[1] * J[3]) * id;
+        double gr[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // gr[i*3+j] = d u_i / d x_j
+#pragma unroll 4
+        for (int a = 0; a < NN; ++a) {
+            const double d0 = dn[a * 3 + 0], d1 = dn[a * 3 + 1], d2 = dn[a * 3 + 2];
+            double g[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) g[c] = iJ[c * 3 + 0] * d0 + iJ[c * 3 + 1] * d1 + iJ[c * 3 + 2] * d2;
+            g_s[el][q][a][0] = g[0];
+            g_s[el][q][a][1] = g[1];
+            g_s[el][q][a][2] = g[2];
+            const int nd = conn[(int64_t)a * E + e];
+            const double u0 = dU[3 * nd], u1 = dU[3 * nd + 1], u2 = dU[3 * nd + 2];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                gr[0 + j] += u0 * g[j];
+                gr[3 + j] += u1 * g[j];
+                gr[6 + j] += u2 * g[j];
+            }
+        }
+        double de[6] = {gr[0], gr[4], gr[8], gr[1] + gr[3], gr[2] + gr[6], gr[5] + gr[7]};
+        const int64_t sidx = (int64_t)e * Q + q;
+        double s[6], ms = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) s[c] = sRef[c * EQ + sidx];
+        if (MatState<MAT>::n) ms = sRef[12 * EQ + sidx];
+        double p[NP];
+#pragma unroll
+        for (int k = 0; k < NP; ++k) p[k] = propStride ? props[(int64_t)k * propStride + e] : props[k];
+        double C[36];
+        int st = 0;
+        if (MAT == EFE_LINEAR_ELASTIC) st = efe::le_update(p, s, de, C);
+        else if (MAT == EFE_VONMISES) st = efe::vm_update(p, s, ms, de, C);
+        else st = efe::dp_update(p, s, ms, de, C);
+        if (st) *status = 1;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            sTmp[c * EQ + sidx] = s[c];
+            sTmp[(6 + c) * EQ + sidx] = sRef[(6 + c) * EQ + sidx] + de[c];
+        }
+        if (MatState<MAT>::n) sTmp[12 * EQ + sidx] = ms;
+        (void)NS;
+        const double f = det * qpw[q];
+#pragma unroll
+        for (int k = 0; k < 36; ++k) fC_s[el][q][k] = (KT)(f * C[k]);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) fs_s[el][q][k] = f * s[k];
+    }
+    __syncthreads();
+
+    for (int t = lane; t < EPW * NN * NN; t += 64) {
+        const int el = t / (NN * NN);
+        const int ab = t - el * (NN * NN);
+        const int a = ab / NN, b = ab - (ab / NN) * NN;
+        const int e = e0 + el;
+        if (e >= E) continue;
+        KT acc[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) acc[k] = (KT)0;
+        for (int q = 0; q < Q; ++q) {
+            const KT ax = (KT)g_s[el][q][a][0], ay = (KT)g_s[el][q][a][1], az = (KT)g_s[el][q][a][2];
+            const KT bx = (KT)g_s[el][q][b][0], by = (KT)g_s[el][q][b][1], bz = (KT)g_s[el][q][b][2];
+            const KT* C = fC_s[el][q];
+            KT CB[6][3];
+#pragma unroll
+            for (int r = 0; r < 6; ++r) {
+                CB[r][0] = C[r * 6 + 0] * bx + C[r * 6 + 3] * by + C[r * 6 + 4] * bz;
+                CB[r][1] = C[r * 6 + 1] * by + C[r * 6 + 3] * bx + C[r * 6 + 5] * bz;
+                CB[r][2] = C[r * 6 + 2] * bz + C[r * 6 + 4] * bx + C[r * 6 + 5] * by;
+            }
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                acc[0 + j] += ax * CB[0][j] + ay * CB[3][j] + az * CB[4][j];
+                acc[3 + j] += ay * CB[1][j] + ax * CB[3][j] + az * CB[5][j];
+                acc[6 + j] += az * CB[2][j] + ax * CB[4][j] + ay * CB[5][j];
+            }
+        }
+        KT* dst = Kscr + ((int64_t)e * NN * NN + ab) * 9;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) dst[k] = acc[k];
+    }
+
+    for (int t = lane; t < EPW * NN; t += 64) {
+        const int el = t / NN, a = t - (t / NN) * NN;
+        const int e = e0 + el;
+        if (e >= E) continue;
+        double p0 = 0, p1 = 0, p2 = 0;
+        for (int q = 0; q < Q; ++q) {
+            const double ax = g_s[el][q][a][0], ay = g_s[el][q][a][1], az = g_s[el][q][a][2];
+            const double* s = fs_s[el][q];
+            p0 += ax * s[0] + ay * s[3] + az * s[4];
+            p1 += ay * s[1] + ax * s[3] + az * s[5];
+            p2 += az * s[2] + ax * s[4] + ay * s[5];
+        }
+        double* dst = Pscr + ((int64_t)e * NN + a) * 3;
+        dst[0] = -p0;
+        dst[1] = -p1;
+        dst[2] = -p2;
+    }
+}
+
+__device__ __forceinline__ void inv3(const double m[9], double o[9]) {
+    const double det = m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+                       m[2] * (m[3] * m[7] - m[4] * m[6]);
+    const double id = 1.0 / det;
+    o[0] = (m[4] * m[8] - m[5] * m[7]) * id;
+    o[1] = (m[2] * m[7] - m[1] * m[8]) * id;
+    o[2] = (m[1] * m[5] - m[2] * m[4]) * id;
+    o[3] = (m[5] * m[6] - m[3] * m[8]) * id;
+    o[4] = (m[0] * m[8] - m[2] * m[6]) * id;
+    o[5] = (m[2] * m[3] - m[0] * m[5]) * id;
+    o[6] = (m[3] * m[7] - m[4] * m[6]) * id;
+    o[7] = (m[1] * m[6] - m[0] * m[7]) * id;
+    o[8] = (m[0] * m[4] - m[1] * m[3]) * id;
+}
+
+template <typename KT>
+__global__ __launch_bounds__(256) void assemble_kernel(
+    int N, const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol, const int32_t* __restrict__ coff,
+    const int32_t* __restrict__ clist, const int32_t* __restrict__ nptr, const int32_t* __restrict__ nlist,
+    const int32_t* __restrict__ slice_off, const KT* __restrict__ Kscr, const double* __restrict__ Pscr,
+    const uint8_t* __restrict__ dirflag, int ndir, const double* __restrict__ dirv, double* __restrict__ dircorr,
+    KT* __restrict__ Kval, KT* __restrict__ Minv, double* __restrict__ P, double* __restrict__ F) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int s = n / kSlice, l = n - (n / kSlice) * kSlice;
+    const int64_t vbase = (int64_t)slice_off[s] * 9 + l;
+    bool fr[3] = {false, false, false};
+    if (ndir) {
+        fr[0] = dirflag[3 * n];
+        fr[1] = dirflag[3 * n + 1];
+        fr[2] = dirflag[3 * n + 2];
+    }
+    double t0 = 0, t1 = 0, t2 = 0;
+    double dg[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    const int b0 = bptr[n], b1 = bptr[n + 1];
+    for (int b = b0; b < b1; ++b) {
+        const int m = bcol[b];
+        KT acc[9];
+#pragma unroll
+        for (int v = 0; v < 9; ++v) acc[v] = (KT)0;
+        for (int c = coff[b]; c < coff[b + 1]; ++c) {
+            const KT* src = Kscr + (int64_t)clist[c] * 9;
+#pragma unroll
+            for (int v = 0; v < 9; ++v) acc[v] += src[v];
+        }
+        if (dirv) {
+            const double d0 = dirv[3 * m], d1 = dirv[3 * m + 1], d2 = dirv[3 * m + 2];
+            t0 += (double)acc[0] * d0 + (double)acc[1] * d1 + (double)acc[2] * d2;
+            t1 += (double)acc[3] * d0 + (double)acc[4] * d1 + (double)acc[5] * d2;
+            t2 += (double)acc[6] * d0 + (double)acc[7] * d1 + (double)acc[8] * d2;
+        }
+        if (ndir) {
+            const bool fc[3] = {(bool)dirflag[3 * m], (bool)dirflag[3 * m + 1], (bool)dirflag[3 * m + 2]};
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    if (fr[i] || fc[j]) acc[i * 3 + j] = (m == n && i == j) ? (KT)1 : (KT)0;
+        }
+        const int k = b - b0;
+#pragma unroll
+        for (int v = 0; v < 9; ++v) Kval[vbase + (int64_t)(k * 9 + v) * kSlice] = acc[v];
+        if (m == n) {
+#pragma unroll
+            for (int v = 0; v < 9; ++v) dg[v] = (double)acc[v];
+        }
+    }
+    double iv[9];
+    inv3(dg, iv);
+#pragma unroll
+    for (int v = 0; v < 9; ++v) Minv[(int64_t)n * 9 + v] = (KT)iv[v];
+    if (P) {
+        double p0 = 0, p1 = 0, p2 = 0, f0 = 0, f1 = 0, f2 = 0;
+        for (int c = nptr[n]; c < nptr[n + 1]; ++c) {
+            const double* src = Pscr + (int64_t)nlist[c] * 3;
+            p0 += src[0];
+            p1 += src[1];
+            p2 += src[2];
+            f0 += fabs(src[0]);
+            f1 += fabs(src[1]);
+            f2 += fabs(src[2]);
+        }
+        P[3 * n] = p0;
+        P[3 * n + 1] = p1;
+        P[3 * n + 2] = p2;
+        F[3 * n] = f0;
+        F[3 * n + 1] = f1;
+        F[3 * n + 2] = f2;
+    }
+    if (dirv) {
+        dircorr[3 * n] = fr[0] ? 0.0 : t0;
+        dircorr[3 * n + 1] = fr[1] ? 0.0 : t1;
+        dircorr[3 * n + 2] = fr[2] ? 0.0 : t2;
+    }
+}
+
+// Body force (element.py:348-371): Pe = sum_q N_q (x) f detJ_q w_q with N at the table
+// point and detJ at the evaluation point; written to the element scratch, then gathered.
+template <int NN, int Q>
+__global__ __launch_bounds__(64) void body_force_kernel(int E, int N, const double* __restrict__ coords,
+                                                        const int32_t* __restrict__ conn,
+                                                        const double* __restrict__ dNdxi,
+                                                        const double* __restrict__ shapeN,
+                                                        const double* __restrict__ qpw, double fx, double fy,
+                                                        double fz, double* __restrict__ Pscr) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    double acc[NN];
+#pragma unroll
+    for (int a = 0; a < NN; ++a) acc[a] = 0.0;
+    for (int q = 0; q < Q; ++q) {
+        const double* dn = dNdxi + (size_t)q * NN * 3;
+        double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int a = 0; a < NN; ++a) {
+            const int nd = conn[(int64_t)a * E + e];
+            const double x = coords[nd], y = coords[N + nd], z = coords[2 * N + nd];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                J[r * 3 + 0] += dn[a * 3 + r] * x;
+                J[r * 3 + 1] += dn[a * 3 + r] * y;
+                J[r * 3 + 2] += dn[a * 3 + r] * z;
+            }
+        }
+        const double det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
+                           J[2] * (J[3] * J[7] - J[4] * J[6]);
+#pragma unroll
+        for (int a = 0; a < NN; ++a) acc[a] += shapeN[q * NN + a] * det * qpw[q];
+    }
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+        double* dst = Pscr + ((int64_t)e * NN + a) * 3;
+        dst[0] = acc[a] * fx;
+        dst[1] = acc[a] * fy;
+        dst[2] = acc[a] * fz;
+    }
+}
+
+__global__ void gather_add_kernel(int N, const int32_t* __restrict__ nptr, const int32_t* __restrict__ nlist,
+                                  const double* __restrict__ Pscr, double* __restrict__ out) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double p0 = 0, p1 = 0, p2 = 0;
+    for (int c = nptr[n]; c < nptr[n + 1]; ++c) {
+        const double* src = Pscr + (int64_t)nlist[c] * 3;
+        p0 += src[0];
+        p1 += src[1];
+        p2 += src[2];
+    }
+    out[3 * n] += p0;
+    out[3 * n + 1] += p1;
+    out[3 * n + 2] += p2;
+}
+
+template <typename KT>
+__global__ void export_csr_kernel(int N, const int32_t* __restrict__ bptr, const int32_t* __restrict__ slice_off,
+                                  const KT* __restrict__ Kval, double* __restrict__ out) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int s = n / kSlice, l = n - (n / kSlice) * kSlice;
+    const int64_t vbase = (int64_t)slice_off[s] * 9 + l;
+    const int deg = bptr[n + 1] - bptr[n];
+    const int64_t r0 = (int64_t)9 * bptr[n];
+    for (int k = 0; k < deg; ++k)
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                out[r0 + (int64_t)3 * i * deg + 3 * k + j] = (double)Kval[vbase + (int64_t)(k * 9 + i * 3 + j) * kSlice];
+}
+
+template <int NN, int Q, int MAT, typename KT, int EPW>
+void launch_elem_t(efe_ctx* c, const double* dU) {
+    const int grid = (c->E + EPW - 1) / EPW;
+    const int propStride = c->props_per_element ? c->E : 0;
+    element_kernel<NN, Q, MAT, KT, EPW><<<grid, 64, 0, c->stream>>>(
+        c->E, c->N, c->coords, c->conn, c->props, propStride, c->dNdxi, c->qpw, dU, c->stateRef, c->stateTmp,
+        (KT*)c->Kscr, c->Pscr, c->status);
+}
+
+template <int NN, int Q, int MAT, typename KT>
+int launch_elem_epw(efe_ctx* c, const double* dU) {
+    int epw = c->epw;
+    if (epw <= 0) epw = (NN == 8) ? 4 : (Q == 8 ? 2 : 1);
+    switch (epw) {
+        case 1: launch_elem_t<NN, Q, MAT, KT, 1>(c, dU); break;
+        case 2: launch_elem_t<NN, Q, MAT, KT, 2>(c, dU); break;
+        case 4:
+            if (NN == 8) { launch_elem_t<NN, Q, MAT, KT, (NN == 8 ? 4 : 1)>(c, dU); break; }
+            [[fallthrough]];
+        case 8:
+            if (NN == 8 && Q == 8) { launch_elem_t<NN, Q, MAT, KT, (NN == 8 && Q == 8 ? 8 : 1)>(c, dU); break; }
+            [[fallthrough]];
+        default:
+            efe_set_error("elements per workgroup must be 1, 2, 4 or 8 (hexa20: 1 or 2)");
+            return EFE_ERR_ARG;
+    }
+    return EFE_OK;
+}
+
+template <int NN, int Q, typename KT>
+int launch_elem_mat(efe_ctx* c, const double* dU) {
+    switch (c->material) {
+        case EFE_LINEAR_ELASTIC: return launch_elem_epw<NN, Q, EFE_LINEAR_ELASTIC, KT>(c, dU);
+        case EFE_VONMISES: return launch_elem_epw<NN, Q, EFE_VONMISES, KT>(c, dU);
+        case EFE_DRUCKER_PRAGER: return launch_elem_epw<NN, Q, EFE_DRUCKER_PRAGER, KT>(c, dU);
+    }
+    efe_set_error("unknown material");
+    return EFE_ERR_ARG;
+}
+
+template <typename KT>
+int launch_elem_type(efe_ctx* c, const double* dU) {
+    switch (c->eltype) {
+        case EFE_C3D8: return launch_elem_mat<8, 8, KT>(c, dU);
+        case EFE_C3D20: return launch_elem_mat<20, 27, KT>(c, dU);
+        case EFE_C3D20R: return launch_elem_mat<20, 8, KT>(c, dU);
+    }
+    efe_set_error("unknown element type");
+    return EFE_ERR_ARG;
+}
+
+}  // namespace
+
+int efe_launch_elements(efe_ctx* c, const double* dU, int precision) {
+    int rc = (precision == EFE_FP32) ? launch_elem_type<float>(c, dU) : launch_elem_type<double>(c, dU);
+    if (rc != EFE_OK) return rc;
+    EFE_HIP(hipGetLastError());
+    return EFE_OK;
+}
+
+int efe_launch_assembly(efe_ctx* c, double* P, double* F, const double* dir_values, double* dir_correction,
+                        int precision) {
+    const int bs = 256, grid = (c->N + bs - 1) / bs;
+    if (precision == EFE_FP32) {
+        assemble_kernel<float><<<grid, bs, 0, c->stream>>>(
+            c->N, c->bptr, c->bcol, c->coff, c->clist, c->nptr, c->nlist, c->slice_off, (const float*)c->Kscr,
+            c->Pscr, c->dirflag, c->ndir, dir_values, dir_correction, c->K32, c->Minv32, P, F);
+    } else {
+        assemble_kernel<double><<<grid, bs, 0, c->stream>>>(
+            c->N, c->bptr, c->bcol, c->coff, c->clist, c->nptr, c->nlist, c->slice_off, (const double*)c->Kscr,
+            c->Pscr, c->dirflag, c->ndir, dir_values, dir_correction, c->K64, c->Minv64, P, F);
+    }
+    EFE_HIP(hipGetLastError());
+    c->Kprec = precision;
+    return EFE_OK;
+}
+
+int efe_launch_body_force(efe_ctx* c, const double* f3, double* PExt) {
+    const int grid = (c->E + 63) / 64;
+    switch (c->eltype) {
+        case EFE_C3D8:
+            body_force_kernel<8, 8><<<grid, 64, 0, c->stream>>>(c->E, c->N, c->coords, c->conn, c->dNdxi,
+                                                               c->shapeN, c->qpw, f3[0], f3[1], f3[2], c->Pscr);
+            break;
+        case EFE_C3D20:
+            body_force_kernel<20, 27><<<grid, 64, 0, c->stream>>>(c->E, c->N, c->coords, c->conn, c->dNdxi,
+                                                                 c->shapeN, c->qpw, f3[0], f3[1], f3[2], c->Pscr);
+            break;
+        case EFE_C3D20R:
+            body_force_kernel<20, 8><<<grid, 64, 0, c->stream>>>(c->E, c->N, c->coords, c->conn, c->dNdxi,
+                                                                c->shapeN, c->qpw, f3[0], f3[1], f3[2], c->Pscr);
+            break;
+        default:
+            efe_set_error("unknown element type");
+            return EFE_ERR_ARG;
+    }
+    gather_add_kernel<<<(c->N + 255) / 256, 256, 0, c->stream>>>(c->N, c->nptr, c->nlist, c->Pscr, PExt);
+    EFE_HIP(hipGetLastError());
+    return EFE_OK;
+}
+
+int efe_launch_export_csr(efe_ctx* c, double* values) {
+    const int grid = (c->N + 255) / 256;
+    if (c->Kprec == EFE_FP32)
+        export_csr_kernel<float><<<grid, 256, 0, c->stream>>>(c->N, c->bptr, c->slice_off, c->K32, values);
+    else
+        export_csr_kernel<double><<<grid, 256, 0, c->stream>>>(c->N, c->bptr, c->slice_off, c->K64, values);
+    EFE_HIP(hipGetLastError());
+    return EFE_OK;
+}

@@ -1,0 +1,576 @@
+// Krylov solvers for the assembled tangent on gfx950.
+//
+// Operator layout: sliced ELL of 3x3 blocks, slice height 64 (one wavefront).
+// Thread (slice s, lane l) owns node row n = 64 s + l; block slot k of that row
+// stores its 9 values at Kval[slice_off[s]*9 + (9k+v)*64 + l], so every value
+// load of a wavefront is one contiguous 512 B (FP64) / 256 B (FP32) segment and
+// the block column index (1 int per 9 values) is shared by the three rows of a
+// node.  Padding slots hold zeros and point at the row itself.
+//
+// PCG with 3x3 block-Jacobi (the survey's replacement for the direct solve of
+// nonlinearimplicitstatic.py:727-751 on symmetric tangents).  One iteration is
+// three kernels:
+//   A: q = K p, per-workgroup partials of p.q
+//   B: alpha = rz/pq; x += alpha p; r -= alpha q; z = Minv r; partials of r.z, r.r
+//   C: beta = rz_new/rz; p = z + beta p; convergence test
+// Global scalars are never stored by one workgroup for the others: every
+// workgroup re-reduces the (few thousand) partials of the previous kernel in the
+// same fixed order, so all workgroups hold bitwise identical scalars and the
+// whole solve is deterministic.  The two r.z partial buffers alternate with the
+// iteration parity, so a batch of an even number of iterations is a fixed
+// sequence of launches that is captured once into a hipGraph and replayed.
+#include <cmath>
+#include <cstring>
+
+#include "efe_internal.h"
+
+struct KrylovWork {
+    int D = 0, N = 0, nwg = 0, block = 0;
+    // FP64 and FP32 vector sets
+    double *x64 = nullptr, *r64 = nullptr, *z64 = nullptr, *p64 = nullptr, *q64 = nullptr;
+    float *x32 = nullptr, *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr;
+    double* part = nullptr;  // [5][nwg]: pq, rz0, rz1, rr, bb
+    int* flags = nullptr;    // device: [done, nan, iters_local, pad]
+    double* dscal = nullptr; // device: [tol2, scratch sum]
+    int* hflags = nullptr;   // pinned host copy
+    double* hpart = nullptr; // pinned host scalars
+    hipStream_t ks = nullptr;
+    hipEvent_t evA = nullptr, evB = nullptr, t0 = nullptr, t1 = nullptr;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_batch[2] = {0, 0};
+    int graph_block[2] = {0, 0};
+};
+
+namespace {
+
+constexpr int kMaxBlock = 1024;
+
+template <int BS>
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    // deterministic tree: fixed pairing order
+    sh[threadIdx.x] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = BS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// Sum of a partial array in a fixed order, computed identically by every workgroup.
+template <int BS>
+__device__ __forceinline__ double reduce_partials(const double* __restrict__ part, int n, double* sh) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += BS) v += part[i];
+    return block_sum<BS>(v, sh);
+}
+
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void spmv_kernel(int N, const int32_t* __restrict__ slice_off,
+                                                  const int32_t* __restrict__ scol, const T* __restrict__ K,
+                                                  const T* __restrict__ x, T* __restrict__ y,
+                                                  double* __restrict__ part, const int* __restrict__ flags) {
+    __shared__ double sh[BS];
+    if (flags && flags[0]) return;
+    const int n = blockIdx.x * BS + threadIdx.x;
+    double dot = 0.0;
+    if (n < N) {
+        const int s = n / kSlice, l = n - (n / kSlice) * kSlice;
+        const int o0 = slice_off[s], deg = (slice_off[s + 1] - o0) / kSlice;
+        const int32_t* cp = scol + o0 + l;
+        const T* kp = K + (int64_t)o0 * 9 + l;
+        T y0 = 0, y1 = 0, y2 = 0;
+        for (int k = 0; k < deg; ++k) {
+            const int m = cp[k * kSlice];
+            const T x0 = x[3 * m], x1 = x[3 * m + 1], x2 = x[3 * m + 2];
+            const T* v = kp + (int64_t)k * 9 * kSlice;
+            y0 += v[0 * kSlice] * x0 + v[1 * kSlice] * x1 + v[2 * kSlice] * x2;
+            y1 += v[3 * kSlice] * x0 + v[4 * kSlice] * x1 + v[5 * kSlice] * x2;
+            y2 += v[6 * kSlice] * x0 + v[7 * kSlice] * x1 + v[8 * kSlice] * x2;
+        }
+        y[3 * n] = y0;
+        y[3 * n + 1] = y1;
+        y[3 * n + 2] = y2;
+        if (part) dot = (double)x[3 * n] * y0 + (double)x[3 * n + 1] * y1 + (double)x[3 * n + 2] * y2;
+    }
+    if (part) {
+        double s = block_sum<BS>(dot, sh);
+        if (threadIdx.x == 0) part[blockIdx.x] = s;
+    }
+}
+
+// x = 0, r = b (b_dir = b on Dirichlet DOFs is part of the system), z = Minv r, p = z,
+// partials r.z -> part[1], b.b -> part[4].
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void pcg_init_kernel(int N, const double* __restrict__ b, const T* __restrict__ Minv,
+                                                      T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+                                                      T* __restrict__ p, double* __restrict__ part, int nwg,
+                                                      int* __restrict__ flags) {
+    __shared__ double sh[BS];
+    const int n = blockIdx.x * BS + threadIdx.x;
+    double rz = 0, bb = 0;
+    if (n < N) {
+        T rv[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            rv[i] = (T)b[3 * n + i];
+            x[3 * n + i] = 0;
+            r[3 * n + i] = rv[i];
+            bb += (double)b[3 * n + i] * b[3 * n + i];
+        }
+        const T* M = Minv + (int64_t)n * 9;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            T zi = M[3 * i] * rv[0] + M[3 * i + 1] * rv[1] + M[3 * i + 2] * rv[2];
+            z[3 * n + i] = zi;
+            p[3 * n + i] = zi;
+            rz += (double)rv[i] * zi;
+        }
+    }
+    double s1 = block_sum<BS>(rz, sh);
+    double s2 = block_sum<BS>(bb, sh);
+    if (threadIdx.x == 0) {
+        part[1 * nwg + blockIdx.x] = s1;
+        part[4 * nwg + blockIdx.x] = s2;
+        if (blockIdx.x == 0) {
+            flags[0] = 0;
+            flags[1] = 0;
+            flags[2] = 0;
+        }
+    }
+}
+
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void pcg_update_kernel(int N, const T* __restrict__ Minv, T* __restrict__ x,
+                                                        T* __restrict__ r, T* __restrict__ z,
+                                                        const T* __restrict__ p, const T* __restrict__ q,
+                                                        double* __restrict__ part, int nwg, int par,
+                                                        const int* __restrict__ flags) {
+    __shared__ double sh[BS];
+    if (flags[0]) return;
+    const double pq = reduce_partials<BS>(part, nwg, sh);
+    const double rz = reduce_partials<BS>(part + (1 + par) * nwg, nwg, sh);
+    const T alpha = (T)(rz / pq);
+    const int n = blockIdx.x * BS + threadIdx.x;
+    double rzn = 0, rr = 0;
+    if (n < N) {
+        T rv[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            x[3 * n + i] += alpha * p[3 * n + i];
+            rv[i] = r[3 * n + i] - alpha * q[3 * n + i];
+            r[3 * n + i] = rv[i];
+            rr += (double)rv[i] * rv[i];
+        }
+        const T* M = Minv + (int64_t)n * 9;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            T zi = M[3 * i] * rv[0] + M[3 * i + 1] * rv[1] + M[3 * i + 2] * rv[2];
+            z[3 * n + i] = zi;
+            rzn += (double)rv[i] * zi;
+        }
+    }
+    double s1 = block_sum<BS>(rzn, sh);
+    double s2 = block_sum<BS>(rr, sh);
+    if (threadIdx.x == 0) {
+        part[(2 - par) * nwg + blockIdx.x] = s1;
+        part[3 * nwg + blockIdx.x] = s2;
+    }
+}
+
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void pcg_direction_kernel(int N, const T* __restrict__ z, T* __restrict__ p,
+                                                           const double* __restrict__ part, int nwg, int par,
+                                                           int jlocal, const double* __restrict__ dtol2,
+                                                           int* __restrict__ flags) {
+    __shared__ double sh[BS];
+    if (flags[0]) return;
+    const double rzo = reduce_partials<BS>(part + (1 + par) * nwg, nwg, sh);
+    const double rzn = reduce_partials<BS>(part + (2 - par) * nwg, nwg, sh);
+    const double rr = reduce_partials<BS>(part + 3 * nwg, nwg, sh);
+    const bool bad = !(rr == rr) || !(rzn == rzn);
+    const bool done = bad || rr <= dtol2[0];
+    if (done) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            flags[2] = jlocal + 1;
+            flags[1] = bad ? 1 : 0;
+            flags[0] = 1;
+        }
+        return;
+    }
+    const T beta = (T)(rzn / rzo);
+    const int n = blockIdx.x * BS + threadIdx.x;
+    if (n < N) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) p[3 * n + i] = z[3 * n + i] + beta * p[3 * n + i];
+    }
+}
+
+// x_out = x (converted), with Dirichlet DOFs taken from b exactly.
+template <typename T>
+__global__ void pcg_finish_kernel(int D, const T* __restrict__ x, const double* __restrict__ b,
+                                  const uint8_t* __restrict__ dirflag, int ndir, double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    out[i] = (ndir && dirflag[i]) ? b[i] : (double)x[i];
+}
+
+// Convergence quantities: max|R| (first index on ties), max|ddU|, sum|F|, NaN flag.
+template <int BS>
+__global__ __launch_bounds__(BS) void norms_stage1(int D, const double* __restrict__ R, const double* __restrict__ ddU,
+                                                   const double* __restrict__ F, double* __restrict__ part,
+                                                   int nwg) {
+    __shared__ double sv[BS];
+    __shared__ long long si[BS];
+    __shared__ double sh[BS];
+    double mr = -1.0, md = 0.0, sf = 0.0, nan = 0.0;
+    long long ir = 0x7fffffffffffffffLL;
+    for (int i = blockIdx.x * BS + threadIdx.x; i < D; i += BS * gridDim.x) {
+        const double a = fabs(R[i]);
+        if (!(a == a)) nan = 1.0;
+        if (a > mr) {
+            mr = a;
+            ir = i;
+        }
+        if (ddU) {
+            const double d = fabs(ddU[i]);
+            if (!(d == d)) nan = 1.0;
+            md = d > md ? d : md;
+        }
+        sf += fabs(F[i]);
+    }
+    sv[threadIdx.x] = mr;
+    si[threadIdx.x] = ir;
+    __syncthreads();
+    for (int s = BS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            const double o = sv[threadIdx.x + s];
+            const long long oi = si[threadIdx.x + s];
+            if (o > sv[threadIdx.x] || (o == sv[threadIdx.x] && oi < si[threadIdx.x])) {
+                sv[threadIdx.x] = o;
+                si[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    const double smax = sv[0];
+    const long long simax = si[0];
+    __syncthreads();
+    sh[threadIdx.x] = md;
+    __syncthreads();
+    for (int s = BS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const double dmax = sh[0];
+    __syncthreads();
+    const double fsum = block_sum<BS>(sf, sh);
+    const double nn = block_sum<BS>(nan, sh);
+    if (threadIdx.x == 0) {
+        part[0 * nwg + blockIdx.x] = smax;
+        part[1 * nwg + blockIdx.x] = (double)simax;
+        part[2 * nwg + blockIdx.x] = dmax;
+        part[3 * nwg + blockIdx.x] = fsum;
+        part[4 * nwg + blockIdx.x] = nn;
+    }
+}
+
+__global__ void sum_partials_kernel(const double* __restrict__ part, int n, double* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += part[i];
+    *out = s;
+}
+
+__global__ void norms_stage2(const double* __restrict__ part, int nwg, double* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    double mr = -1.0, ir = 0, md = 0, sf = 0, nn = 0;
+    for (int w = 0; w < nwg; ++w) {
+        const double v = part[w];
+        if (v > mr || (v == mr && part[nwg + w] < ir)) {
+            mr = v;
+            ir = part[nwg + w];
+        }
+        md = fmax(md, part[2 * nwg + w]);
+        sf += part[3 * nwg + w];
+        nn += part[4 * nwg + w];
+    }
+    out[0] = mr < 0 ? 0.0 : mr;
+    out[1] = ir;
+    out[2] = md;
+    out[3] = sf;
+    out[4] = nn > 0 ? 1.0 : 0.0;
+}
+
+// Per-specimen norms (batched contexts): one workgroup per specimen over its DOF range.
+template <int BS>
+__global__ __launch_bounds__(BS) void norms_batched_kernel(const int64_t* __restrict__ dofoff,
+                                                           const double* __restrict__ R,
+                                                           const double* __restrict__ ddU,
+                                                           const double* __restrict__ F, double* __restrict__ out) {
+    __shared__ double sh[BS];
+    const int sp = blockIdx.x;
+    double mr = 0, md = 0, sf = 0, nan = 0;
+    for (int64_t i = dofoff[sp] + threadIdx.x; i < dofoff[sp + 1]; i += BS) {
+        const double a = fabs(R[i]);
+        if (!(a == a)) nan = 1.0;
+        mr = fmax(mr, a);
+        if (ddU) {
+            const double d = fabs(ddU[i]);
+            if (!(d == d)) nan = 1.0;
+            md = fmax(md, d);
+        }
+        sf += fabs(F[i]);
+    }
+    sh[threadIdx.x] = mr;
+    __syncthreads();
+    for (int s = BS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const double rmax = sh[0];
+    __syncthreads();
+    sh[threadIdx.x] = md;
+    __syncthreads();
+    for (int s = BS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const double dmax = sh[0];
+    __syncthreads();
+    const double fs = block_sum<BS>(sf, sh);
+    const double nn = block_sum<BS>(nan, sh);
+    if (threadIdx.x == 0) {
+        out[sp * 4 + 0] = rmax;
+        out[sp * 4 + 1] = dmax;
+        out[sp * 4 + 2] = fs;
+        out[sp * 4 + 3] = nn > 0 ? 1.0 : 0.0;
+    }
+}
+
+template <typename T, int BS>
+void launch_iter(efe_ctx* c, KrylovWork* w, hipStream_t st, T* x, T* r, T* z, T* p, T* q, const T* K,
+                 const T* Minv, int par, int j) {
+    const int grid = w->nwg;
+    spmv_kernel<T, BS><<<grid, BS, 0, st>>>(c->N, c->slice_off, c->scol, K, p, q, w->part, w->flags);
+    pcg_update_kernel<T, BS><<<grid, BS, 0, st>>>(c->N, Minv, x, r, z, p, q, w->part, grid, par, w->flags);
+    pcg_direction_kernel<T, BS><<<grid, BS, 0, st>>>(c->N, z, p, w->part, grid, par, j, w->dscal, w->flags);
+}
+
+template <typename T>
+void launch_iter_bs(efe_ctx* c, KrylovWork* w, hipStream_t st, T* x, T* r, T* z, T* p, T* q, const T* K,
+                    const T* Minv, int par, int j) {
+    switch (w->block) {
+        case 64: launch_iter<T, 64>(c, w, st, x, r, z, p, q, K, Minv, par, j); break;
+        case 128: launch_iter<T, 128>(c, w, st, x, r, z, p, q, K, Minv, par, j); break;
+        case 512: launch_iter<T, 512>(c, w, st, x, r, z, p, q, K, Minv, par, j); break;
+        default: launch_iter<T, 256>(c, w, st, x, r, z, p, q, K, Minv, par, j); break;
+    }
+}
+
+template <typename T>
+void launch_init_bs(efe_ctx* c, KrylovWork* w, hipStream_t st, const double* b, const T* Minv, T* x, T* r, T* z,
+                    T* p) {
+    switch (w->block) {
+        case 64: pcg_init_kernel<T, 64><<<w->nwg, 64, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->nwg, w->flags); break;
+        case 128: pcg_init_kernel<T, 128><<<w->nwg, 128, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->nwg, w->flags); break;
+        case 512: pcg_init_kernel<T, 512><<<w->nwg, 512, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->nwg, w->flags); break;
+        default: pcg_init_kernel<T, 256><<<w->nwg, 256, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->nwg, w->flags); break;
+    }
+}
+
+int ensure_work(efe_ctx* c) {
+    int block = c->kblock;
+    if (block != 64 && block != 128 && block != 256 && block != 512) block = 256;
+    KrylovWork* w = c->kw;
+    if (w && w->block == block) return EFE_OK;
+    if (!w) {
+        w = new KrylovWork();
+        c->kw = w;
+        w->D = c->D;
+        w->N = c->N;
+        size_t D = (size_t)c->D;
+        EFE_HIP(hipMalloc(&w->x64, D * sizeof(double)));
+        EFE_HIP(hipMalloc(&w->r64, D * sizeof(double)));
+        EFE_HIP(hipMalloc(&w->z64, D * sizeof(double)));
+        EFE_HIP(hipMalloc(&w->p64, D * sizeof(double)));
+        EFE_HIP(hipMalloc(&w->q64, D * sizeof(double)));
+        EFE_HIP(hipMalloc(&w->x32, D * sizeof(float)));
+        EFE_HIP(hipMalloc(&w->r32, D * sizeof(float)));
+        EFE_HIP(hipMalloc(&w->z32, D * sizeof(float)));
+        EFE_HIP(hipMalloc(&w->p32, D * sizeof(float)));
+        EFE_HIP(hipMalloc(&w->q32, D * sizeof(float)));
+        EFE_HIP(hipMalloc(&w->flags, 4 * sizeof(int)));
+        EFE_HIP(hipMalloc(&w->dscal, 4 * sizeof(double)));
+        EFE_HIP(hipHostMalloc(&w->hflags, 4 * sizeof(int), hipHostMallocDefault));
+        EFE_HIP(hipHostMalloc(&w->hpart, 8 * sizeof(double), hipHostMallocDefault));
+        EFE_HIP(hipStreamCreateWithFlags(&w->ks, hipStreamNonBlocking));
+        EFE_HIP(hipEventCreateWithFlags(&w->evA, hipEventDisableTiming));
+        EFE_HIP(hipEventCreateWithFlags(&w->evB, hipEventDisableTiming));
+        EFE_HIP(hipEventCreate(&w->t0));
+        EFE_HIP(hipEventCreate(&w->t1));
+    }
+    for (int i = 0; i < 2; ++i) {
+        if (w->graph[i]) hipGraphExecDestroy(w->graph[i]);
+        w->graph[i] = nullptr;
+    }
+    if (w->part) hipFree(w->part);
+    w->block = block;
+    w->nwg = (c->N + block - 1) / block;
+    EFE_HIP(hipMalloc(&w->part, (size_t)5 * w->nwg * sizeof(double)));
+    return EFE_OK;
+}
+
+template <typename T>
+int pcg_solve(efe_ctx* c, const double* b, double* xout, const efe_solve_opts* o, efe_solve_info* info, T* x, T* r,
+              T* z, T* p, T* q, const T* K, const T* Minv, int gi) {
+    KrylovWork* w = c->kw;
+    hipStream_t st = w->ks;
+    // order after the caller's stream
+    EFE_HIP(hipEventRecord(w->evA, c->stream));
+    EFE_HIP(hipStreamWaitEvent(st, w->evA, 0));
+    launch_init_bs<T>(c, w, st, b, Minv, x, r, z, p);
+    EFE_HIP(hipGetLastError());
+    // b.b (host) to fix the absolute tolerance
+    sum_partials_kernel<<<1, 64, 0, st>>>(w->part + 4 * w->nwg, w->nwg, w->dscal + 1);
+    EFE_HIP(hipMemcpyAsync(w->hpart, w->dscal + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+    EFE_HIP(hipStreamSynchronize(st));
+    const double bnorm = sqrt(w->hpart[0]);
+    info->rhs_norm = bnorm;
+    info->iterations = 0;
+    info->status = EFE_OK;
+    info->residual = bnorm;
+    const int D = c->D;
+    if (bnorm == 0.0 || !(bnorm == bnorm)) {
+        pcg_finish_kernel<T><<<(D + 255) / 256, 256, 0, st>>>(D, x, b, c->dirflag, c->ndir, xout);
+        EFE_HIP(hipEventRecord(w->evB, st));
+        EFE_HIP(hipStreamWaitEvent(c->stream, w->evB, 0));
+        if (!(bnorm == bnorm)) info->status = EFE_NAN;
+        return EFE_OK;
+    }
+    const double tol = fmax(o->rtol * bnorm, o->atol);
+    const double tol2 = tol * tol;
+    int batch = o->check_every > 0 ? o->check_every : 16;
+    if (batch & 1) batch += 1;
+    const int maxit = o->max_iter > 0 ? o->max_iter : 10000;
+    w->hpart[1] = tol2;
+    EFE_HIP(hipMemcpyAsync(w->dscal, w->hpart + 1, sizeof(double), hipMemcpyHostToDevice, st));
+    // (re)build the captured batch when its shape changed
+    if (!w->graph[gi] || w->graph_batch[gi] != batch || w->graph_block[gi] != w->block) {
+        if (w->graph[gi]) hipGraphExecDestroy(w->graph[gi]);
+        w->graph[gi] = nullptr;
+        hipGraph_t g;
+        EFE_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        for (int j = 0; j < batch; ++j) launch_iter_bs<T>(c, w, st, x, r, z, p, q, K, Minv, j & 1, j);
+        EFE_HIP(hipStreamEndCapture(st, &g));
+        EFE_HIP(hipGraphInstantiate(&w->graph[gi], g, nullptr, nullptr, 0));
+        EFE_HIP(hipGraphDestroy(g));
+        w->graph_batch[gi] = batch;
+        w->graph_block[gi] = w->block;
+    }
+    int it = 0;
+    EFE_HIP(hipEventRecord(w->t0, st));
+    while (true) {
+        EFE_HIP(hipGraphLaunch(w->graph[gi], st));
+        EFE_HIP(hipMemcpyAsync(w->hflags, w->flags, 3 * sizeof(int), hipMemcpyDeviceToHost, st));
+        EFE_HIP(hipStreamSynchronize(st));
+        if (w->hflags[0]) {
+            it += w->hflags[2];
+            if (w->hflags[1]) info->status = EFE_NAN;
+            break;
+        }
+        it += batch;
+        if (it >= maxit) {
+            info->status = EFE_NOT_CONVERGED;
+            break;
+        }
+    }
+    EFE_HIP(hipEventRecord(w->t1, st));
+    pcg_finish_kernel<T><<<(D + 255) / 256, 256, 0, st>>>(D, x, b, c->dirflag, c->ndir, xout);
+    EFE_HIP(hipGetLastError());
+    EFE_HIP(hipEventRecord(w->evB, st));
+    EFE_HIP(hipStreamWaitEvent(c->stream, w->evB, 0));
+    // final residual of the recursion
+    sum_partials_kernel<<<1, 64, 0, st>>>(w->part + 3 * w->nwg, w->nwg, w->dscal + 1);
+    EFE_HIP(hipMemcpyAsync(w->hpart, w->dscal + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+    EFE_HIP(hipStreamSynchronize(st));
+    info->residual = sqrt(w->hpart[0]);
+    info->iterations = it;
+    float ms = 0;
+    hipEventElapsedTime(&ms, w->t0, w->t1);
+    c->t_kry = it > 0 ? ms / it : 0.0;
+    return EFE_OK;
+}
+
+}  // namespace
+
+int efe_krylov_solve(efe_ctx* c, const double* b, double* x, const efe_solve_opts* o, efe_solve_info* info) {
+    if (c->Kprec < 0) {
+        efe_set_error("no assembled operator: call efe_compute_elements first");
+        return EFE_ERR_STATE;
+    }
+    if (o->precision != c->Kprec) {
+        efe_set_error("solve precision differs from the precision K was assembled in");
+        return EFE_ERR_ARG;
+    }
+    if (o->solver != EFE_SOLVER_CG || o->precond != EFE_PRECOND_BLOCK_JACOBI) {
+        efe_set_error("solver/preconditioner combination not available for this context");
+        return EFE_ERR_ARG;
+    }
+    int rc = ensure_work(c);
+    if (rc != EFE_OK) return rc;
+    KrylovWork* w = c->kw;
+    if (o->precision == EFE_FP32)
+        return pcg_solve<float>(c, b, x, o, info, w->x32, w->r32, w->z32, w->p32, w->q32, c->K32, c->Minv32, 1);
+    return pcg_solve<double>(c, b, x, o, info, w->x64, w->r64, w->z64, w->p64, w->q64, c->K64, c->Minv64, 0);
+}
+
+int efe_launch_spmv64(efe_ctx* c, const double* x, double* y) {
+    if (c->Kprec != EFE_FP64) {
+        efe_set_error("efe_spmv needs an FP64 operator");
+        return EFE_ERR_STATE;
+    }
+    spmv_kernel<double, 256><<<(c->N + 255) / 256, 256, 0, c->stream>>>(c->N, c->slice_off, c->scol, c->K64, x, y,
+                                                                         nullptr, nullptr);
+    EFE_HIP(hipGetLastError());
+    return EFE_OK;
+}
+
+int efe_launch_norms(efe_ctx* c, const double* R, const double* ddU, const double* F, double* out) {
+    const int BS = 256;
+    int nwg = (c->D + BS - 1) / BS;
+    if (nwg > 1024) nwg = 1024;
+    norms_stage1<BS><<<nwg, BS, 0, c->stream>>>(c->D, R, ddU, F, c->red + 8, nwg);
+    norms_stage2<<<1, 1, 0, c->stream>>>(c->red + 8, nwg, c->red);
+    EFE_HIP(hipGetLastError());
+    EFE_HIP(hipMemcpyAsync(out, c->red, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    EFE_HIP(hipStreamSynchronize(c->stream));
+    return EFE_OK;
+}
+
+int efe_launch_norms_batched(efe_ctx* c, const double* R, const double* ddU, const double* F, double* out) {
+    (void)c; (void)R; (void)ddU; (void)F; (void)out;
+    efe_set_error("batched norms not available for this context");
+    return EFE_ERR_STATE;
+}
+
+void efe_krylov_free(efe_ctx* c) {
+    KrylovWork* w = c->kw;
+    if (!w) return;
+    for (int i = 0; i < 2; ++i)
+        if (w->graph[i]) hipGraphExecDestroy(w->graph[i]);
+    hipFree(w->x64); hipFree(w->r64); hipFree(w->z64); hipFree(w->p64); hipFree(w->q64);
+    hipFree(w->x32); hipFree(w->r32); hipFree(w->z32); hipFree(w->p32); hipFree(w->q32);
+    hipFree(w->part); hipFree(w->flags); hipFree(w->dscal);
+    hipHostFree(w->hflags); hipHostFree(w->hpart);
+    if (w->ks) hipStreamDestroy(w->ks);
+    if (w->evA) hipEventDestroy(w->evA);
+    if (w->evB) hipEventDestroy(w->evB);
+    if (w->t0) hipEventDestroy(w->t0);
+    if (w->t1) hipEventDestroy(w->t1);
+    delete w;
+    c->kw = nullptr;
+}

@@ -1,0 +1,117 @@
+// Host-side construction of the system-matrix pattern and gather maps.
+//
+// The CSR pattern is that of edelweissfe/numerics/csrgenerator.pyx:68-77: SciPy's
+// canonical csr_matrix of the VIJ pairs (dofmanager.py:543-555), i.e. for DOF row
+// 3n+i every column 3m+j of every node m sharing an element with n, ascending.
+// Because each element couples all DOFs of all its nodes, that pattern is the
+// node adjacency expanded by 3x3, so it is stored as node-block rows (bptr/bcol)
+// and expanded to CSR only on request.
+//
+// The element->CSR map replaces csrgenerator.pyx:81-98: instead of a COO-pair ->
+// CSR-slot index (a scatter, which needs atomics or a serial loop on a GPU), every
+// 3x3 block slot lists the element stiffness blocks that land on it, in element
+// order (the order in which updateCSR adds them, csrgenerator.pyx:112-113), so the
+// device gathers and sums each slot once, deterministically.
+#include <algorithm>
+#include <cstring>
+
+#include "efe_internal.h"
+
+bool efe_build_pattern(HostPattern& hp, int N, int E, int NN, const int32_t* conn, std::string& err) {
+    hp.N = N;
+    hp.E = E;
+    hp.NN = NN;
+    for (int64_t i = 0; i < (int64_t)E * NN; ++i) {
+        if (conn[i] < 0 || conn[i] >= N) {
+            err = "connectivity references a node outside [0, n_nodes)";
+            return false;
+        }
+    }
+    // node -> (element, local node) incidences, element order
+    hp.nptr.assign(N + 1, 0);
+    for (int64_t i = 0; i < (int64_t)E * NN; ++i) hp.nptr[conn[i] + 1]++;
+    for (int n = 0; n < N; ++n) hp.nptr[n + 1] += hp.nptr[n];
+    hp.nlist.resize((size_t)E * NN);
+    {
+        std::vector<int32_t> fill(hp.nptr.begin(), hp.nptr.end() - 1);
+        for (int e = 0; e < E; ++e)
+            for (int a = 0; a < NN; ++a) hp.nlist[fill[conn[(int64_t)e * NN + a]]++] = e * NN + a;
+    }
+    // node adjacency
+    hp.bptr.assign(N + 1, 0);
+    std::vector<std::vector<int32_t>> rows(N);
+    for (int n = 0; n < N; ++n) {
+        auto& r = rows[n];
+        r.reserve(27);
+        for (int32_t k = hp.nptr[n]; k < hp.nptr[n + 1]; ++k) {
+            int e = hp.nlist[k] / NN;
+            for (int b = 0; b < NN; ++b) r.push_back(conn[(int64_t)e * NN + b]);
+        }
+        std::sort(r.begin(), r.end());
+        r.erase(std::unique(r.begin(), r.end()), r.end());
+        if (r.empty()) r.push_back(n);  // a node outside every element keeps its diagonal
+        hp.bptr[n + 1] = hp.bptr[n] + (int32_t)r.size();
+    }
+    int64_t nb = hp.bptr[N];
+    if (nb * 9 > INT32_MAX) {
+        err = "CSR nnz exceeds the int32 index range of csrgenerator.pyx:65";
+        return false;
+    }
+    hp.bcol.resize(nb);
+    hp.diagk.resize(N);
+    for (int n = 0; n < N; ++n) {
+        std::copy(rows[n].begin(), rows[n].end(), hp.bcol.begin() + hp.bptr[n]);
+        auto it = std::lower_bound(rows[n].begin(), rows[n].end(), n);
+        hp.diagk[n] = (it != rows[n].end() && *it == n) ? (int32_t)(it - rows[n].begin()) : -1;
+        std::vector<int32_t>().swap(rows[n]);
+    }
+    // contributions per block slot, element order
+    std::vector<int32_t> slot((size_t)E * NN * NN);
+    hp.coff.assign(nb + 1, 0);
+    for (int e = 0; e < E; ++e) {
+        const int32_t* ce = conn + (int64_t)e * NN;
+        for (int a = 0; a < NN; ++a) {
+            int n = ce[a];
+            const int32_t* b0 = hp.bcol.data() + hp.bptr[n];
+            const int32_t* b1 = hp.bcol.data() + hp.bptr[n + 1];
+            for (int b = 0; b < NN; ++b) {
+                const int32_t* it = std::lower_bound(b0, b1, ce[b]);
+                int32_t s = (int32_t)(it - hp.bcol.data());
+                slot[((size_t)e * NN + a) * NN + b] = s;
+                hp.coff[s + 1]++;
+            }
+        }
+    }
+    for (int64_t s = 0; s < nb; ++s) hp.coff[s + 1] += hp.coff[s];
+    hp.clist.resize((size_t)E * NN * NN);
+    {
+        std::vector<int32_t> fill(hp.coff.begin(), hp.coff.end() - 1);
+        for (size_t k = 0; k < slot.size(); ++k) hp.clist[fill[slot[k]]++] = (int32_t)k;
+    }
+    // sliced-ELL layout of the block rows
+    int ns = (N + kSlice - 1) / kSlice;
+    hp.slice_off.assign(ns + 1, 0);
+    hp.slice_deg.assign(ns, 0);
+    for (int s = 0; s < ns; ++s) {
+        int d = 0;
+        for (int l = 0; l < kSlice; ++l) {
+            int n = s * kSlice + l;
+            if (n < N) d = std::max(d, hp.bptr[n + 1] - hp.bptr[n]);
+        }
+        hp.slice_deg[s] = d;
+        hp.slice_off[s + 1] = hp.slice_off[s] + d * kSlice;
+    }
+    hp.padded_blocks = hp.slice_off[ns];
+    hp.scol.assign(hp.padded_blocks, 0);
+    for (int s = 0; s < ns; ++s) {
+        for (int l = 0; l < kSlice; ++l) {
+            int n = s * kSlice + l;
+            for (int k = 0; k < hp.slice_deg[s]; ++k) {
+                int32_t col = 0;
+                if (n < N) col = (k < hp.bptr[n + 1] - hp.bptr[n]) ? hp.bcol[hp.bptr[n] + k] : n;
+                hp.scol[hp.slice_off[s] + (int64_t)k * kSlice + l] = col;
+            }
+        }
+    }
+    return true;
+}

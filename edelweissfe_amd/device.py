@@ -1,0 +1,148 @@
+"""Device-side system of one homogeneous element set: a thin owner of an efe_ctx.
+
+Vectors are torch float64 tensors on the GPU; they are passed to the library
+by address and all work is ordered on torch's current stream.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check
+
+
+class DeviceSystem:
+    def __init__(self, coords, conn, el_type, material, props, specimen_of_node=None, n_specimens=1, device=None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("DeviceSystem needs a GPU (torch.cuda.is_available() is False)")
+        self.lib = _lib.load()
+        self.device = torch.device(device or "cuda")
+        coords = np.ascontiguousarray(coords, dtype=np.float64)
+        conn = np.ascontiguousarray(conn, dtype=np.int32)
+        props = np.ascontiguousarray(props, dtype=np.float64)
+        self.el_type = el_type.upper()
+        self.material = material.lower()
+        self.n_nodes = coords.shape[0]
+        self.n_elements = conn.shape[0]
+        if self.el_type not in _lib.ELEMENT_TYPES:
+            raise ValueError("element type {} not supported (C3D8, C3D20, C3D20R)".format(el_type))
+        if self.material not in _lib.MATERIALS:
+            raise ValueError("material {} not supported".format(material))
+        nn = 8 if self.el_type == "C3D8" else 20
+        if conn.shape[1] != nn:
+            raise ValueError("connectivity has {} nodes per element, {} needs {}".format(conn.shape[1], el_type, nn))
+        if conn.min() < 0 or conn.max() >= self.n_nodes:
+            raise ValueError("connectivity references nodes outside [0, n_nodes)")
+        per_el = props.ndim == 2
+        if per_el and props.shape[0] != self.n_elements:
+            raise ValueError("per-element properties need one row per element")
+        spec = None
+        if specimen_of_node is not None:
+            spec = np.ascontiguousarray(specimen_of_node, dtype=np.int32)
+        self._keep = (coords, conn, props, spec)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        desc = _lib.MeshDesc(
+            n_nodes=self.n_nodes, n_elements=self.n_elements, element_type=_lib.ELEMENT_TYPES[self.el_type],
+            material=_lib.MATERIALS[self.material], coords=coords.ctypes.data, conn=conn.ctypes.data,
+            n_props=props.shape[-1], props_per_element=1 if per_el else 0, props=props.ctypes.data,
+            specimen_of_node=spec.ctypes.data if spec is not None else None, n_specimens=n_specimens,
+            stream=stream)
+        self.ctx = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            check(self.lib.efe_ctx_create(ctypes.byref(desc), ctypes.byref(self.ctx)), "efe_ctx_create")
+        sizes = (ctypes.c_int64 * 4)()
+        check(self.lib.efe_get_sizes(self.ctx, sizes), "efe_get_sizes")
+        self.n_dof, self.nnz, self.n_qp, self.n_state = (int(v) for v in sizes)
+
+    # -- helpers
+    def _ptr(self, t):
+        if t is None:
+            return None
+        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.numel() == self.n_dof):
+            raise ValueError("expected a contiguous float64 CUDA tensor of length n_dof")
+        return ctypes.c_void_p(t.data_ptr())
+
+    def zeros(self):
+        return torch.zeros(self.n_dof, dtype=torch.float64, device=self.device)
+
+    def close(self):
+        if getattr(self, "ctx", None) is not None and self.ctx.value:
+            self.lib.efe_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- entry points
+    def csr_pattern(self):
+        indptr = np.empty(self.n_dof + 1, dtype=np.int32)
+        indices = np.empty(self.nnz, dtype=np.int32)
+        check(self.lib.efe_get_csr_pattern(self.ctx, indptr.ctypes.data, indices.ctypes.data), "efe_get_csr_pattern")
+        return indptr, indices
+
+    def set_dirichlet(self, dofs):
+        dofs = np.ascontiguousarray(np.unique(np.asarray(dofs, dtype=np.int64)).astype(np.int32))
+        check(self.lib.efe_set_dirichlet(self.ctx, dofs.ctypes.data, int(dofs.size)), "efe_set_dirichlet")
+
+    def compute_elements(self, dU, P, F, dir_values=None, dir_correction=None, precision=_lib.EFE_FP64, time=(0.0, 0.0),
+                         dT=0.0, U=None):
+        """Returns the status (EFE_OK or EFE_CUTBACK) and pNewDT."""
+        t = (ctypes.c_double * 2)(*time)
+        pnew = ctypes.c_double(1.0)
+        rc = self.lib.efe_compute_elements(self.ctx, self._ptr(U) if U is not None else None, self._ptr(dU), t,
+                                           float(dT), self._ptr(P), self._ptr(F), self._ptr(dir_values),
+                                           self._ptr(dir_correction), int(precision), ctypes.byref(pnew))
+        check(rc, "efe_compute_elements")
+        return rc, pnew.value
+
+    def body_force(self, force, PExt):
+        f = (ctypes.c_double * 3)(*[float(v) for v in force])
+        check(self.lib.efe_body_force(self.ctx, f, self._ptr(PExt)), "efe_body_force")
+
+    def export_csr_values(self):
+        out = torch.empty(self.nnz, dtype=torch.float64, device=self.device)
+        check(self.lib.efe_export_csr_values(self.ctx, ctypes.c_void_p(out.data_ptr())), "efe_export_csr_values")
+        return out
+
+    def spmv(self, x, y):
+        check(self.lib.efe_spmv(self.ctx, self._ptr(x), self._ptr(y)), "efe_spmv")
+
+    def linsolve(self, b, x, solver=_lib.EFE_SOLVER_CG, precond=_lib.EFE_PRECOND_BLOCK_JACOBI,
+                 precision=_lib.EFE_FP64, rtol=1e-10, atol=0.0, max_iter=20000, restart=50, check_every=32):
+        o = _lib.SolveOpts(solver=solver, precond=precond, precision=precision, max_iter=max_iter, restart=restart,
+                           check_every=check_every, rtol=rtol, atol=atol)
+        info = _lib.SolveInfo()
+        check(self.lib.efe_linsolve(self.ctx, self._ptr(b), self._ptr(x), ctypes.byref(o), ctypes.byref(info)),
+              "efe_linsolve")
+        return info
+
+    def norms(self, R, ddU, F):
+        out = (ctypes.c_double * 5)()
+        check(self.lib.efe_convergence_norms(self.ctx, self._ptr(R), self._ptr(ddU) if ddU is not None else None,
+                                             self._ptr(F), out), "efe_convergence_norms")
+        return np.array(out[:])
+
+    def accept_state(self):
+        check(self.lib.efe_accept_state(self.ctx), "efe_accept_state")
+
+    def qp_state(self, which=0):
+        out = np.empty((self.n_elements, self.n_qp, self.n_state))
+        check(self.lib.efe_get_qp_state(self.ctx, int(which), out.ctypes.data), "efe_get_qp_state")
+        return out
+
+    def set_qp_state(self, state):
+        state = np.ascontiguousarray(state, dtype=np.float64)
+        check(self.lib.efe_set_qp_state(self.ctx, state.ctypes.data), "efe_set_qp_state")
+
+    def set_launch(self, elements_per_workgroup=0, krylov_block=0):
+        check(self.lib.efe_set_launch(self.ctx, int(elements_per_workgroup), int(krylov_block)), "efe_set_launch")
+
+    def phase_times(self):
+        out = (ctypes.c_double * 4)()
+        check(self.lib.efe_get_phase_times(self.ctx, out), "efe_get_phase_times")
+        return {"elements": out[0], "assembly": out[1], "krylov_iteration": out[2], "norms": out[3]}

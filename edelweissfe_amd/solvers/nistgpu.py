@@ -1,0 +1,287 @@
+"""NISTGPU: the nonlinear implicit static Newton step on one MI355X.
+
+Drop-in twin of the reference solver plugin ``NIST``
+(edelweissfe/solvers/nonlinearimplicitstatic.py:60-458): same option names and
+defaults (NISTOptions, :73-80, plus the device options below), same
+``solveStep(step, model, fieldOutputController, outputmanagers)`` entry point,
+same increment / Newton control flow and exceptions:
+
+* zero increment first, linear extrapolation of dU from the previous
+  increment (:947-986);
+* iteration 0 of a non-extrapolated increment with Dirichlet BCs: R[dir] = delta
+  and an unconditional solve (:428-430); afterwards R[dir] = 0 and the field-wise
+  convergence test ||ddU||_inf < 1e-8 and max|R| <= max(1e-8 * avgFlux, 1e-7),
+  5e-3 from iteration 15 on (:625-725, phenomena.py:59-94);
+* divergence after maxGrowIter residual growths, ReachedMaxIterations at
+  maxIter, CutbackRequest(0.5) from a failing material, DivergingSolution on NaN
+  (:247-279, :443-449, :748-749);
+* state accepted only on convergence (:296).
+
+Every per-DOF and per-element operation runs on the GPU through libefe_gpu
+(elements + assembly, Dirichlet elimination, norms, Krylov solve); only the
+scalar control decisions are taken on the host, after one readback of the
+convergence quantities per Newton iteration.
+
+Device options (in addition to NISTOptions):
+  precision       "fp64" (default) or "mixed": FP32 stiffness contractions,
+                  FP32 Krylov inner solve, FP64 material update, residual and
+                  Newton update (the Newton loop is the FP64 refinement);
+  linsolver       "pcg" (default, block-Jacobi PCG);
+  krylovAtolFactor the Krylov solve stops once ||r||_2 <= krylovAtolFactor times the
+                  Newton flux threshold max(tol * avgFlux, 1e-7) of the current
+                  iteration (default 1e-2), so the next convergence check sees the
+                  same residual class a direct solve would leave;
+  krylovRtol      relative floor of the FP64 solve (default 1e-13);
+  krylovRtolMixed relative residual of the FP32 inner solve (default 1e-5);
+  krylovMaxIter, krylovCheckEvery.
+"""
+
+import time as _time
+
+import numpy as np
+import torch
+
+from .. import _lib
+from ..device import DeviceSystem
+from ..exceptions import (CutbackRequest, DivergingSolution, ReachedMaxIncrements, ReachedMaxIterations,
+                          ReachedMinIncrementSize, StepFailed)
+
+FIELD_CORRECTION_TOL = 1e-8      # phenomena.py fieldCorrectionTolerance["displacement"]
+FLUX_RESIDUAL_TOL = 1e-8         # phenomena.py fluxResidualTolerance["displacement"]
+FLUX_RESIDUAL_TOL_ALT = 5e-3     # phenomena.py fluxResidualToleranceAlternative["displacement"]
+
+
+class NISTGPU:
+    identification = "NISTGPUSolver"
+
+    NISTOptions = {
+        "defaultMaxIter": 10,
+        "defaultCriticalIter": 5,
+        "defaultMaxGrowingIter": 10,
+        "extrapolation": "linear",
+        "linsolver": "pcg",
+        "linsolverConfigFile": "",
+        "precision": "fp64",
+        "krylovAtolFactor": 1e-2,
+        "krylovRtol": 1e-13,
+        "krylovRtolMixed": 1e-5,
+        "krylovMaxIter": 50000,
+        "krylovCheckEvery": 32,
+    }
+
+    def __init__(self, jobInfo=None, journal=None, **kwargs):
+        self.journal = journal
+        self.options = self.NISTOptions.copy()
+        self._updateOptions(kwargs)
+        self.system = None
+        self.history = []          # per increment: dict(inc, stepProgress, iters, converged, krylov)
+        self.computationTimes = {k: 0.0 for k in ("elements", "linear solve", "convergence check", "loads")}
+        self.krylovIterations = 0
+        self.linearSolves = 0
+
+    def _updateOptions(self, updated):
+        for k, v in updated.items():
+            if k not in self.NISTOptions:
+                raise AttributeError("Invalid option {:} for {:}".format(k, self.identification))
+            self.options[k] = type(self.NISTOptions[k])(v)
+
+    # ---- setup ----------------------------------------------------------------------------------------
+    def _setupSystem(self, model):
+        mat = model.material
+        if mat is None:
+            raise ValueError("model has no material")
+        if self.system is None or self.system.n_nodes != model.nNodes:
+            self.system = DeviceSystem(model.coordinates, model.connectivity, model.elementType, mat["name"],
+                                       mat["properties"])
+            if model.qpState is not None:
+                self.system.set_qp_state(model.qpState)
+        return self.system
+
+    @property
+    def _mixed(self):
+        return self.options["precision"].lower() == "mixed"
+
+    # ---- public interface (nonlinearimplicitstatic.py:110) --------------------------------------------------
+    def solveStep(self, step, model, fieldOutputController=None, outputmanagers=()):
+        opts = step.actions.get("options", {}).get("NISTSolver", None)
+        if opts:
+            self._updateOptions(opts)
+        sysm = self._setupSystem(model)
+        dev = sysm.device
+        D = sysm.n_dof
+        dirichlets = list(step.actions["dirichlet"].values())
+        self._dirIdx = [d.indices(model) for d in dirichlets]
+        allDir = np.concatenate(self._dirIdx) if dirichlets else np.zeros(0, dtype=np.int64)
+        sysm.set_dirichlet(allDir)
+        self._dirAll = torch.as_tensor(allDir, dtype=torch.int64, device=dev)
+        self._dirIdxT = [torch.as_tensor(ix, dtype=torch.int64, device=dev) for ix in self._dirIdx]
+        self._nAccFlux = model.nElements * (8 if model.elementType == "C3D8" else 20) * 3
+
+        U = torch.as_tensor(model.U, dtype=torch.float64, device=dev).clone()
+        dU = torch.zeros(D, dtype=torch.float64, device=dev)
+        P = torch.zeros(D, dtype=torch.float64, device=dev)
+        prevTimeStep = None
+        maxIter, criticalIter = step.maxIter, step.criticalIter
+        maxGrowingIter, cutbackFactor = step.maxGrowIter, step.cutbackFactor
+        try:
+            for timeStep in step.getTimeStep():
+                record = {"inc": timeStep.number, "stepProgress": timeStep.stepProgress, "iters": None,
+                          "converged": False, "krylov": []}
+                try:
+                    U_new, dU, P, iters = self.solveIncrement(U, dU, P, step, model, timeStep, prevTimeStep,
+                                                              self.options["extrapolation"], maxIter,
+                                                              maxGrowingIter, record)
+                except CutbackRequest as e:
+                    step.discardAndChangeIncrement(max(e.cutbackSize, cutbackFactor))
+                    prevTimeStep = None
+                    record["iters"] = np.inf
+                    record["notes"] = str(e)
+                except (ReachedMaxIterations, DivergingSolution) as e:
+                    step.discardAndChangeIncrement(cutbackFactor)
+                    prevTimeStep = None
+                    record["iters"] = np.inf
+                    record["notes"] = str(e)
+                else:
+                    prevTimeStep = timeStep
+                    if iters >= criticalIter:
+                        step.preventIncrementIncrease()
+                    U = U_new
+                    sysm.accept_state()
+                    model.time = timeStep.totalTime
+                    record["iters"] = iters
+                    record["converged"] = True
+                self.history.append(record)
+        except (ReachedMaxIncrements, ReachedMinIncrementSize):
+            raise StepFailed()
+        for group in ("dirichlet", "nodeforces", "bodyforce"):
+            for action in step.actions.get(group, {}).values():
+                action.applyAtStepEnd(model)
+        torch.cuda.synchronize(dev)
+        model.U = U.cpu().numpy()
+        model.P = P.cpu().numpy()
+        model.dU = dU.cpu().numpy()
+        model.qpState = sysm.qp_state(0)
+        return True, model
+
+    # ---- one increment (nonlinearimplicitstatic.py:334-458) ---------------------------------------------------
+    def solveIncrement(self, U_n, dU, P, step, model, timeStep, prevTimeStep, extrapolation, maxIter, maxGrowingIter,
+                       record=None):
+        sysm = self.system
+        dev = sysm.device
+        D = sysm.n_dof
+        dirichlets = list(step.actions["dirichlet"].values())
+        prec = _lib.EFE_FP32 if self._mixed else _lib.EFE_FP64
+        iterationCounter = 0
+        history = (0.0, 0)
+        ddU = None
+        F = torch.zeros(D, dtype=torch.float64, device=dev)
+        R = torch.zeros(D, dtype=torch.float64, device=dev)
+        dirVals = None
+
+        deltas = [torch.as_tensor(d.getDelta(model, timeStep), dtype=torch.float64, device=dev) for d in dirichlets]
+        if extrapolation == "linear" and prevTimeStep is not None and prevTimeStep.timeIncrement:
+            dU = dU * (timeStep.stepProgressIncrement / prevTimeStep.stepProgressIncrement)
+            for ix, dl in zip(self._dirIdxT, deltas):
+                dU[ix] = dl
+            extrapolated = True
+        else:
+            extrapolated = False
+            dU = torch.zeros(D, dtype=torch.float64, device=dev)
+        PExt = self._externalLoads(step, model, timeStep)
+
+        while True:
+            first = iterationCounter == 0 and not extrapolated and bool(dirichlets)
+            if first:
+                dirVals = torch.zeros(D, dtype=torch.float64, device=dev)
+                for ix, dl in zip(self._dirIdxT, deltas):
+                    dirVals[ix] = dl
+                corr = torch.empty(D, dtype=torch.float64, device=dev)
+            else:
+                corr = None
+            tic = _time.perf_counter()
+            st, pnew = sysm.compute_elements(dU, P, F, dir_values=dirVals if first else None, dir_correction=corr,
+                                             precision=prec, time=(timeStep.stepTime, timeStep.totalTime),
+                                             dT=timeStep.timeIncrement)
+            self.computationTimes["elements"] += _time.perf_counter() - tic
+            if st == _lib.EFE_CUTBACK:
+                raise CutbackRequest("Von Mises Newton failed.", pnew)
+            torch.add(P, PExt, out=R)
+            if first:
+                R[self._dirAll] = dirVals[self._dirAll]
+            elif self._dirAll.numel():
+                R[self._dirAll] = 0.0
+            tic = _time.perf_counter()
+            norms = self.system.norms(R, ddU, F)
+            self.computationTimes["convergence check"] += _time.perf_counter() - tic
+            fluxThreshold = self._fluxThreshold(norms, iterationCounter)
+            if not first:
+                converged, history = self._checkConvergence(norms, ddU, iterationCounter, history)
+                if converged:
+                    break
+                if history[1] > maxGrowingIter:
+                    raise DivergingSolution("Residual grew {:} times, cutting back".format(maxGrowingIter))
+                if iterationCounter == maxIter:
+                    raise ReachedMaxIterations("Reached max. iterations in current increment, cutting back")
+            b = R - corr if first else R
+            ddU = torch.empty(D, dtype=torch.float64, device=dev)
+            tic = _time.perf_counter()
+            info = self._linearSolve(b, ddU, prec, fluxThreshold)
+            self.computationTimes["linear solve"] += _time.perf_counter() - tic
+            if record is not None:
+                record["krylov"].append(int(info.iterations))
+            if info.status == _lib.EFE_NAN:
+                raise DivergingSolution("Obtained NaN in linear solve")
+            if info.status == _lib.EFE_NOT_CONVERGED:
+                raise DivergingSolution("Krylov solver did not converge")
+            dU = dU + ddU
+            iterationCounter += 1
+        return U_n + dU, dU, P, iterationCounter
+
+    def _linearSolve(self, b, x, prec, fluxThreshold):
+        o = self.options
+        rtol = o["krylovRtolMixed"] if prec == _lib.EFE_FP32 else o["krylovRtol"]
+        if o["linsolver"].lower() != "pcg":
+            raise AttributeError("invalid linear solver {:} requested".format(o["linsolver"]))
+        info = self.system.linsolve(b, x, solver=_lib.EFE_SOLVER_CG, precond=_lib.EFE_PRECOND_BLOCK_JACOBI,
+                                    precision=prec, rtol=rtol, atol=o["krylovAtolFactor"] * fluxThreshold,
+                                    max_iter=o["krylovMaxIter"], check_every=o["krylovCheckEvery"])
+        self.krylovIterations += info.iterations
+        self.linearSolves += 1
+        return info
+
+    def _externalLoads(self, step, model, timeStep):
+        """assembleLoads (nonlinearimplicitstatic.py:903-945): node forces and body forces."""
+        sysm = self.system
+        PExt = torch.zeros(sysm.n_dof, dtype=torch.float64, device=sysm.device)
+        for nf in step.actions.get("nodeforces", {}).values():
+            nodes = model.nodeSets[nf.nSet]
+            idx = torch.as_tensor(model.dofsOfNodeSet(nf.nSet), device=sysm.device)
+            load = torch.as_tensor(np.tile(nf.getCurrentLoad(timeStep), (len(nodes), 1)).ravel(),
+                                   dtype=torch.float64, device=sysm.device)
+            PExt[idx] += load
+        for bf in step.actions.get("bodyforce", {}).values():
+            if len(model.elementSets[bf.elSet]) != model.nElements:
+                raise NotImplementedError("body forces on element subsets")
+            sysm.body_force(bf.getCurrentLoad(timeStep), PExt)
+        return PExt
+
+    def _fluxThreshold(self, norms, iterationCounter):
+        """max(tol * avgFlux, 1e-7) with avgFlux = max(1e-10, ||F||_1 / nAccumulatedFluxes)
+        (nonlinearimplicitstatic.py:666-682, :786-790)."""
+        avgFlux = max(1e-10, norms[3] / self._nAccFlux)
+        tolFlux = FLUX_RESIDUAL_TOL if iterationCounter < 15 else FLUX_RESIDUAL_TOL_ALT
+        return max(tolFlux * avgFlux, 1e-7)
+
+    def _checkConvergence(self, n, ddU, iterationCounter, history):
+        """checkConvergence for the displacement field (nonlinearimplicitstatic.py:625-725)."""
+        fluxResidual, correction, nan = n[0], n[2], n[4]
+        if nan:
+            raise DivergingSolution("Obtained NaN in the residual")
+        if ddU is None:
+            correction = 0.0
+        convergedCorrection = correction < FIELD_CORRECTION_TOL
+        convergedFlux = fluxResidual <= self._fluxThreshold(n, iterationCounter)
+        prev, nGrew = history
+        if fluxResidual > prev:
+            nGrew += 1
+        return convergedCorrection and convergedFlux, (fluxResidual, nGrew)

@@ -1,0 +1,84 @@
+"""Step actions of the hot path, same meaning as the reference's:
+
+* Dirichlet   edelweissfe/stepactions/dirichlet.py:53-158: prescribed values on
+              components of a node set; the increment of an increment is
+              value * (amp(p) - amp(p - dp)); inactive (zero) after the step.
+* NodeForces  edelweissfe/stepactions/nodeforces.py:53-152: the current load is
+              start + delta * amp(p).
+* BodyForce   edelweissfe/stepactions/bodyforce.py: force per volume on an element
+              set, start + delta * amp(p).
+amp(t) = t unless an amplitude callable is given (the reference's f(t)).
+"""
+
+import numpy as np
+
+
+def _identity(t):
+    return t
+
+
+class Dirichlet:
+    def __init__(self, name, nSet, values, amplitude=None):
+        """values: {component index (0..2): prescribed value over the step}."""
+        self.name = name
+        self.nSet = nSet
+        self.components = sorted(values)
+        self.values = np.array([float(values[c]) for c in self.components])
+        self.amplitude = amplitude or _identity
+        self.active = True
+
+    def indices(self, model):
+        nodes = np.asarray(model.nodeSets[self.nSet])
+        return (3 * nodes[:, None] + np.asarray(self.components)[None, :]).ravel()
+
+    def getDelta(self, model, timeStep):
+        n = len(model.nodeSets[self.nSet])
+        delta = np.tile(self.values, (n, 1))
+        if not self.active:
+            return (delta * 0.0).ravel()
+        a = self.amplitude
+        return (delta * (a(timeStep.stepProgress) - a(timeStep.stepProgress - timeStep.stepProgressIncrement))).ravel()
+
+    def applyAtStepEnd(self, model):
+        self.active = False
+
+
+class NodeForces:
+    def __init__(self, name, nSet, vector, amplitude=None):
+        self.name = name
+        self.nSet = nSet
+        self.delta = np.asarray(vector, dtype=float)
+        self.start = np.zeros(3)
+        self.amplitude = amplitude or _identity
+        self.idle = False
+
+    def getCurrentLoad(self, timeStep):
+        if self.idle:
+            return self.start
+        return self.start + self.delta * self.amplitude(timeStep.stepProgress)
+
+    def applyAtStepEnd(self, model):
+        if not self.idle:
+            self.start = self.start + self.delta * self.amplitude(1.0)
+            self.delta = np.zeros(3)
+            self.idle = True
+
+
+class BodyForce:
+    def __init__(self, name, elSet, vector, amplitude=None):
+        self.name = name
+        self.elSet = elSet
+        self.delta = np.asarray(vector, dtype=float)
+        self.start = np.zeros(3)
+        self.amplitude = amplitude or _identity
+        self.idle = False
+
+    def getCurrentLoad(self, timeStep):
+        t = 1.0 if self.idle else timeStep.stepProgress
+        return self.start + self.delta * self.amplitude(t)
+
+    def applyAtStepEnd(self, model):
+        if not self.idle:
+            self.start = self.start + self.delta * self.amplitude(1.0)
+            self.delta = np.zeros(3)
+            self.idle = True

@@ -1,0 +1,179 @@
+/*
+ * efe_gpu.h -- C ABI of the MI355X (gfx950) implementation of EdelweissFE's
+ * nonlinear implicit static (NIST) Newton step.
+ *
+ * Every pointer marked "device" is a device address on the context's GPU
+ * (for instance a torch tensor's data_ptr()); every pointer marked "host" is
+ * ordinary host memory.  All work is enqueued on the context's stream; calls
+ * that return numbers to the host (norms, status, iteration counts)
+ * synchronise that stream.  A context is not thread safe, like the
+ * single-threaded NIST solver it serves.
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout):
+ *   element loop     edelweissfe/solvers/nonlinearimplicitstatic.py:794-849 computeElements,
+ *                    edelweissfe/solvers/nonlinearimplicitstaticparallelmk2.pyx:112-198,
+ *                    native element ABI edelweissfe/elements/marmotelement/element.pxd:97-103
+ *   material update  edelweissfe/materials/base/basehypoelasticmaterial.py:133-154 computeStress
+ *   DOF/VIJ/CSR      edelweissfe/numerics/dofmanager.py:445-557, edelweissfe/numerics/csrgenerator.pyx:47-115
+ *   Dirichlet on K   edelweissfe/solvers/nonlinearimplicitstatic.py:559-593 applyDirichletK
+ *   linear solve     edelweissfe/solvers/nonlinearimplicitstatic.py:727-751 linearSolve,
+ *                    edelweissfe/config/linsolve.py:46-86 getLinSolverByName
+ *   convergence      edelweissfe/solvers/nonlinearimplicitstatic.py:625-725,771-792
+ *   state accept     edelweissfe/models/femodel.py:245-258, elements/displacementelement/element.py:373-379
+ */
+#ifndef EFE_GPU_H
+#define EFE_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define EFE_OK 0
+#define EFE_CUTBACK 1          /* a material asked for a smaller increment (CutbackRequest, vonmises.py:230-231) */
+#define EFE_NAN 2              /* NaN in a result (DivergingSolution, nonlinearimplicitstatic.py:748-749) */
+#define EFE_NOT_CONVERGED 3    /* Krylov solver hit its iteration limit */
+#define EFE_ERR_ARG -1
+#define EFE_ERR_HIP -2
+#define EFE_ERR_STATE -3
+
+/* element types (elements/library.py:212-291) */
+#define EFE_C3D8 0    /* hexa8, 2x2x2 Gauss */
+#define EFE_C3D20 1   /* hexa20, 3x3x3 Gauss */
+#define EFE_C3D20R 2  /* hexa20, 2x2x2 Gauss */
+
+/* materials */
+#define EFE_LINEAR_ELASTIC 0 /* props [E, nu]                          linearelastic.py:59-61 */
+#define EFE_VONMISES 1       /* props [E, nu, fy0, HLin, dfy, delta]   vonmises.py:105-113 */
+#define EFE_DRUCKER_PRAGER 2 /* props [E, nu, phi_deg, psi_deg, c0, H] (no reference counterpart) */
+
+/* precision of the stiffness contractions, the stored K and the Krylov solve */
+#define EFE_FP64 0
+#define EFE_FP32 1
+
+#define EFE_SOLVER_CG 0
+#define EFE_SOLVER_GMRES 1
+#define EFE_PRECOND_BLOCK_JACOBI 0
+#define EFE_PRECOND_ILU0 1
+
+typedef struct efe_ctx efe_ctx;
+
+typedef struct {
+    int32_t n_nodes;
+    int32_t n_elements;
+    int32_t element_type;       /* EFE_C3D8 | EFE_C3D20 | EFE_C3D20R */
+    int32_t material;           /* EFE_LINEAR_ELASTIC | EFE_VONMISES | EFE_DRUCKER_PRAGER */
+    const double* coords;       /* host [n_nodes][3] */
+    const int32_t* conn;        /* host [n_elements][nodes_per_element], node positions, boxGen node order */
+    int32_t n_props;            /* 2 (linear elastic) or 6 */
+    int32_t props_per_element;  /* 0: props is [n_props]; 1: props is [n_elements][n_props] */
+    const double* props;        /* host */
+    const int32_t* specimen_of_node; /* host [n_nodes] or NULL: independent specimens (block-diagonal system) */
+    int32_t n_specimens;        /* 1 when specimen_of_node is NULL */
+    void* stream;               /* hipStream_t or NULL for the default stream */
+} efe_mesh_desc;
+
+typedef struct {
+    int32_t solver;            /* EFE_SOLVER_CG | EFE_SOLVER_GMRES */
+    int32_t precond;           /* EFE_PRECOND_BLOCK_JACOBI | EFE_PRECOND_ILU0 */
+    int32_t precision;         /* EFE_FP64 | EFE_FP32 (must match the precision K was assembled in) */
+    int32_t max_iter;
+    int32_t restart;           /* GMRES restart length */
+    int32_t check_every;       /* Krylov iterations between convergence readbacks */
+    double rtol;               /* ||r|| <= rtol ||b|| */
+    double atol;
+} efe_solve_opts;
+
+typedef struct {
+    int32_t iterations;
+    int32_t status;            /* EFE_OK | EFE_NOT_CONVERGED | EFE_NAN */
+    double residual;           /* final ||r||_2 of the solver's own recursion */
+    double rhs_norm;
+} efe_solve_info;
+
+/* last error text of the calling thread */
+const char* efe_last_error(void);
+
+/* Builds the DOF map (3 p + c), the bit-exact CSR pattern of csrgenerator.pyx:68-77,
+ * the element->CSR gather map, and allocates the QP state (dofmanager.py:153-218,
+ * csrgenerator.pyx:47-98, element.py:207-240). */
+int efe_ctx_create(const efe_mesh_desc* desc, efe_ctx** out);
+void efe_ctx_destroy(efe_ctx* ctx);
+
+/* sizes: [n_dof, csr_nnz, n_qp_per_element, n_state_per_qp] (host out) */
+int efe_get_sizes(efe_ctx* ctx, int64_t out[4]);
+
+/* CSR pattern, identical to SciPy's canonical csr_matrix((0,(I,J))) (csrgenerator.pyx:68-77):
+ * indptr host [n_dof+1], indices host [csr_nnz]. */
+int efe_get_csr_pattern(efe_ctx* ctx, int32_t* indptr, int32_t* indices);
+
+/* Marks the constrained DOFs of the current step (nonlinearimplicitstatic.py:1077-1084).
+ * dofs host [n]; replaces the previous set.  Subsequent assemblies eliminate these
+ * rows and columns symmetrically with a unit diagonal. */
+int efe_set_dirichlet(efe_ctx* ctx, const int32_t* dofs, int32_t n);
+
+/* One computeElements pass (nonlinearimplicitstatic.py:794-849) fused with
+ * updateCSR (csrgenerator.pyx:100-115):
+ *   for every element: dEps = B dU per QP, material update of the reference state
+ *   into the temporary state, Ke = sum B^T C B detJ w, Pe = -sum B^T sigma detJ w;
+ *   P = sum Pe, F = sum |Pe| (device [n_dof], overwritten);
+ *   K (context-owned) = deterministic sum of Ke in element order, in `precision`,
+ *   with the Dirichlet DOFs eliminated.
+ * dir_values (device [n_dof] or NULL): prescribed corrections on the Dirichlet DOFs;
+ *   when given, dir_correction (device [n_dof]) receives K_raw[:,dir] * dir_values on
+ *   free rows and 0 on Dirichlet rows, so that b = R - dir_correction is the
+ *   right-hand side of the eliminated system.
+ * time host [2] (step time, total time) and dT mirror computeYourself's arguments.
+ * Returns EFE_OK or EFE_CUTBACK (then *pNewDT = 0.5, vonmises.py:231). */
+int efe_compute_elements(efe_ctx* ctx, const double* U, const double* dU, const double* time, double dT,
+                         double* P, double* F, const double* dir_values, double* dir_correction,
+                         int32_t precision, double* pNewDT);
+
+/* External body-force vector (element.py:348-371), added into PExt (device [n_dof]). */
+int efe_body_force(efe_ctx* ctx, const double* force3, double* PExt);
+
+/* The assembled K (before Dirichlet elimination only if no Dirichlet set is active)
+ * in CSR value order (device [csr_nnz], FP64). */
+int efe_export_csr_values(efe_ctx* ctx, double* values);
+
+/* y = K x with the context's K (device [n_dof], FP64 vectors). */
+int efe_spmv(efe_ctx* ctx, const double* x, double* y);
+
+/* Solves K x = b with the context's (eliminated) K (device [n_dof]); nonlinearimplicitstatic.py:727-751.
+ * For EFE_FP32 the Krylov iteration runs entirely in FP32; x on Dirichlet DOFs is b exactly. */
+int efe_linsolve(efe_ctx* ctx, const double* b, double* x, const efe_solve_opts* opts, efe_solve_info* info);
+
+/* Convergence quantities of nonlinearimplicitstatic.py:671-687 and :786-790 (host out[5]):
+ * [max|R|, argmax|R| (first), ||ddU||_inf (0 if ddU is NULL), ||F||_1, any NaN in R or ddU]. */
+int efe_convergence_norms(efe_ctx* ctx, const double* R, const double* ddU, const double* F, double out[5]);
+
+/* Per-specimen convergence quantities for a batched context (host out[n_specimens][4]):
+ * [max|R|, ||ddU||_inf, ||F||_1, NaN flag]. */
+int efe_convergence_norms_batched(efe_ctx* ctx, const double* R, const double* ddU, const double* F, double* out);
+
+/* Accepts the temporary QP state as the new reference state (femodel.py:245-258). */
+int efe_accept_state(efe_ctx* ctx);
+
+/* Copies the reference (which=0) or temporary (which=1) QP state to the host,
+ * layout [n_elements][n_qp][n_state] with [stress 6, strain 6, material state]
+ * (element.py:222-236). */
+int efe_get_qp_state(efe_ctx* ctx, int32_t which, double* out);
+
+/* Overwrites the reference QP state from the host (same layout). */
+int efe_set_qp_state(efe_ctx* ctx, const double* in);
+
+/* Elements per workgroup of the element kernel and threads per workgroup of the
+ * Krylov kernels; 0 keeps the current choice. */
+int efe_set_launch(efe_ctx* ctx, int32_t elements_per_workgroup, int32_t krylov_block);
+
+/* Average device time (ms) of the last call of each phase, measured with HIP events
+ * on the context stream (host out[4]): element kernel, assembly, one Krylov iteration, norms. */
+int efe_get_phase_times(efe_ctx* ctx, double out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

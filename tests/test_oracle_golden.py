@@ -1,0 +1,68 @@
+"""Pins the oracle against the reference's own regression vectors (testfiles/<Name>/U.ref),
+with the reference's pass criterion max|U - U.ref| < 1e-6 (_run_tests_edelweissfe.py:101-106),
+and much tighter relative agreement."""
+
+import numpy as np
+import pytest
+
+from oracle import materials, mesh, nist
+
+
+def _box(n, l, npe, el_type, E, nu, dirichlets, nodeforces, bodyforces=()):
+    m = mesh.box_mesh(*n, *l, nodes_per_element=npe)
+    model = nist.Model(m["coords"], el_type, m["conn"], materials.LinearElastic([E, nu]), m["nodesets"], m["elsets"])
+    step = nist.Step(maxInc=1.0, minInc=1e-9, maxNumInc=1000, maxIter=25, extrapolation="off",
+                     dirichlets=dirichlets, nodeforces=nodeforces, bodyforces=list(bodyforces))
+    s = nist.NIST()
+    return s.solve_step(model, step), s
+
+
+CASES = {
+    # testfiles/TensionBarHexa8/test.inp
+    "TensionBarHexa8": dict(n=(4, 4, 30), l=(5, 5, 500), npe=8, el_type="C3D8", E=1.8e4, nu=0.22,
+                            dirichlets=[dict(nset="gen_front", values={0: 0.0, 1: 0.0, 2: 0.0})],
+                            nodeforces=[dict(nset="gen_back", vector=(0, 0, -4.0))]),
+    # testfiles/SimpleBeamHexa8/test.inp
+    "SimpleBeamHexa8": dict(n=(200, 2, 2), l=(2500, 50, 50), npe=8, el_type="C3D8", E=210000, nu=0.22,
+                            dirichlets=[dict(nset="gen_leftBack", values={0: 0.0, 1: 0.0, 2: 0.0}),
+                                        dict(nset="gen_rightBack", values={0: 0.0, 1: 0.0, 2: 0.0})],
+                            nodeforces=[dict(nset="gen_front", vector=(0, 0, -4.16))]),
+    # testfiles/WallShearHexa8/test.inp
+    "WallShearHexa8": dict(n=(20, 20, 2), l=(1000, 1000, 5), npe=8, el_type="C3D8", E=2.1e4, nu=0.22,
+                           dirichlets=[dict(nset="gen_bottom", values={0: 0.0, 1: 0.0, 2: 0.0})],
+                           nodeforces=[dict(nset="gen_top", vector=(1.0, 0, 0))],
+                           bodyforces=[dict(elset="all", vector=(0.0, -0.000077, 0))]),
+    # testfiles/WallShearHexa20/test.inp (C3D20, 27 QP)
+    "WallShearHexa20": dict(n=(10, 10, 1), l=(1000, 1000, 5), npe=20, el_type="C3D20", E=2.1e4, nu=0.22,
+                            dirichlets=[dict(nset="gen_bottom", values={0: 0.0, 1: 0.0, 2: 0.0})],
+                            nodeforces=[dict(nset="gen_top", vector=(1.18868, 0, 0))],
+                            bodyforces=[dict(elset="all", vector=(0.0, -0.000077, 0))]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_oracle_reproduces_reference_u_ref(golden, name):
+    U, s = _box(**CASES[name])
+    ref = golden["uref_" + name]
+    assert U.shape == ref.shape
+    assert np.abs(U - ref).max() < 1e-6
+    assert np.abs(U - ref).max() <= 1e-9 * np.abs(ref).max()
+    # zero increment (1 solve) then the load increment (solve + converged check)
+    assert [h[2] for h in s.history] == [1, 2]
+
+
+def test_oracle_reproduces_mesh_file_cantilever(golden):
+    """testfiles/CantileverBeamHexa8/test.inp: Abaqus-ordered mesh (not boxGen), prescribed tip displacement."""
+    nodes = golden["cantilever_nodes"]
+    els = golden["cantilever_elements"]
+    pos = {int(lbl): i for i, lbl in enumerate(nodes[:, 0])}
+    conn = np.vectorize(pos.get)(els[:, 1:])
+    sets = {k: np.array([pos[int(v)] for v in golden["cantilever_nset_" + k]]) for k in ("nSetLeft", "nSetRight")}
+    model = nist.Model(nodes[:, 1:], "C3D8", conn, materials.LinearElastic([30000.0, 0.15]), sets,
+                       {"all": np.arange(len(els))})
+    step = nist.Step(maxInc=2.5, minInc=1e-8, maxNumInc=1000, maxIter=100, extrapolation="off",
+                     dirichlets=[dict(nset="nSetLeft", values={0: 0.0, 1: 0.0, 2: 0.0}),
+                                 dict(nset="nSetRight", values={1: -1.1})])
+    U = nist.NIST().solve_step(model, step)
+    ref = golden["uref_CantileverBeamHexa8"]
+    assert np.abs(U - ref).max() < 1e-9 * np.abs(ref).max()
