@@ -43,7 +43,16 @@ struct KrylovWork {
 
 namespace {
 
-constexpr int kMaxBlock = 1024;
+// Workgroups of a launch are dealt round-robin to the 8 XCDs of the MI355X, each with
+// its own L2.  Renumbering them so that every XCD owns one contiguous range of node
+// rows keeps the x gathers of an XCD inside a slab of the mesh (plus halo), which
+// fits its L2.  The grid size of these launches is a multiple of 8.
+__device__ __forceinline__ int xcd_block() {
+    const int per = gridDim.x >> 3;
+    return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+
+__host__ __forceinline__ int round8(int n) { return (n + 7) & ~7; }
 
 template <int BS>
 __device__ __forceinline__ double block_sum(double v, double* sh) {
@@ -75,7 +84,8 @@ __global__ __launch_bounds__(BS) void spmv_kernel(int N, const int32_t* __restri
                                                   double* __restrict__ part, const int* __restrict__ flags) {
     __shared__ double sh[BS];
     if (flags && flags[0]) return;
-    const int n = blockIdx.x * BS + threadIdx.x;
+    const int blk = xcd_block();
+    const int n = blk * BS + threadIdx.x;
     double dot = 0.0;
     if (n < N) {
         const int s = n / kSlice, l = n - (n / kSlice) * kSlice;
@@ -83,13 +93,19 @@ __global__ __launch_bounds__(BS) void spmv_kernel(int N, const int32_t* __restri
         const int32_t* cp = scol + o0 + l;
         const T* kp = K + (int64_t)o0 * 9 + l;
         T y0 = 0, y1 = 0, y2 = 0;
+        // matrix values and indices are streamed once per product: non-temporal loads
+        // keep them from displacing the gathered x in L2
+#pragma unroll 3
         for (int k = 0; k < deg; ++k) {
-            const int m = cp[k * kSlice];
+            const int m = __builtin_nontemporal_load(cp + k * kSlice);
             const T x0 = x[3 * m], x1 = x[3 * m + 1], x2 = x[3 * m + 2];
             const T* v = kp + (int64_t)k * 9 * kSlice;
-            y0 += v[0 * kSlice] * x0 + v[1 * kSlice] * x1 + v[2 * kSlice] * x2;
-            y1 += v[3 * kSlice] * x0 + v[4 * kSlice] * x1 + v[5 * kSlice] * x2;
-            y2 += v[6 * kSlice] * x0 + v[7 * kSlice] * x1 + v[8 * kSlice] * x2;
+            T a[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) a[i] = __builtin_nontemporal_load(v + i * kSlice);
+            y0 += a[0] * x0 + a[1] * x1 + a[2] * x2;
+            y1 += a[3] * x0 + a[4] * x1 + a[5] * x2;
+            y2 += a[6] * x0 + a[7] * x1 + a[8] * x2;
         }
         y[3 * n] = y0;
         y[3 * n + 1] = y1;
@@ -98,7 +114,7 @@ __global__ __launch_bounds__(BS) void spmv_kernel(int N, const int32_t* __restri
     }
     if (part) {
         double s = block_sum<BS>(dot, sh);
-        if (threadIdx.x == 0) part[blockIdx.x] = s;
+        if (threadIdx.x == 0) part[blk] = s;
     }
 }
 
@@ -110,7 +126,8 @@ __global__ __launch_bounds__(BS) void pcg_init_kernel(int N, const double* __res
                                                       T* __restrict__ p, double* __restrict__ part, int nwg,
                                                       int* __restrict__ flags) {
     __shared__ double sh[BS];
-    const int n = blockIdx.x * BS + threadIdx.x;
+    const int blk = xcd_block();
+    const int n = blk * BS + threadIdx.x;
     double rz = 0, bb = 0;
     if (n < N) {
         T rv[3];
@@ -133,9 +150,9 @@ __global__ __launch_bounds__(BS) void pcg_init_kernel(int N, const double* __res
     double s1 = block_sum<BS>(rz, sh);
     double s2 = block_sum<BS>(bb, sh);
     if (threadIdx.x == 0) {
-        part[1 * nwg + blockIdx.x] = s1;
-        part[4 * nwg + blockIdx.x] = s2;
-        if (blockIdx.x == 0) {
+        part[1 * nwg + blk] = s1;
+        part[4 * nwg + blk] = s2;
+        if (blk == 0) {
             flags[0] = 0;
             flags[1] = 0;
             flags[2] = 0;
@@ -154,7 +171,8 @@ __global__ __launch_bounds__(BS) void pcg_update_kernel(int N, const T* __restri
     const double pq = reduce_partials<BS>(part, nwg, sh);
     const double rz = reduce_partials<BS>(part + (1 + par) * nwg, nwg, sh);
     const T alpha = (T)(rz / pq);
-    const int n = blockIdx.x * BS + threadIdx.x;
+    const int blk = xcd_block();
+    const int n = blk * BS + threadIdx.x;
     double rzn = 0, rr = 0;
     if (n < N) {
         T rv[3];
@@ -176,8 +194,8 @@ __global__ __launch_bounds__(BS) void pcg_update_kernel(int N, const T* __restri
     double s1 = block_sum<BS>(rzn, sh);
     double s2 = block_sum<BS>(rr, sh);
     if (threadIdx.x == 0) {
-        part[(2 - par) * nwg + blockIdx.x] = s1;
-        part[3 * nwg + blockIdx.x] = s2;
+        part[(2 - par) * nwg + blk] = s1;
+        part[3 * nwg + blk] = s2;
     }
 }
 
@@ -202,7 +220,7 @@ __global__ __launch_bounds__(BS) void pcg_direction_kernel(int N, const T* __res
         return;
     }
     const T beta = (T)(rzn / rzo);
-    const int n = blockIdx.x * BS + threadIdx.x;
+    const int n = xcd_block() * BS + threadIdx.x;
     if (n < N) {
 #pragma unroll
         for (int i = 0; i < 3; ++i) p[3 * n + i] = z[3 * n + i] + beta * p[3 * n + i];
@@ -278,31 +296,61 @@ __global__ __launch_bounds__(BS) void norms_stage1(int D, const double* __restri
     }
 }
 
-__global__ void sum_partials_kernel(const double* __restrict__ part, int n, double* __restrict__ out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double s = 0.0;
-    for (int i = 0; i < n; ++i) s += part[i];
-    *out = s;
+// Fixed-order sum of n partials by one 256-thread workgroup.
+__global__ __launch_bounds__(256) void sum_partials_kernel(const double* __restrict__ part, int n,
+                                                           double* __restrict__ out) {
+    __shared__ double sh[256];
+    const double s = reduce_partials<256>(part, n, sh);
+    if (threadIdx.x == 0) *out = s;
 }
 
-__global__ void norms_stage2(const double* __restrict__ part, int nwg, double* __restrict__ out) {
-    if (threadIdx.x != 0) return;
-    double mr = -1.0, ir = 0, md = 0, sf = 0, nn = 0;
-    for (int w = 0; w < nwg; ++w) {
+// Final reduction of the per-workgroup norms by one 256-thread workgroup (fixed order).
+__global__ __launch_bounds__(256) void norms_stage2(const double* __restrict__ part, int nwg,
+                                                    double* __restrict__ out) {
+    __shared__ double sv[256];
+    __shared__ double si[256];
+    __shared__ double sh[256];
+    double mr = -1.0, ir = 1e300, md = 0;
+    for (int w = threadIdx.x; w < nwg; w += 256) {
         const double v = part[w];
         if (v > mr || (v == mr && part[nwg + w] < ir)) {
             mr = v;
             ir = part[nwg + w];
         }
         md = fmax(md, part[2 * nwg + w]);
-        sf += part[3 * nwg + w];
-        nn += part[4 * nwg + w];
     }
-    out[0] = mr < 0 ? 0.0 : mr;
-    out[1] = ir;
-    out[2] = md;
-    out[3] = sf;
-    out[4] = nn > 0 ? 1.0 : 0.0;
+    sv[threadIdx.x] = mr;
+    si[threadIdx.x] = ir;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            const double o = sv[threadIdx.x + s], oi = si[threadIdx.x + s];
+            if (o > sv[threadIdx.x] || (o == sv[threadIdx.x] && oi < si[threadIdx.x])) {
+                sv[threadIdx.x] = o;
+                si[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    const double rmax = sv[0], rarg = si[0];
+    __syncthreads();
+    sh[threadIdx.x] = md;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const double dmax = sh[0];
+    __syncthreads();
+    const double sf = reduce_partials<256>(part + 3 * nwg, nwg, sh);
+    const double nn = reduce_partials<256>(part + 4 * nwg, nwg, sh);
+    if (threadIdx.x == 0) {
+        out[0] = rmax < 0 ? 0.0 : rmax;
+        out[1] = rmax < 0 ? 0.0 : rarg;
+        out[2] = dmax;
+        out[3] = sf;
+        out[4] = nn > 0 ? 1.0 : 0.0;
+    }
 }
 
 // Per-specimen norms (batched contexts): one workgroup per specimen over its DOF range.
@@ -419,7 +467,7 @@ int ensure_work(efe_ctx* c) {
     }
     if (w->part) hipFree(w->part);
     w->block = block;
-    w->nwg = (c->N + block - 1) / block;
+    w->nwg = round8((c->N + block - 1) / block);
     EFE_HIP(hipMalloc(&w->part, (size_t)5 * w->nwg * sizeof(double)));
     return EFE_OK;
 }
@@ -435,7 +483,7 @@ int pcg_solve(efe_ctx* c, const double* b, double* xout, const efe_solve_opts* o
     launch_init_bs<T>(c, w, st, b, Minv, x, r, z, p);
     EFE_HIP(hipGetLastError());
     // b.b (host) to fix the absolute tolerance
-    sum_partials_kernel<<<1, 64, 0, st>>>(w->part + 4 * w->nwg, w->nwg, w->dscal + 1);
+    sum_partials_kernel<<<1, 256, 0, st>>>(w->part + 4 * w->nwg, w->nwg, w->dscal + 1);
     EFE_HIP(hipMemcpyAsync(w->hpart, w->dscal + 1, sizeof(double), hipMemcpyDeviceToHost, st));
     EFE_HIP(hipStreamSynchronize(st));
     const double bnorm = sqrt(w->hpart[0]);
@@ -494,7 +542,7 @@ int pcg_solve(efe_ctx* c, const double* b, double* xout, const efe_solve_opts* o
     EFE_HIP(hipEventRecord(w->evB, st));
     EFE_HIP(hipStreamWaitEvent(c->stream, w->evB, 0));
     // final residual of the recursion
-    sum_partials_kernel<<<1, 64, 0, st>>>(w->part + 3 * w->nwg, w->nwg, w->dscal + 1);
+    sum_partials_kernel<<<1, 256, 0, st>>>(w->part + 3 * w->nwg, w->nwg, w->dscal + 1);
     EFE_HIP(hipMemcpyAsync(w->hpart, w->dscal + 1, sizeof(double), hipMemcpyDeviceToHost, st));
     EFE_HIP(hipStreamSynchronize(st));
     info->residual = sqrt(w->hpart[0]);
@@ -533,8 +581,8 @@ int efe_launch_spmv64(efe_ctx* c, const double* x, double* y) {
         efe_set_error("efe_spmv needs an FP64 operator");
         return EFE_ERR_STATE;
     }
-    spmv_kernel<double, 256><<<(c->N + 255) / 256, 256, 0, c->stream>>>(c->N, c->slice_off, c->scol, c->K64, x, y,
-                                                                         nullptr, nullptr);
+    spmv_kernel<double, 256><<<round8((c->N + 255) / 256), 256, 0, c->stream>>>(c->N, c->slice_off, c->scol, c->K64,
+                                                                                 x, y, nullptr, nullptr);
     EFE_HIP(hipGetLastError());
     return EFE_OK;
 }
@@ -544,7 +592,7 @@ int efe_launch_norms(efe_ctx* c, const double* R, const double* ddU, const doubl
     int nwg = (c->D + BS - 1) / BS;
     if (nwg > 1024) nwg = 1024;
     norms_stage1<BS><<<nwg, BS, 0, c->stream>>>(c->D, R, ddU, F, c->red + 8, nwg);
-    norms_stage2<<<1, 1, 0, c->stream>>>(c->red + 8, nwg, c->red);
+    norms_stage2<<<1, 256, 0, c->stream>>>(c->red + 8, nwg, c->red);
     EFE_HIP(hipGetLastError());
     EFE_HIP(hipMemcpyAsync(out, c->red, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     EFE_HIP(hipStreamSynchronize(c->stream));

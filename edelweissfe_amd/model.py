@@ -25,7 +25,21 @@ class FEModel:
         self.U = np.zeros(self.nDof)
         self.P = np.zeros(self.nDof)
         self.dU = np.zeros(self.nDof)
-        self.qpState = None
+        self._qpState = None
+        self.deviceSystem = None  # holds the QP state on the GPU after a device solve
+
+    @property
+    def qpState(self):
+        """QP state [element][QP][stress 6, strain 6, material state] (element.py:222-236),
+        copied from the device on first access after a solve."""
+        if self._qpState is None and self.deviceSystem is not None:
+            self._qpState = self.deviceSystem.qp_state(0)
+        return self._qpState
+
+    @qpState.setter
+    def qpState(self, value):
+        self._qpState = value
+        self.deviceSystem = None
 
     @property
     def nNodes(self):

@@ -78,6 +78,7 @@ class NISTGPU:
         self.computationTimes = {k: 0.0 for k in ("elements", "linear solve", "convergence check", "loads")}
         self.krylovIterations = 0
         self.linearSolves = 0
+        self.elementPasses = 0
 
     def _updateOptions(self, updated):
         for k, v in updated.items():
@@ -93,9 +94,19 @@ class NISTGPU:
         if self.system is None or self.system.n_nodes != model.nNodes:
             self.system = DeviceSystem(model.coordinates, model.connectivity, model.elementType, mat["name"],
                                        mat["properties"])
-            if model.qpState is not None:
-                self.system.set_qp_state(model.qpState)
+            if model._qpState is not None:
+                self.system.set_qp_state(model._qpState)
+        elif model.deviceSystem is self.system:
+            pass  # a following step of the same model: the state is already on the device
+        elif model._qpState is not None:
+            self.system.set_qp_state(model._qpState)
+        else:
+            self.system.reset_state()
         return self.system
+
+    def attach(self, system):
+        """Use an already created DeviceSystem (mesh, pattern and buffers resident on the GPU)."""
+        self.system = system
 
     @property
     def _mixed(self):
@@ -111,10 +122,15 @@ class NISTGPU:
         D = sysm.n_dof
         dirichlets = list(step.actions["dirichlet"].values())
         self._dirIdx = [d.indices(model) for d in dirichlets]
-        allDir = np.concatenate(self._dirIdx) if dirichlets else np.zeros(0, dtype=np.int64)
+        cat = np.concatenate(self._dirIdx) if dirichlets else np.zeros(0, dtype=np.int64)
+        # R[idx] = delta is applied Dirichlet by Dirichlet in the reference (:616-618): on a DOF
+        # named twice the last one wins, so keep the last occurrence of every DOF
+        rev = cat[::-1]
+        _, first_in_rev = np.unique(rev, return_index=True)
+        self._dirPick = np.sort(cat.size - 1 - first_in_rev)
+        allDir = cat[self._dirPick]
         sysm.set_dirichlet(allDir)
         self._dirAll = torch.as_tensor(allDir, dtype=torch.int64, device=dev)
-        self._dirIdxT = [torch.as_tensor(ix, dtype=torch.int64, device=dev) for ix in self._dirIdx]
         self._nAccFlux = model.nElements * (8 if model.elementType == "C3D8" else 20) * 3
 
         U = torch.as_tensor(model.U, dtype=torch.float64, device=dev).clone()
@@ -160,7 +176,8 @@ class NISTGPU:
         model.U = U.cpu().numpy()
         model.P = P.cpu().numpy()
         model.dU = dU.cpu().numpy()
-        model.qpState = sysm.qp_state(0)
+        model._qpState = None
+        model.deviceSystem = sysm
         return True, model
 
     # ---- one increment (nonlinearimplicitstatic.py:334-458) ---------------------------------------------------
@@ -178,11 +195,13 @@ class NISTGPU:
         R = torch.zeros(D, dtype=torch.float64, device=dev)
         dirVals = None
 
-        deltas = [torch.as_tensor(d.getDelta(model, timeStep), dtype=torch.float64, device=dev) for d in dirichlets]
+        if dirichlets:
+            delta = np.concatenate([d.getDelta(model, timeStep) for d in dirichlets])[self._dirPick]
+            delta = torch.as_tensor(delta, dtype=torch.float64).to(dev, non_blocking=True)
         if extrapolation == "linear" and prevTimeStep is not None and prevTimeStep.timeIncrement:
             dU = dU * (timeStep.stepProgressIncrement / prevTimeStep.stepProgressIncrement)
-            for ix, dl in zip(self._dirIdxT, deltas):
-                dU[ix] = dl
+            if dirichlets:
+                dU[self._dirAll] = delta
             extrapolated = True
         else:
             extrapolated = False
@@ -193,8 +212,7 @@ class NISTGPU:
             first = iterationCounter == 0 and not extrapolated and bool(dirichlets)
             if first:
                 dirVals = torch.zeros(D, dtype=torch.float64, device=dev)
-                for ix, dl in zip(self._dirIdxT, deltas):
-                    dirVals[ix] = dl
+                dirVals[self._dirAll] = delta
                 corr = torch.empty(D, dtype=torch.float64, device=dev)
             else:
                 corr = None
@@ -203,11 +221,12 @@ class NISTGPU:
                                              precision=prec, time=(timeStep.stepTime, timeStep.totalTime),
                                              dT=timeStep.timeIncrement)
             self.computationTimes["elements"] += _time.perf_counter() - tic
+            self.elementPasses += 1
             if st == _lib.EFE_CUTBACK:
                 raise CutbackRequest("Von Mises Newton failed.", pnew)
             torch.add(P, PExt, out=R)
             if first:
-                R[self._dirAll] = dirVals[self._dirAll]
+                R[self._dirAll] = delta
             elif self._dirAll.numel():
                 R[self._dirAll] = 0.0
             tic = _time.perf_counter()
