@@ -234,6 +234,7 @@ void efe_ctx_destroy(efe_ctx* c) {
     if (!c) return;
     hipDeviceSynchronize();
     efe_krylov_free(c);
+    efe_gmres_free(c);
     void* bufs[] = {c->coords, c->conn,  c->props,  c->dNdxi,  c->shapeN, c->qpw,    c->stateRef, c->stateTmp,
                     c->Kscr,   c->Pscr,  c->bptr,   c->bcol,   c->coff,   c->clist,  c->nptr,     c->nlist,
                     c->diagk,  c->slice_off, c->scol, c->K64,  c->K32,    c->Minv64, c->Minv32,   c->dirflag,

@@ -35,11 +35,25 @@ struct HostPattern {
     int64_t padded_blocks = 0;
 };
 
+// Multicolor block ILU(0) structure (rows in elimination order = (color, node)).
+struct IluPattern {
+    int ncolors = 0;
+    std::vector<int32_t> color_start;  // [ncolors+1] first factor row of each color
+    std::vector<int32_t> perm, iperm;  // factor row -> node, node -> factor row
+    std::vector<int32_t> slice_off;    // sliced layout of the factor
+    std::vector<int32_t> scol;         // factor block columns (new numbering), ascending per row
+    std::vector<int32_t> kpos;         // per factor block: value base of the same block in the operator
+    std::vector<int32_t> rowlen, dslot;
+};
+
+bool efe_build_ilu_pattern(const HostPattern& hp, IluPattern& ip, std::string& err);
+
 // Builds the adjacency, gather maps and sliced layout from the element connectivity
 // (host, conn [E][NN] node positions).  Returns false on inconsistent input.
 bool efe_build_pattern(HostPattern& hp, int N, int E, int NN, const int32_t* conn, std::string& err);
 
 struct KrylovWork;
+struct GmresWork;
 
 struct efe_ctx {
     int N = 0, E = 0, NN = 0, Q = 0, D = 0;
@@ -84,6 +98,8 @@ struct efe_ctx {
     int redn = 0;
 
     KrylovWork* kw = nullptr;
+    GmresWork* gw = nullptr;
+    int Kversion = 0;            // incremented by every assembly
 
     // launch configuration
     int epw = 0;       // elements per workgroup of the element kernel (0 = default)
@@ -107,3 +123,7 @@ int efe_launch_spmv64(efe_ctx* c, const double* x, double* y);
 int efe_launch_norms(efe_ctx* c, const double* R, const double* ddU, const double* F, double* out);
 int efe_launch_norms_batched(efe_ctx* c, const double* R, const double* ddU, const double* F, double* out);
 void efe_krylov_free(efe_ctx* c);
+
+// GMRES (gmres.hip)
+int efe_gmres_solve(efe_ctx* c, const double* b, double* x, const efe_solve_opts* o, efe_solve_info* info);
+void efe_gmres_free(efe_ctx* c);

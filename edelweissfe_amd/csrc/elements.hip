@@ -422,6 +422,7 @@ int efe_launch_assembly(efe_ctx* c, double* P, double* F, const double* dir_valu
     }
     EFE_HIP(hipGetLastError());
     c->Kprec = precision;
+    c->Kversion++;
     return EFE_OK;
 }
 

@@ -564,8 +564,9 @@ int efe_krylov_solve(efe_ctx* c, const double* b, double* x, const efe_solve_opt
         efe_set_error("solve precision differs from the precision K was assembled in");
         return EFE_ERR_ARG;
     }
+    if (o->solver == EFE_SOLVER_GMRES) return efe_gmres_solve(c, b, x, o, info);
     if (o->solver != EFE_SOLVER_CG || o->precond != EFE_PRECOND_BLOCK_JACOBI) {
-        efe_set_error("solver/preconditioner combination not available for this context");
+        efe_set_error("CG runs with the block-Jacobi preconditioner (ILU(0) is available with GMRES)");
         return EFE_ERR_ARG;
     }
     int rc = ensure_work(c);

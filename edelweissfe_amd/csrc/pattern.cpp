@@ -115,3 +115,81 @@ bool efe_build_pattern(HostPattern& hp, int N, int E, int NN, const int32_t* con
     }
     return true;
 }
+
+// Multicolor elimination order for block ILU(0): greedy coloring of the node graph in
+// natural order (no two adjacent nodes share a color), elimination order (color, node).
+// The factor is stored in its own sliced-ELL layout whose rows follow that order, so
+// the rows of one color are one contiguous, mutually independent range.
+bool efe_build_ilu_pattern(const HostPattern& hp, IluPattern& ip, std::string& err) {
+    const int N = hp.N;
+    std::vector<int32_t> color(N, -1);
+    std::vector<int32_t> mark;
+    int ncolors = 0;
+    for (int n = 0; n < N; ++n) {
+        mark.assign(ncolors + 1, 0);
+        for (int32_t b = hp.bptr[n]; b < hp.bptr[n + 1]; ++b) {
+            int m = hp.bcol[b];
+            if (m != n && color[m] >= 0) mark[color[m]] = 1;
+        }
+        int c = 0;
+        while (mark[c]) ++c;
+        color[n] = c;
+        if (c + 1 > ncolors) ncolors = c + 1;
+    }
+    ip.ncolors = ncolors;
+    ip.color_start.assign(ncolors + 1, 0);
+    for (int n = 0; n < N; ++n) ip.color_start[color[n] + 1]++;
+    for (int c = 0; c < ncolors; ++c) ip.color_start[c + 1] += ip.color_start[c];
+    ip.perm.resize(N);   // new -> old
+    ip.iperm.resize(N);  // old -> new
+    {
+        std::vector<int32_t> fill(ip.color_start.begin(), ip.color_start.end() - 1);
+        for (int n = 0; n < N; ++n) {
+            int i = fill[color[n]]++;
+            ip.perm[i] = n;
+            ip.iperm[n] = i;
+        }
+    }
+    // factor rows in the new numbering, columns sorted in the new numbering
+    const int ns = (N + kSlice - 1) / kSlice;
+    ip.slice_off.assign(ns + 1, 0);
+    ip.rowlen.assign(N, 0);
+    ip.dslot.assign(N, -1);
+    for (int s = 0; s < ns; ++s) {
+        int d = 0;
+        for (int l = 0; l < kSlice; ++l) {
+            int i = s * kSlice + l;
+            if (i < N) d = std::max(d, hp.bptr[ip.perm[i] + 1] - hp.bptr[ip.perm[i]]);
+        }
+        ip.slice_off[s + 1] = ip.slice_off[s] + d * kSlice;
+    }
+    const int64_t padded = ip.slice_off[ns];
+    ip.scol.assign(padded, 0);
+    ip.kpos.assign(padded, -1);
+    std::vector<std::pair<int32_t, int32_t>> row;
+    for (int i = 0; i < N; ++i) {
+        const int n = ip.perm[i];
+        row.clear();
+        for (int32_t b = hp.bptr[n]; b < hp.bptr[n + 1]; ++b) row.emplace_back(ip.iperm[hp.bcol[b]], b - hp.bptr[n]);
+        std::sort(row.begin(), row.end());
+        const int s = i / kSlice, l = i % kSlice;
+        const int ks = n / kSlice, kl = n % kSlice;
+        ip.rowlen[i] = (int32_t)row.size();
+        for (size_t k = 0; k < row.size(); ++k) {
+            ip.scol[ip.slice_off[s] + (int64_t)k * kSlice + l] = row[k].first;
+            if (row[k].first == i) ip.dslot[i] = (int32_t)k;
+            // value base of the same block in the operator's sliced layout
+            const int64_t kb = (int64_t)hp.slice_off[ks] * 9 + (int64_t)row[k].second * 9 * kSlice + kl;
+            if (kb > INT32_MAX) {
+                err = "operator too large for the 32-bit factor map";
+                return false;
+            }
+            ip.kpos[ip.slice_off[s] + (int64_t)k * kSlice + l] = (int32_t)kb;
+        }
+        if (ip.dslot[i] < 0) {
+            err = "a node row without its diagonal block";
+            return false;
+        }
+    }
+    return true;
+}

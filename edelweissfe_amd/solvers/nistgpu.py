@@ -26,7 +26,10 @@ Device options (in addition to NISTOptions):
   precision       "fp64" (default) or "mixed": FP32 stiffness contractions,
                   FP32 Krylov inner solve, FP64 material update, residual and
                   Newton update (the Newton loop is the FP64 refinement);
-  linsolver       "pcg" (default, block-Jacobi PCG);
+  linsolver       "pcg" (default, block-Jacobi PCG, symmetric tangents) or "gmres"
+                  (GMRES(gmresRestart) for non-symmetric tangents);
+  preconditioner  "auto" (pcg: block Jacobi, gmres: multicolor block ILU(0)), "bjacobi", "ilu0";
+  gmresRestart    restart length m of GMRES(m) (default 50);
   krylovAtolFactor the Krylov solve stops once ||r||_2 <= krylovAtolFactor times the
                   Newton flux threshold max(tol * avgFlux, 1e-7) of the current
                   iteration (default 1e-2), so the next convergence check sees the
@@ -61,6 +64,8 @@ class NISTGPU:
         "extrapolation": "linear",
         "linsolver": "pcg",
         "linsolverConfigFile": "",
+        "preconditioner": "auto",
+        "gmresRestart": 50,
         "precision": "fp64",
         "krylovAtolFactor": 1e-2,
         "krylovRtol": 1e-13,
@@ -259,11 +264,17 @@ class NISTGPU:
     def _linearSolve(self, b, x, prec, fluxThreshold):
         o = self.options
         rtol = o["krylovRtolMixed"] if prec == _lib.EFE_FP32 else o["krylovRtol"]
-        if o["linsolver"].lower() != "pcg":
+        name = o["linsolver"].lower()
+        if name not in ("pcg", "gmres"):
             raise AttributeError("invalid linear solver {:} requested".format(o["linsolver"]))
-        info = self.system.linsolve(b, x, solver=_lib.EFE_SOLVER_CG, precond=_lib.EFE_PRECOND_BLOCK_JACOBI,
-                                    precision=prec, rtol=rtol, atol=o["krylovAtolFactor"] * fluxThreshold,
-                                    max_iter=o["krylovMaxIter"], check_every=o["krylovCheckEvery"])
+        solver = _lib.EFE_SOLVER_CG if name == "pcg" else _lib.EFE_SOLVER_GMRES
+        pc = o["preconditioner"].lower()
+        if pc == "auto":
+            pc = "bjacobi" if name == "pcg" else "ilu0"
+        precond = {"bjacobi": _lib.EFE_PRECOND_BLOCK_JACOBI, "ilu0": _lib.EFE_PRECOND_ILU0}[pc]
+        info = self.system.linsolve(b, x, solver=solver, precond=precond, precision=prec, rtol=rtol,
+                                    atol=o["krylovAtolFactor"] * fluxThreshold, max_iter=o["krylovMaxIter"],
+                                    restart=o["gmresRestart"], check_every=o["krylovCheckEvery"])
         self.krylovIterations += info.iterations
         self.linearSolves += 1
         return info

@@ -142,9 +142,13 @@ def superlu(A, b):
 class NIST:
     """Oracle NIST solver.  ``linsolver(A_csr, b) -> x``."""
 
-    def __init__(self, linsolver=superlu, element_kernel=None):
+    def __init__(self, linsolver=superlu, element_kernel=None, convention="reference"):
+        """convention: "reference" assembles Ke^T like DisplacementElement (element.py:318 read
+        column-major by dofmanager.py:551-555); "colmajor" assembles Ke (the native element ABI,
+        marmotelement/element.pxd:97-103), required for non-symmetric tangents."""
         self.linsolver = linsolver
         self.element_kernel = element_kernel
+        self.convention = convention
         self.history = []   # (increment number, stepProgress, iterations or None on failure)
 
     # -- setup, as solveStep :133-164
@@ -177,7 +181,7 @@ class NIST:
         if self.element_kernel is not None:
             return self.element_kernel(model, dU)
         Ke, Pe, st = el.compute(dU[self.dof_map], model.state, model.material)
-        V = assembly.vij_values_reference(Ke)
+        V = assembly.vij_values_reference(Ke) if self.convention == "reference" else assembly.vij_values_colmajor(Ke)
         P = np.zeros(model.nDof)
         F = np.zeros(model.nDof)
         np.add.at(P, self.dof_map.ravel(), Pe.ravel())

@@ -250,3 +250,92 @@ def test_device_path_is_bitwise_reproducible():
     _run_device(b, kw, bcs)
     assert np.array_equal(a.U, b.U)
     assert np.array_equal(a.qpState, b.qpState)
+
+
+def _dirichlet_system(om, pm, sysm, bcs, dU, precision):
+    """Assemble at dU with the Dirichlet set of `bcs`, iteration-0 right-hand side; returns
+    (b, oracle K with row replacement, oracle rhs, Dirichlet DOFs, prescribed values)."""
+    dirs = np.concatenate([pm.dofsOfNodeSet(s, sorted(v)) for _, s, v in bcs])
+    sysm.set_dirichlet(dirs)
+    delta = np.zeros(om.nDof)
+    for _, s, v in bcs:
+        for c, val in v.items():
+            delta[pm.dofsOfNodeSet(s, [c])] = val
+    dev = torch.device("cuda")
+    P, F, corr = sysm.zeros(), sysm.zeros(), sysm.zeros()
+    dirv = torch.as_tensor(delta, device=dev)
+    sysm.compute_elements(torch.as_tensor(dU, device=dev), P, F, dir_values=dirv, dir_correction=corr,
+                          precision=precision)
+    idx = torch.as_tensor(dirs, device=dev)
+    R = P.clone()
+    R[idx] = dirv[idx]
+    b = R - corr
+    Ke, Pe, _ = om.elements.compute(dU[om.elements.dof_map], om.state, om.material)
+    I, J, indptr, indices = _oracle_pattern(om)
+    K = oasm.csr_matrix(oasm.update_csr(oasm.vij_values_colmajor(Ke), oasm.x_map(I, J, indptr, indices),
+                                        indices.size), indptr, indices)
+    for row in dirs:
+        K.data[K.indptr[row]:K.indptr[row + 1]] = 0.0
+    d = K.diagonal()
+    d[dirs] = 1.0
+    K.setdiag(d)
+    K.eliminate_zeros()
+    Rr = np.zeros(om.nDof)
+    np.add.at(Rr, om.elements.dof_map.ravel(), Pe.ravel())
+    Rr[dirs] = delta[dirs]
+    return b, K, Rr, dirs, delta
+
+
+@pytest.mark.parametrize("precond", ["ilu0", "bjacobi"])
+def test_gmres_matches_direct_solve_on_nonsymmetric_tangent(precond):
+    from edelweissfe_amd import _lib
+
+    n = (3, 3, 3)
+    om = oracle_model(n, (1, 1, 1), "C3D20R", "druckerprager", DP)
+    pm = device_model(n, (1, 1, 1), "C3D20R", "druckerprager", DP)
+    sysm = _system(pm)
+    # a plastic state: load first, then assemble the non-symmetric consistent tangent there
+    dU = np.zeros(om.nDof)
+    coords = om.coords
+    dU[1::3] = -2e-3 * coords[:, 1]
+    dU[0::3] = 1e-3 * coords[:, 1]
+    bcs = [("bottom", "gen_bottom", {0: 0.0, 1: 0.0, 2: 0.0}), ("top", "gen_top", {1: -1e-3})]
+    b, K, Rr, dirs, delta = _dirichlet_system(om, pm, sysm, bcs, dU, _lib.EFE_FP64)
+    x = sysm.zeros()
+    pc = _lib.EFE_PRECOND_ILU0 if precond == "ilu0" else _lib.EFE_PRECOND_BLOCK_JACOBI
+    info = sysm.linsolve(b, x, solver=_lib.EFE_SOLVER_GMRES, precond=pc, precision=_lib.EFE_FP64, rtol=1e-12,
+                         max_iter=5000, restart=50)
+    assert info.status == 0, (info.iterations, info.residual)
+    xr = spla.spsolve(K, Rr, use_umfpack=False)
+    xg = x.cpu().numpy()
+    assert abs(K - K.T).max() > 1e-6 * abs(K).max(), "tangent must be non-symmetric"
+    assert rel_err(xg, xr) < 1e-9
+    # FP32 GMRES on the FP32 operator reaches its own tolerance
+    b32, _, _, _, _ = _dirichlet_system(om, pm, sysm, bcs, dU, _lib.EFE_FP32)
+    x32 = sysm.zeros()
+    info32 = sysm.linsolve(b32, x32, solver=_lib.EFE_SOLVER_GMRES, precond=pc, precision=_lib.EFE_FP32, rtol=1e-5,
+                           max_iter=5000, restart=50)
+    sysm.close()
+    assert info32.status == 0
+    assert rel_err(x32.cpu().numpy(), xr) < 1e-3  # rtol 1e-5 in FP32 times the conditioning
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_nist_drucker_prager_hexa20r_matches_oracle(precision):
+    from oracle import nist as onist
+
+    n, l = (3, 3, 3), (1.0, 1.0, 1.0)
+    om = oracle_model(n, l, "C3D20R", "druckerprager", DP)
+    pm = device_model(n, l, "C3D20R", "druckerprager", DP)
+    bcs = [("bottom", "gen_bottom", {0: 0.0, 1: 0.0, 2: 0.0}), ("top", "gen_top", {1: -4e-3, 0: 1e-3})]
+    kw = dict(maxInc=0.25, minInc=1e-6, maxNumInc=100, maxIter=15)
+    so = onist.NIST(convention="colmajor")
+    so.solve_step(om, oracle_step(bcs, **kw))
+    sd = _run_device(pm, kw, bcs, precision=precision, linsolver="gmres")
+    assert (om.state[..., 12] > 0).any()
+    if precision == "fp64":
+        assert [h[2] for h in so.history] == [r["iters"] for r in sd.history]
+        assert rel_err(pm.U, om.U) < 1e-9
+        assert rel_err(pm.qpState[..., 0:6], om.state[..., 0:6]) < 1e-8
+    else:
+        assert rel_err(pm.U, om.U) < 1e-8
