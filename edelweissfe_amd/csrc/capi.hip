@@ -194,7 +194,28 @@ int efe_ctx_create(const efe_mesh_desc* d, efe_ctx** out) {
         if ((rc = upload(&c->shapeN, sn.data(), sn.size(), st))) return fail(rc);
         if ((rc = upload(&c->qpw, w.data(), w.size(), st))) return fail(rc);
         if (d->specimen_of_node) {
+            // specimens: contiguous, non-decreasing node ranges; every element inside one specimen
+            const int32_t* sp = d->specimen_of_node;
+            std::vector<int64_t> off(c->nspec + 1, 0);
+            bool okk = sp[0] == 0;
+            for (int n = 0; n < c->N && okk; ++n) {
+                if (sp[n] < 0 || sp[n] >= c->nspec || (n > 0 && sp[n] != sp[n - 1] && sp[n] != sp[n - 1] + 1)) okk = false;
+                else off[sp[n] + 1] = n + 1;
+            }
+            okk = okk && sp[c->N - 1] == c->nspec - 1;
+            std::vector<int32_t> se(c->E);
+            for (int e = 0; e < c->E && okk; ++e) {
+                se[e] = sp[d->conn[(size_t)e * NN]];
+                for (int a = 1; a < NN; ++a)
+                    if (sp[d->conn[(size_t)e * NN + a]] != se[e]) okk = false;
+            }
+            if (!okk) {
+                efe_set_error("specimens must be contiguous node ranges 0..n_specimens-1 with elements inside one specimen");
+                return fail(EFE_ERR_ARG);
+            }
             if ((rc = upload(&c->specOfNode, d->specimen_of_node, (size_t)c->N, st))) return fail(rc);
+            if ((rc = upload(&c->specNodeOff, off.data(), off.size(), st))) return fail(rc);
+            if ((rc = upload(&c->specOfElem, se.data(), se.size(), st))) return fail(rc);
         }
     }
     if ((rc = upload(&c->bptr, hp.bptr.data(), hp.bptr.size(), st))) return fail(rc);
@@ -235,10 +256,11 @@ void efe_ctx_destroy(efe_ctx* c) {
     hipDeviceSynchronize();
     efe_krylov_free(c);
     efe_gmres_free(c);
+    efe_batched_free(c);
     void* bufs[] = {c->coords, c->conn,  c->props,  c->dNdxi,  c->shapeN, c->qpw,    c->stateRef, c->stateTmp,
                     c->Kscr,   c->Pscr,  c->bptr,   c->bcol,   c->coff,   c->clist,  c->nptr,     c->nlist,
                     c->diagk,  c->slice_off, c->scol, c->K64,  c->K32,    c->Minv64, c->Minv32,   c->dirflag,
-                    c->status, c->red,   c->specOfNode};
+                    c->status, c->red,   c->specOfNode, c->specNodeOff, c->specOfElem};
     for (void* p : bufs)
         if (p) hipFree(p);
     for (auto& ev : c->ev)
@@ -368,6 +390,30 @@ int efe_convergence_norms(efe_ctx* c, const double* R, const double* ddU, const 
 int efe_convergence_norms_batched(efe_ctx* c, const double* R, const double* ddU, const double* F, double* out) {
     if (int rc = check(c)) return rc;
     return efe_launch_norms_batched(c, R, ddU, F, out);
+}
+
+int efe_linsolve_batched(efe_ctx* c, const double* b, double* x, const efe_solve_opts* o, const double* spec_atol,
+                         const uint8_t* spec_active, int32_t* spec_iterations, efe_solve_info* info) {
+    if (int rc = check(c)) return rc;
+    if (!b || !x || !o || !info || !spec_atol) {
+        efe_set_error("efe_linsolve_batched: invalid arguments");
+        return EFE_ERR_ARG;
+    }
+    return efe_batched_linsolve(c, b, x, o, spec_atol, spec_active, spec_iterations, info);
+}
+
+int efe_accept_state_batched(efe_ctx* c, const uint8_t* accept) {
+    if (int rc = check(c)) return rc;
+    if (!c->specOfElem || !accept) {
+        efe_set_error("efe_accept_state_batched needs a context created with specimens");
+        return EFE_ERR_STATE;
+    }
+    return efe_launch_accept_batched(c, accept);
+}
+
+int efe_get_specimen_count(efe_ctx* c) {
+    if (int rc = check(c)) return rc;
+    return c->nspec;
 }
 
 int efe_accept_state(efe_ctx* c) {

@@ -54,6 +54,7 @@ bool efe_build_pattern(HostPattern& hp, int N, int E, int NN, const int32_t* con
 
 struct KrylovWork;
 struct GmresWork;
+struct BatchWork;
 
 struct efe_ctx {
     int N = 0, E = 0, NN = 0, Q = 0, D = 0;
@@ -77,7 +78,9 @@ struct efe_ctx {
     void* Kscr = nullptr;        // [E][NN][NN][9] in the assembly precision
     int KscrPrec = -1;
     double* Pscr = nullptr;      // [E][NN][3]
-    int32_t* specOfNode = nullptr;
+    int32_t* specOfNode = nullptr;   // [N] specimen of each node (batched contexts)
+    int64_t* specNodeOff = nullptr;  // [nspec+1] first node of each specimen
+    int32_t* specOfElem = nullptr;   // [E]
 
     // device pattern
     int32_t *bptr = nullptr, *bcol = nullptr, *coff = nullptr, *clist = nullptr;
@@ -99,6 +102,7 @@ struct efe_ctx {
 
     KrylovWork* kw = nullptr;
     GmresWork* gw = nullptr;
+    BatchWork* bw = nullptr;
     int Kversion = 0;            // incremented by every assembly
 
     // launch configuration
@@ -123,6 +127,12 @@ int efe_launch_spmv64(efe_ctx* c, const double* x, double* y);
 int efe_launch_norms(efe_ctx* c, const double* R, const double* ddU, const double* F, double* out);
 int efe_launch_norms_batched(efe_ctx* c, const double* R, const double* ddU, const double* F, double* out);
 void efe_krylov_free(efe_ctx* c);
+
+// batched specimens (batched.hip)
+int efe_batched_linsolve(efe_ctx* c, const double* b, double* x, const efe_solve_opts* o, const double* spec_tol,
+                         const uint8_t* spec_active, int32_t* spec_iters, efe_solve_info* info);
+int efe_launch_accept_batched(efe_ctx* c, const uint8_t* accept);
+void efe_batched_free(efe_ctx* c);
 
 // GMRES (gmres.hip)
 int efe_gmres_solve(efe_ctx* c, const double* b, double* x, const efe_solve_opts* o, efe_solve_info* info);

@@ -600,11 +600,6 @@ int efe_launch_norms(efe_ctx* c, const double* R, const double* ddU, const doubl
     return EFE_OK;
 }
 
-int efe_launch_norms_batched(efe_ctx* c, const double* R, const double* ddU, const double* F, double* out) {
-    (void)c; (void)R; (void)ddU; (void)F; (void)out;
-    efe_set_error("batched norms not available for this context");
-    return EFE_ERR_STATE;
-}
 
 void efe_krylov_free(efe_ctx* c) {
     KrylovWork* w = c->kw;

@@ -55,6 +55,7 @@ class DeviceSystem:
         sizes = (ctypes.c_int64 * 4)()
         check(self.lib.efe_get_sizes(self.ctx, sizes), "efe_get_sizes")
         self.n_dof, self.nnz, self.n_qp, self.n_state = (int(v) for v in sizes)
+        self.n_specimens = int(n_specimens) if specimen_of_node is not None else 1
 
     # -- helpers
     def _ptr(self, t):
@@ -126,6 +127,31 @@ class DeviceSystem:
         check(self.lib.efe_convergence_norms(self.ctx, self._ptr(R), self._ptr(ddU) if ddU is not None else None,
                                              self._ptr(F), out), "efe_convergence_norms")
         return np.array(out[:])
+
+    def linsolve_batched(self, b, x, spec_atol, spec_active=None, precision=_lib.EFE_FP64, rtol=0.0, max_iter=20000,
+                         check_every=16):
+        """One block-Jacobi PCG per specimen, all at once; returns (info, iterations per specimen)."""
+        o = _lib.SolveOpts(solver=_lib.EFE_SOLVER_CG, precond=_lib.EFE_PRECOND_BLOCK_JACOBI, precision=precision,
+                           max_iter=max_iter, restart=0, check_every=check_every, rtol=rtol, atol=0.0)
+        info = _lib.SolveInfo()
+        tol = np.ascontiguousarray(spec_atol, dtype=np.float64)
+        act = None if spec_active is None else np.ascontiguousarray(spec_active, dtype=np.uint8)
+        its = np.zeros(self.n_specimens, dtype=np.int32)
+        check(self.lib.efe_linsolve_batched(self.ctx, self._ptr(b), self._ptr(x), ctypes.byref(o), tol.ctypes.data,
+                                            act.ctypes.data if act is not None else None, its.ctypes.data,
+                                            ctypes.byref(info)), "efe_linsolve_batched")
+        return info, its
+
+    def norms_batched(self, R, ddU, F):
+        """[n_specimens, 4]: max|R|, ||ddU||_inf, ||F||_1, NaN flag."""
+        out = np.empty((self.n_specimens, 4))
+        check(self.lib.efe_convergence_norms_batched(self.ctx, self._ptr(R), self._ptr(ddU) if ddU is not None else None,
+                                                     self._ptr(F), out.ctypes.data), "efe_convergence_norms_batched")
+        return out
+
+    def accept_state_batched(self, accept):
+        a = np.ascontiguousarray(accept, dtype=np.uint8)
+        check(self.lib.efe_accept_state_batched(self.ctx, a.ctypes.data), "efe_accept_state_batched")
 
     def accept_state(self):
         check(self.lib.efe_accept_state(self.ctx), "efe_accept_state")

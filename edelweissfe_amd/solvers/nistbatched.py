@@ -1,0 +1,274 @@
+"""NISTBatched: many independent specimen models solved together on one GPU.
+
+Each specimen is an ordinary model with its own step (time stepper, Dirichlet and node
+force step actions) and is solved with the semantics of the reference NIST solver
+(edelweissfe/solvers/nonlinearimplicitstatic.py:110-458): its own increments, Newton
+iteration counter, convergence test, divergence count, cutbacks and state acceptance.
+What is shared is the device work: all specimens form one block-diagonal system, every
+Newton iteration is one element/assembly launch over all specimens, the convergence
+quantities come from one per-specimen reduction, and the linear solves of all specimens
+still iterating are one batched PCG with per-specimen scalars and tolerances.
+
+Increments advance in lockstep rounds: in every round each unfinished specimen takes the
+next increment of its own time stepper.  A specimen that has converged idles (its
+correction is zero, its temporary state is recomputed unchanged) until the slowest
+specimen of the round has converged or failed; a failed specimen cuts back by its step's
+cutback factor and retries in the next round.
+
+Restriction: the device status word is shared by the batch, so a material cutback request
+(CutbackRequest) fails the whole batch; J2 with linear hardening never requests one.
+"""
+
+import numpy as np
+import torch
+
+from .. import _lib
+from ..device import DeviceSystem
+from ..exceptions import ReachedMaxIncrements, ReachedMinIncrementSize, StepFailed
+
+FIELD_CORRECTION_TOL = 1e-8      # phenomena.py fieldCorrectionTolerance["displacement"]
+FLUX_RESIDUAL_TOL = 1e-8         # phenomena.py fluxResidualTolerance["displacement"]
+FLUX_RESIDUAL_TOL_ALT = 5e-3     # phenomena.py fluxResidualToleranceAlternative["displacement"]
+
+ITERATING, CONVERGED, FAILED, IDLE = 0, 1, 2, 3
+
+
+class NISTBatched:
+    identification = "NISTBatchedSolver"
+
+    options_default = {"extrapolation": "linear", "precision": "fp64", "krylovAtolFactor": 1e-2,
+                       "krylovRtol": 1e-13, "krylovRtolMixed": 1e-5, "krylovMaxIter": 50000,
+                       "krylovCheckEvery": 16, "autotune": True}
+
+    def __init__(self, **kwargs):
+        self.options = dict(self.options_default)
+        for k, v in kwargs.items():
+            if k not in self.options:
+                raise AttributeError("Invalid option {:} for {:}".format(k, self.identification))
+            self.options[k] = type(self.options_default[k])(v)
+        self.system = None
+        self.tuning = None
+        self.linearSolves = 0
+        self.elementPasses = 0
+        self.krylovIterations = 0
+        self.rounds = 0
+
+    @property
+    def _mixed(self):
+        return self.options["precision"].lower() == "mixed"
+
+    # ---- assembling the batch --------------------------------------------------------------------------
+    def build(self, models):
+        """Creates the block-diagonal device system of `models` (one element type, one material)."""
+        et = models[0].elementType
+        name = models[0].material["name"]
+        if any(m.elementType != et or m.material["name"].lower() != name.lower() for m in models):
+            raise ValueError("a batch holds one element type and one material")
+        coords, conn, props, spec = [], [], [], []
+        self.nodeOffset = [0]
+        self.elementOffset = [0]
+        for s, m in enumerate(models):
+            coords.append(m.coordinates)
+            conn.append(m.connectivity + self.nodeOffset[-1])
+            p = np.asarray(m.material["properties"], dtype=np.float64)
+            props.append(np.broadcast_to(p, (m.nElements, p.shape[-1])))
+            spec.append(np.full(m.nNodes, s, dtype=np.int32))
+            self.nodeOffset.append(self.nodeOffset[-1] + m.nNodes)
+            self.elementOffset.append(self.elementOffset[-1] + m.nElements)
+        self.system = DeviceSystem(np.vstack(coords), np.vstack(conn), et, name, np.vstack(props),
+                                   specimen_of_node=np.concatenate(spec), n_specimens=len(models))
+        self.dofOffset = [3 * o for o in self.nodeOffset]
+        if self.options["autotune"]:
+            from .. import autotune
+
+            self.tuning = autotune.tune(self.system, _lib.EFE_FP32 if self._mixed else _lib.EFE_FP64)
+        return self.system
+
+    # ---- public interface ------------------------------------------------------------------------------
+    def solveSteps(self, models, steps):
+        """Solves steps[s] on models[s] for every specimen s; returns (True, models) with
+        model.U and model.time updated.  Raises StepFailed when a specimen's increment
+        control gives up (ReachedMaxIncrements / ReachedMinIncrementSize)."""
+        S = len(models)
+        if len(steps) != S:
+            raise ValueError("one step per model")
+        sysm = self.system if self.system is not None else self.build(models)
+        if sysm.n_specimens != S:
+            raise ValueError("the batch was built for {} specimens".format(sysm.n_specimens))
+        dev = sysm.device
+        D = sysm.n_dof
+        prec = _lib.EFE_FP32 if self._mixed else _lib.EFE_FP64
+        rtol = self.options["krylovRtolMixed"] if self._mixed else self.options["krylovRtol"]
+        nn = 8 if models[0].elementType == "C3D8" else 20
+        nAcc = np.array([m.nElements * nn * 3 for m in models], dtype=np.float64)
+
+        # Dirichlet DOFs in the batch numbering; R[idx] = delta is applied action by action in
+        # the reference (:616-618), so a DOF named twice takes the last action's value
+        owners = []
+        hasDir = np.zeros(S, dtype=bool)
+        for s, (m, st) in enumerate(zip(models, steps)):
+            if st.actions.get("bodyforce"):
+                raise NotImplementedError("body forces in a batch")
+            for d in st.actions["dirichlet"].values():
+                owners.append((s, d, d.indices(m) + self.dofOffset[s]))
+                hasDir[s] = True
+        allDir = np.unique(np.concatenate([ix for _, _, ix in owners])) if owners else np.zeros(0, np.int64)
+        sysm.set_dirichlet(allDir)
+        dirT = torch.as_tensor(allDir, dtype=torch.int64, device=dev)
+        isDir = torch.zeros(D, dtype=torch.bool, device=dev)
+        isDir[dirT] = True
+        specOfDof = torch.as_tensor(np.repeat(np.arange(S), np.diff(self.dofOffset)), device=dev)
+
+        def perDof(a):
+            return torch.as_tensor(np.asarray(a), device=dev)[specOfDof]
+
+        gens = [st.getTimeStep() for st in steps]
+        finished = np.zeros(S, dtype=bool)
+        prevTS = [None] * S
+        U = torch.as_tensor(np.concatenate([m.U for m in models]), dtype=torch.float64, device=dev)
+        dU = torch.zeros(D, dtype=torch.float64, device=dev)
+        P = torch.zeros(D, dtype=torch.float64, device=dev)
+        F = torch.zeros(D, dtype=torch.float64, device=dev)
+        self.history = [[] for _ in range(S)]
+        try:
+            while True:
+                ts = [None] * S
+                for s in range(S):
+                    if not finished[s]:
+                        ts[s] = next(gens[s], None)
+                        finished[s] = ts[s] is None
+                live = ~finished
+                if not live.any():
+                    break
+                self.rounds += 1
+                dU, PExt, deltaT, extrap = self._startIncrements(models, steps, ts, prevTS, live, owners, dU, isDir,
+                                                                 perDof)
+                state, it = self._newtonRound(sysm, steps, live, extrap, hasDir, dU, P, F, PExt, deltaT, dirT,
+                                              perDof, nAcc, prec, rtol)
+                dU = self._lastDU
+                ok = state == CONVERGED
+                sysm.accept_state_batched(ok)
+                U = torch.where(perDof(ok), U + dU, U)
+                for s in np.flatnonzero(live):
+                    if ok[s]:
+                        prevTS[s] = ts[s]
+                        if it[s] >= steps[s].criticalIter:
+                            steps[s].preventIncrementIncrease()
+                        models[s].time = ts[s].totalTime
+                        self.history[s].append(int(it[s]))
+                    else:
+                        steps[s].discardAndChangeIncrement(steps[s].cutbackFactor)
+                        prevTS[s] = None
+                        self.history[s].append(np.inf)
+                dU = torch.where(perDof(~ok), torch.zeros_like(dU), dU)
+        except (ReachedMaxIncrements, ReachedMinIncrementSize):
+            raise StepFailed()
+        for s, st in enumerate(steps):
+            for group in ("dirichlet", "nodeforces"):
+                for action in st.actions.get(group, {}).values():
+                    action.applyAtStepEnd(models[s])
+        torch.cuda.synchronize(dev)
+        Uh = U.cpu().numpy()
+        for s, m in enumerate(models):
+            m.U = Uh[self.dofOffset[s]:self.dofOffset[s + 1]].copy()
+        return True, models
+
+    # ---- one lockstep round ------------------------------------------------------------------------------
+    def _startIncrements(self, models, steps, ts, prevTS, live, owners, dU, isDir, perDof):
+        """Linear extrapolation (nonlinearimplicitstatic.py:947-986), increments of the
+        prescribed values and the external node forces (:903-945) of every live specimen."""
+        S = len(models)
+        D = dU.numel()
+        dev = dU.device
+        scale = np.zeros(S)
+        extrap = np.zeros(S, dtype=bool)
+        for s in np.flatnonzero(live):
+            p = prevTS[s]
+            if self.options["extrapolation"] == "linear" and p is not None and p.timeIncrement:
+                scale[s] = ts[s].stepProgressIncrement / p.stepProgressIncrement
+                extrap[s] = True
+        delta = np.zeros(D)
+        for s, d, ix in owners:
+            if live[s]:
+                delta[ix] = d.getDelta(models[s], ts[s])
+        deltaT = torch.as_tensor(delta, device=dev)
+        dU = dU * perDof(scale)
+        dU = torch.where(perDof(extrap) & isDir, deltaT, dU)
+        PExt = None
+        for s in np.flatnonzero(live):
+            for nf in steps[s].actions.get("nodeforces", {}).values():
+                if PExt is None:
+                    PExt = np.zeros(D)
+                nodes = np.asarray(models[s].nodeSets[nf.nSet]) + self.nodeOffset[s]
+                PExt[(3 * nodes[:, None] + np.arange(3)[None, :]).ravel()] += np.tile(nf.getCurrentLoad(ts[s]),
+                                                                                        len(nodes))
+        if PExt is not None:
+            PExt = torch.as_tensor(PExt, device=dev)
+        return dU, PExt, deltaT, extrap
+
+    def _newtonRound(self, sysm, steps, live, extrap, hasDir, dU, P, F, PExt, deltaT, dirT, perDof, nAcc, prec, rtol):
+        """Newton iterations of all live specimens until each has converged or failed
+        (nonlinearimplicitstatic.py:334-458 per specimen)."""
+        S = live.size
+        D = dU.numel()
+        dev = dU.device
+        o = self.options
+        it = np.zeros(S, dtype=int)
+        grow = [(0.0, 0) for _ in range(S)]
+        state = np.where(live, ITERATING, IDLE)
+        ddU = None
+        while True:
+            first = live & (it == 0) & ~extrap & hasDir
+            if first.any():
+                dirVals = torch.where(perDof(first), deltaT, torch.zeros_like(deltaT))
+                corr = torch.empty(D, dtype=torch.float64, device=dev)
+            else:
+                dirVals = corr = None
+            st, _ = sysm.compute_elements(dU, P, F, dir_values=dirVals, dir_correction=corr, precision=prec)
+            self.elementPasses += 1
+            if st == _lib.EFE_CUTBACK:
+                raise StepFailed("a material requested a cutback inside a batch")
+            R = P + PExt if PExt is not None else P.clone()
+            if dirT.numel():
+                R[dirT] = dirVals[dirT] if dirVals is not None else 0.0
+            nrm = sysm.norms_batched(R, ddU, F)
+            avg = np.maximum(1e-10, nrm[:, 2] / nAcc)
+            tolF = np.where(it < 15, FLUX_RESIDUAL_TOL, FLUX_RESIDUAL_TOL_ALT)
+            fluxThreshold = np.maximum(tolF * avg, 1e-7)
+            for s in np.flatnonzero((state == ITERATING) & ~first):
+                flux = nrm[s, 0]
+                correction = nrm[s, 1] if ddU is not None else 0.0
+                prev, nGrew = grow[s]
+                grow[s] = (flux, nGrew + (1 if flux > prev else 0))
+                if nrm[s, 3]:
+                    state[s] = FAILED
+                elif correction < FIELD_CORRECTION_TOL and flux <= fluxThreshold[s]:
+                    state[s] = CONVERGED
+                elif grow[s][1] > steps[s].maxGrowIter or it[s] == steps[s].maxIter:
+                    state[s] = FAILED
+            go = state == ITERATING
+            if not go.any():
+                break
+            b = R - corr if corr is not None else R
+            ddU = torch.empty(D, dtype=torch.float64, device=dev)
+            info, its = sysm.linsolve_batched(b, ddU, o["krylovAtolFactor"] * fluxThreshold, go.astype(np.uint8),
+                                              precision=prec, rtol=rtol, max_iter=o["krylovMaxIter"],
+                                              check_every=o["krylovCheckEvery"])
+            self.linearSolves += int(go.sum())
+            self.krylovIterations += int(its[go].sum())
+            # a Krylov solve that did not converge or produced NaN fails its specimen
+            # (DivergingSolution, :748-749); its correction is discarded below
+            bad = go & (its >= o["krylovMaxIter"])
+            if info.status == _lib.EFE_NAN:
+                bad |= go & ~np.isfinite(sysm.norms_batched(ddU, None, ddU)[:, 0])
+            state[bad] = FAILED
+            dU = dU + ddU
+            it[go] += 1
+        self._lastDU = dU
+        return state, it
+
+    def qpStates(self, models, which=0):
+        """Reference (0) or temporary (1) QP state of every specimen, as a list of
+        [n_elements][n_qp][n_state] arrays."""
+        st = self.system.qp_state(which)
+        return [st[self.elementOffset[s]:self.elementOffset[s + 1]] for s in range(len(models))]

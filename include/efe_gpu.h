@@ -153,6 +153,23 @@ int efe_convergence_norms(efe_ctx* ctx, const double* R, const double* ddU, cons
  * [max|R|, ||ddU||_inf, ||F||_1, NaN flag]. */
 int efe_convergence_norms_batched(efe_ctx* ctx, const double* R, const double* ddU, const double* F, double* out);
 
+/* Batched contexts (specimen_of_node given): one block-Jacobi PCG per specimen, run
+ * together.  Each specimen has its own alpha/beta, its own stop rule
+ * ||r_s||_2 <= max(spec_atol[s], opts->rtol ||b_s||_2) and freezes once converged;
+ * spec_active (host [n_specimens] or NULL = all) excludes specimens (x = 0 there);
+ * spec_iterations (host [n_specimens] or NULL) receives the iteration counts.
+ * Replaces one linearSolve per specimen model (nonlinearimplicitstatic.py:727-751). */
+int efe_linsolve_batched(efe_ctx* ctx, const double* b, double* x, const efe_solve_opts* opts,
+                         const double* spec_atol, const uint8_t* spec_active, int32_t* spec_iterations,
+                         efe_solve_info* info);
+
+/* Accepts the temporary state of the specimens with accept[s] != 0 (host [n_specimens]);
+ * femodel.py:245-258 per specimen model. */
+int efe_accept_state_batched(efe_ctx* ctx, const uint8_t* accept);
+
+/* Number of specimens of the context (1 for an ordinary context). */
+int efe_get_specimen_count(efe_ctx* ctx);
+
 /* Accepts the temporary QP state as the new reference state (femodel.py:245-258). */
 int efe_accept_state(efe_ctx* ctx);
 

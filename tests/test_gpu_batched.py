@@ -1,0 +1,126 @@
+"""Batched specimens through the C ABI: each specimen of a block-diagonal batch follows its
+own oracle NIST run (increments, cutbacks, Newton counts, U and QP stress)."""
+
+import numpy as np
+import pytest
+
+from helpers import device_model, device_step, oracle_model, oracle_step, rel_err, uniaxial_bcs
+from oracle.nist import NIST
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+# mesh, E, yield stress, applied strain, maxInc, maxIter; the second specimen needs three
+# Newton iterations in its first plastic increment but may take two, so it cuts back
+SPECIMENS = [
+    ((2, 2, 2), 210000.0, 250.0, 0.010, 0.25, 15),
+    ((3, 3, 3), 180000.0, 300.0, 0.020, 1.0, 2),
+    ((4, 3, 2), 150000.0, 200.0, 0.015, 0.2, 15),
+    ((5, 4, 4), 250000.0, 350.0, 0.010, 0.1, 15),
+]
+
+
+def _props(E, fy):
+    return [E, 0.3, fy, 1000.0, 0.0, 0.0]
+
+
+def _batch(precision="fp64", autotune=False):
+    from edelweissfe_amd.solvers.nistbatched import NISTBatched
+
+    models, steps = [], []
+    for s, (n, E, fy, strain, maxInc, maxIter) in enumerate(SPECIMENS):
+        pm = device_model(n, (1.0, 1.0, 1.0), "C3D8", "vonmises", _props(E, fy), jitter=0.1, seed=s)
+        models.append(pm)
+        steps.append(device_step(pm, uniaxial_bcs(strain), maxInc=maxInc, minInc=1e-6, maxNumInc=100,
+                                 maxIter=maxIter))
+    solver = NISTBatched(precision=precision, autotune=autotune)
+    solver.solveSteps(models, steps)
+    return solver, models
+
+
+def _oracle(s):
+    n, E, fy, strain, maxInc, maxIter = SPECIMENS[s]
+    om = oracle_model(n, (1.0, 1.0, 1.0), "C3D8", "vonmises", _props(E, fy), jitter=0.1, seed=s)
+    so = NIST()
+    so.solve_step(om, oracle_step(uniaxial_bcs(strain), maxInc=maxInc, minInc=1e-6, maxNumInc=100, maxIter=maxIter))
+    return so, om
+
+
+def test_batched_specimens_follow_their_oracle_runs(tmp_path, monkeypatch):
+    monkeypatch.setenv("EFE_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    solver, models = _batch(autotune=True)
+    assert solver.tuning["elements_per_workgroup"] in (1, 2, 4, 8)
+    states = solver.qpStates(models)
+    for s, pm in enumerate(models):
+        so, om = _oracle(s)
+        want = [np.inf if h[2] is None else h[2] for h in so.history]
+        print(s, solver.history[s], want)
+        assert solver.history[s] == want
+        assert rel_err(pm.U, om.U) < 1e-9
+        assert rel_err(states[s][..., 0:6], om.state[..., 0:6]) < 1e-9
+        assert pm.time == pytest.approx(1.0)
+    assert np.inf in solver.history[1]
+
+
+def test_batched_mixed_precision_error_vs_fp64():
+    _, m64 = _batch("fp64")
+    _, m32 = _batch("mixed")
+    for a, b in zip(m32, m64):
+        assert rel_err(a.U, b.U) < 1e-8
+
+
+def _two_specimen_system():
+    from edelweissfe_amd.device import DeviceSystem
+
+    a = device_model((3, 2, 2), (1.0, 1.0, 1.0), "C3D8", "vonmises", _props(210000.0, 250.0), jitter=0.1, seed=3)
+    b = device_model((2, 3, 4), (1.0, 1.0, 1.0), "C3D8", "vonmises", _props(150000.0, 180.0), jitter=0.1, seed=4)
+    coords = np.vstack([a.coordinates, b.coordinates])
+    conn = np.vstack([a.connectivity, b.connectivity + a.nNodes])
+    props = np.vstack([np.tile(a.material["properties"], (a.nElements, 1)),
+                       np.tile(b.material["properties"], (b.nElements, 1))])
+    spec = np.r_[np.zeros(a.nNodes, np.int32), np.ones(b.nNodes, np.int32)]
+    return DeviceSystem(coords, conn, "C3D8", "vonmises", props, specimen_of_node=spec, n_specimens=2), a, b
+
+
+def test_batched_norms_solve_and_selective_accept():
+    sysm, a, b = _two_specimen_system()
+    D, da = sysm.n_dof, 3 * a.nNodes
+    g = torch.Generator(device="cpu").manual_seed(7)
+    dU = (torch.rand(D, generator=g, dtype=torch.float64) * 4e-3 - 2e-3).cuda()
+    P, F = sysm.zeros(), sysm.zeros()
+    sysm.compute_elements(dU, P, F)
+    # per-specimen convergence quantities
+    ddU = (torch.rand(D, generator=g, dtype=torch.float64) - 0.5).cuda()
+    out = sysm.norms_batched(P, ddU, F)
+    Ph, Fh, dh = P.cpu().numpy(), F.cpu().numpy(), ddU.cpu().numpy()
+    for s, sl in enumerate((slice(0, da), slice(da, D))):
+        np.testing.assert_allclose(out[s, 0], np.abs(Ph[sl]).max(), rtol=0, atol=0)
+        np.testing.assert_allclose(out[s, 1], np.abs(dh[sl]).max(), rtol=0, atol=0)
+        np.testing.assert_allclose(out[s, 2], np.abs(Fh[sl]).sum(), rtol=1e-13)
+        assert out[s, 3] == 0.0
+    # batched PCG against a direct solve of each specimen's block
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    dirs = np.r_[a.dofsOfNodeSet("gen_bottom"), da + b.dofsOfNodeSet("gen_bottom")]
+    sysm.set_dirichlet(dirs)
+    sysm.compute_elements(dU, P, F)
+    K = sp.csr_matrix((sysm.export_csr_values().cpu().numpy(), sysm.csr_pattern()[1], sysm.csr_pattern()[0]),
+                      shape=(D, D))
+    rhs = P.clone()
+    rhs[torch.as_tensor(dirs, device=rhs.device)] = 0.0
+    x = torch.empty_like(rhs)
+    info, its = sysm.linsolve_batched(rhs, x, np.array([1e-11, 1e-11]), rtol=1e-14, max_iter=5000)
+    assert info.status == 0 and (its > 0).all()
+    ref = spla.spsolve(K.tocsc(), rhs.cpu().numpy())
+    assert rel_err(x.cpu().numpy(), ref) < 1e-9
+    # an excluded specimen gets x = 0 and no iterations
+    info, its = sysm.linsolve_batched(rhs, x, np.array([1e-9, 1e-9]), spec_active=[1, 0], rtol=1e-14, max_iter=5000)
+    assert its[1] == 0 and not x[da:].abs().max().item()
+    # accept specimen 1 only: its reference state becomes the temporary state, specimen 0 keeps zeros
+    sysm.accept_state_batched([0, 1])
+    ref_state, tmp_state = sysm.qp_state(0), sysm.qp_state(1)
+    ea = a.nElements
+    assert not ref_state[:ea].any()
+    assert np.array_equal(ref_state[ea:], tmp_state[ea:])
+    assert np.abs(tmp_state[ea:, :, 0:6]).max() > 0
