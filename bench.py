@@ -35,12 +35,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_case(n, increments, strain):
+def lognormal_props(n_elements, seed=1, median=250.0, sigma_log=0.1):
+    """J2 properties per element with sigma_y lognormal (median 250, sigma_log 0.1, seed 1)."""
+    from edelweissfe_amd.utils import rng
+
+    props = np.tile(np.asarray(J2), (n_elements, 1))
+    props[:, 2] = rng.lognormal(seed, n_elements, median, sigma_log)
+    return props
+
+
+def build_case(n, increments, strain, lognormal=False):
     from edelweissfe_amd.generators.boxgen import generateBox
     from edelweissfe_amd.steps import AdaptiveStep, Dirichlet
 
     model = generateBox(n, n, n, 1.0, 1.0, 1.0, elType="C3D8", jitter=0.1, seed=0)
-    model.assignMaterial("vonmises", J2)
+    model.assignMaterial("vonmises", lognormal_props(model.nElements) if lognormal else J2)
 
     def step_for(m):
         acts = {"dirichlet": {
@@ -143,11 +152,141 @@ def load_traffic(name):
     return None
 
 
+def reset_batch(solver, count, strain, increments):
+    from edelweissfe_amd.generators.specimens import specimenBatch
+
+    models, steps, par = specimenBatch(count, seed=2, strain=strain, increments=increments)
+    if solver.system is not None:
+        solver.system.reset_state()
+    return models, steps, par
+
+
+def run_batched(solver, count, strain, increments, steps, warmup):
+    import torch
+
+    times, models = [], None
+    for k in range(warmup + steps):
+        models, st, par = reset_batch(solver, count, strain, increments)
+        if solver.system is None:
+            t0 = time.perf_counter()
+            solver.build(models)
+            torch.cuda.synchronize()
+            log("[bench] batch system: %d specimens, %d DOFs, %d nnz, tuned %s (%.1f s)" % (
+                count, solver.system.n_dof, solver.system.nnz, solver.tuning and {
+                    k2: v for k2, v in solver.tuning.items() if k2 != "timings"}, time.perf_counter() - t0))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        solver.solveSteps(models, st)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        log("[bench] batched %s %s: %.3f s, %d rounds, %d element passes, %d specimen solves, %d Krylov its" % (
+            solver.options["precision"], "warmup" if k < warmup else "timed", dt, solver.rounds,
+            solver.elementPasses, solver.linearSolves, solver.krylovIterations))
+        solver.lastRun = {"rounds": solver.rounds, "element_passes": solver.elementPasses,
+                          "krylov_iterations": solver.krylovIterations}
+        solver.rounds = solver.elementPasses = solver.linearSolves = solver.krylovIterations = 0
+        if k >= warmup:
+            times.append(dt)
+    return times, models, par
+
+
+def bench_batched(args):
+    """Batched form: 512 independent J2 cube specimens n^3, n uniform in {8..24}, E uniform in
+    [150000, 250000], sigma_y uniform in [150, 350] (seed 2), 1 % uniaxial compression in 10
+    increments each, FP64 and mixed."""
+    import torch
+
+    from edelweissfe_amd import autotune, _lib
+    from edelweissfe_amd.solvers.nistbatched import NISTBatched
+
+    count, inc = args.specimens, args.batch_increments
+    s64 = NISTBatched(precision="fp64")
+    t64, m64, par = run_batched(s64, count, args.strain, inc, args.steps, args.warmup)
+    sysm = s64.system
+    D = np.diff(s64.dofOffset)
+    solves64 = s64.specimenSolves.copy()
+    hist64 = [list(h) for h in s64.history]
+    counts64 = (s64.specimenPasses.copy(), s64.specimenSolves.copy(), s64.specimenKrylov.copy())
+    q64 = sysm.qp_state(0)
+    spmv_ms, spmv_alg = spmv_roofline(sysm)
+    log("[bench] batch SpMV %.4f ms, %.1f GB/s algorithmic" % (spmv_ms, spmv_alg / spmv_ms / 1e6))
+    wall64 = sum(t64) / len(t64)
+    result = {
+        "metric": METRIC,
+        "value": float(solves64.sum()) / wall64,
+        "unit": "Newton iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * wall64,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "%d independent hexa8 C3D8 J2 unit-cube specimens n^3, n uniform in {8..24}, "
+                               "E uniform in [150000, 250000], sigma_y uniform in [150, 350] (seed 2), 10%% jitter, "
+                               "uniaxial compression to %g strain in %d increments each, batched block-Jacobi PCG"
+                               % (count, args.strain, inc),
+                   "specimens": count, "elements": int(sysm.n_elements), "dofs": int(sysm.n_dof),
+                   "csr_nnz": int(sysm.nnz)},
+        "note": "value counts specimen Newton iterations (linear solves) per second over the whole batch",
+        "dof_updates_per_s": float((solves64 * D).sum()) / wall64,
+        "batch_element_passes": s64.lastRun["element_passes"],
+        "lockstep_rounds": s64.lastRun["rounds"],
+        "batch_krylov_iterations": s64.lastRun["krylov_iterations"],
+        "specimen_newton_iterations": int(solves64.sum()),
+        "autotune": {k: v for k, v in (s64.tuning or {}).items() if k != "timings"},
+    }
+    if not args.no_mixed:
+        s32 = NISTBatched(precision="mixed")
+        s32.attach(s64)
+        s32.tuning = autotune.tune(sysm, _lib.EFE_FP32)
+        t32, m32, _ = run_batched(s32, count, args.strain, inc, args.steps, args.warmup)
+        wall32 = sum(t32) / len(t32)
+        q32 = sysm.qp_state(0)
+        uerr = [float(np.abs(a.U - b.U).max() / np.abs(b.U).max()) for a, b in zip(m32, m64)]
+        result["mixed"] = {
+            "newton_iterations_per_s": float(s32.specimenSolves.sum()) / wall32,
+            "dof_updates_per_s": float((s32.specimenSolves * D).sum()) / wall32,
+            "ms_per_batch_history": 1000.0 * wall32,
+            "history_speedup_vs_fp64": wall64 / wall32,
+            "same_newton_counts_as_fp64": [list(h) for h in s32.history] == hist64,
+            "autotune": {k: v for k, v in (s32.tuning or {}).items() if k != "timings"},
+            "error_vs_fp64": {
+                "U_rel_inf_max_over_specimens": max(uerr),
+                "stress_rel_inf": float(np.abs(q32[..., 0:6] - q64[..., 0:6]).max() / np.abs(q64[..., 0:6]).max()),
+            },
+        }
+        log("[bench] batched mixed: %.3f s (%.2fx), U rel err %.2e" % (wall32, wall64 / wall32, max(uerr)))
+    result["roofline"] = {
+        "kernel": "spmv_kernel<double> (sliced-ELL 3x3 blocks) on the block-diagonal batch operator",
+        "bound": "hbm",
+        "achieved": spmv_alg / (spmv_ms * 1e-3) / 1e9,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": spmv_alg / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "traffic": None,
+        "algorithmic_bytes": spmv_alg,
+        "avg_launch_ms": spmv_ms,
+    }
+    if not args.no_cpu:
+        try:
+            from oracle import cpu_baseline
+
+            log("[bench] CPU baseline (bounded sample)")
+            result["cpu_baseline"] = cpu_baseline.measure_batched(par["n"], *counts64, J2, args.strain, inc,
+                                                                  budget_s=args.cpu_seconds, log=log)
+        except Exception as e:
+            result["cpu_baseline"] = {"error": repr(e)}
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--increments", type=int, default=20)
     ap.add_argument("--strain", type=float, default=0.01)
@@ -155,20 +294,32 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--profile-spmv", action="store_true", help="counter-collection mode (no JSON line)")
+    ap.add_argument("--workload", choices=("cube", "lognormal", "batched"), default="cube",
+                    help="cube: 64^3 J2 load history (default); lognormal: 96^3 J2 with per-element lognormal "
+                         "yield stress, 30 increments to 2 %% strain; batched: 512 uneven J2 specimens")
+    ap.add_argument("--specimens", type=int, default=512)
+    ap.add_argument("--batch-increments", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched":
+        bench_batched(args)
+        return
+    lognormal = args.workload == "lognormal"
+    if lognormal:
+        args.n, args.increments, args.strain = 96, 30, 0.02
 
     import torch
 
     from edelweissfe_amd.device import DeviceSystem
 
     log("[bench] building %d^3 hexa8 J2 model" % args.n)
-    template, step_for = build_case(args.n, args.increments, args.strain)
+    template, step_for = build_case(args.n, args.increments, args.strain, lognormal)
     t0 = time.perf_counter()
-    system = DeviceSystem(template.coordinates, template.connectivity, "C3D8", "vonmises", J2)
+    system = DeviceSystem(template.coordinates, template.connectivity, "C3D8", "vonmises",
+                          template.material["properties"])
     torch.cuda.synchronize()
     log("[bench] device system ready: %d DOFs, %d nnz (%.1f s)" % (system.n_dof, system.nnz, time.perf_counter() - t0))
     if args.profile_spmv:
@@ -198,9 +349,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "%d^3 hexa8 C3D8 unit cube, 10%% jitter, J2 linear hardening, uniaxial compression "
-                               "to %g strain in %d increments, block-Jacobi PCG" % (args.n, args.strain,
-                                                                                   args.increments),
+        "config": {"workload": "%d^3 hexa8 C3D8 unit cube, 10%% jitter, J2 linear hardening%s, uniaxial "
+                               "compression to %g strain in %d increments, block-Jacobi PCG" % (
+                                   args.n, " with per-element sigma_y lognormal (median 250, sigma_log 0.1, seed 1)"
+                                   if lognormal else "", args.strain, args.increments),
                    "elements": int(template.nElements), "dofs": int(D), "csr_nnz": int(system.nnz)},
         "dof_updates_per_s": nis64 * D,
         "newton_iterations_per_history": solves64,
@@ -236,7 +388,7 @@ def main():
         log("[bench] mixed: %.3f s/history (%.2fx), U rel err %.2e" % (
             wall32 / len(t32), result["mixed"]["history_speedup_vs_fp64"], err["U_rel_inf"]))
 
-    traffic = load_traffic("spmv_fp64_64cube_traffic.json")
+    traffic = load_traffic("spmv_fp64_64cube_traffic.json") if args.n == 64 else None
     result["roofline"] = {
         "kernel": "spmv_kernel<double> (sliced-ELL 3x3 blocks)",
         "bound": "hbm",

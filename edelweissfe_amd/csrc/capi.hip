@@ -425,6 +425,10 @@ int efe_accept_state(efe_ctx* c) {
 int efe_get_qp_state(efe_ctx* c, int32_t which, double* out) {
     if (int rc = check(c)) return rc;
     const size_t EQ = (size_t)c->E * c->Q, ns = c->nstate;
+    if (!out) {
+        efe_set_error("efe_get_qp_state: out is NULL");
+        return EFE_ERR_ARG;
+    }
     std::vector<double> buf(EQ * ns);
     EFE_HIP(hipMemcpyAsync(buf.data(), which ? c->stateTmp : c->stateRef, buf.size() * sizeof(double),
                            hipMemcpyDeviceToHost, c->stream));
@@ -437,6 +441,11 @@ int efe_get_qp_state(efe_ctx* c, int32_t which, double* out) {
 int efe_set_qp_state(efe_ctx* c, const double* in) {
     if (int rc = check(c)) return rc;
     const size_t EQ = (size_t)c->E * c->Q, ns = c->nstate;
+    if (!in) {  // zero state of a fresh model, cleared on the device
+        EFE_HIP(hipMemsetAsync(c->stateRef, 0, EQ * ns * sizeof(double), c->stream));
+        EFE_HIP(hipStreamSynchronize(c->stream));
+        return EFE_OK;
+    }
     std::vector<double> buf(EQ * ns);
     for (size_t k = 0; k < ns; ++k)
         for (size_t i = 0; i < EQ; ++i) buf[k * EQ + i] = in[i * ns + k];

@@ -166,8 +166,8 @@ class DeviceSystem:
         check(self.lib.efe_set_qp_state(self.ctx, state.ctypes.data), "efe_set_qp_state")
 
     def reset_state(self):
-        """Zero reference QP state (a fresh model on the same mesh)."""
-        self.set_qp_state(np.zeros((self.n_elements, self.n_qp, self.n_state)))
+        """Zero reference QP state (a fresh model on the same mesh), cleared on the device."""
+        check(self.lib.efe_set_qp_state(self.ctx, None), "efe_set_qp_state")
 
     def set_launch(self, elements_per_workgroup=0, krylov_block=0):
         check(self.lib.efe_set_launch(self.ctx, int(elements_per_workgroup), int(krylov_block)), "efe_set_launch")

@@ -52,6 +52,7 @@ class NISTBatched:
         self.elementPasses = 0
         self.krylovIterations = 0
         self.rounds = 0
+        self.specimenPasses = self.specimenSolves = self.specimenKrylov = None
 
     @property
     def _mixed(self):
@@ -83,6 +84,12 @@ class NISTBatched:
 
             self.tuning = autotune.tune(self.system, _lib.EFE_FP32 if self._mixed else _lib.EFE_FP64)
         return self.system
+
+    def attach(self, other):
+        """Reuse the device system (and batch layout) another NISTBatched built for the same models."""
+        self.system = other.system
+        self.nodeOffset, self.elementOffset, self.dofOffset = other.nodeOffset, other.elementOffset, other.dofOffset
+        self.tuning = other.tuning
 
     # ---- public interface ------------------------------------------------------------------------------
     def solveSteps(self, models, steps):
@@ -130,6 +137,11 @@ class NISTBatched:
         P = torch.zeros(D, dtype=torch.float64, device=dev)
         F = torch.zeros(D, dtype=torch.float64, device=dev)
         self.history = [[] for _ in range(S)]
+        # per-specimen work counts: element passes, linear solves and Krylov iterations the
+        # specimen would need on its own
+        self.specimenPasses = np.zeros(S, dtype=np.int64)
+        self.specimenSolves = np.zeros(S, dtype=np.int64)
+        self.specimenKrylov = np.zeros(S, dtype=np.int64)
         try:
             while True:
                 ts = [None] * S
@@ -226,6 +238,7 @@ class NISTBatched:
                 dirVals = corr = None
             st, _ = sysm.compute_elements(dU, P, F, dir_values=dirVals, dir_correction=corr, precision=prec)
             self.elementPasses += 1
+            self.specimenPasses += state == ITERATING
             if st == _lib.EFE_CUTBACK:
                 raise StepFailed("a material requested a cutback inside a batch")
             R = P + PExt if PExt is not None else P.clone()
@@ -256,6 +269,8 @@ class NISTBatched:
                                               check_every=o["krylovCheckEvery"])
             self.linearSolves += int(go.sum())
             self.krylovIterations += int(its[go].sum())
+            self.specimenSolves += go
+            self.specimenKrylov += np.where(go, its, 0)
             # a Krylov solve that did not converge or produced NaN fails its specimen
             # (DivergingSolution, :748-749); its correction is discarded below
             bad = go & (its >= o["krylovMaxIter"])

@@ -178,7 +178,8 @@ int efe_accept_state(efe_ctx* ctx);
  * (element.py:222-236). */
 int efe_get_qp_state(efe_ctx* ctx, int32_t which, double* out);
 
-/* Overwrites the reference QP state from the host (same layout). */
+/* Overwrites the reference QP state from the host (same layout); in == NULL sets it to
+ * zero (the initial state of a fresh model, element.py:207-240). */
 int efe_set_qp_state(efe_ctx* ctx, const double* in);
 
 /* Elements per workgroup of the element kernel and threads per workgroup of the

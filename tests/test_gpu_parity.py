@@ -339,3 +339,34 @@ def test_nist_drucker_prager_hexa20r_matches_oracle(precision):
         assert rel_err(pm.qpState[..., 0:6], om.state[..., 0:6]) < 1e-8
     else:
         assert rel_err(pm.U, om.U) < 1e-8
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_nist_j2_per_element_lognormal_yield_matches_oracle(precision):
+    """Per-element material properties (sigma_y lognormal per element, the 96^3 workload's
+    material field at 5^3) through the device path against the oracle with the same
+    properties repeated per QP."""
+    from edelweissfe_amd.utils import rng
+
+    n, l = (5, 5, 5), (1.0, 1.0, 1.0)
+    E = n[0] * n[1] * n[2]
+    props = np.tile(np.asarray(J2), (E, 1))
+    props[:, 2] = rng.lognormal(1, E, 250.0, 0.1)
+    om = oracle_model(n, l, "C3D8", "vonmises", np.repeat(props, 8, axis=0), jitter=0.1, seed=0)
+    pm = device_model(n, l, "C3D8", "vonmises", props, jitter=0.1, seed=0)
+    bcs = uniaxial_bcs(0.02)
+    kw = dict(maxInc=1.0 / 6, minInc=1e-6, maxNumInc=100, maxIter=15)
+    so = __import__("oracle.nist", fromlist=["NIST"]).NIST()
+    so.solve_step(om, oracle_step(bcs, **kw))
+    sd = _run_device(pm, kw, bcs, precision=precision)
+    # the yield stresses spread, and so do the plastic strains they leave
+    kap = om.state[..., 12]
+    assert np.ptp(props[:, 2]) > 50.0 and kap.min() < 0.9 * kap.max()
+    if precision == "fp64":
+        assert [h[2] for h in so.history] == [r["iters"] for r in sd.history]
+        tol = 1e-9
+    else:
+        tol = 1e-8
+    assert rel_err(pm.U, om.U) < tol
+    assert rel_err(pm.qpState[..., 0:6], om.state[..., 0:6]) < 10 * tol
+    assert rel_err(pm.qpState[..., 12], kap) < 100 * tol
