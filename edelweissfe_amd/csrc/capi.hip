@@ -181,17 +181,18 @@ int efe_ctx_create(const efe_mesh_desc* d, efe_ctx** out) {
         if ((rc = upload(&c->props, pr.data(), pr.size(), st))) return fail(rc);
         std::vector<double> xi, eta, zeta, w;
         qp_table(c->eltype, xi, eta, zeta, w);
-        std::vector<double> dn((size_t)Q * NN * 3), sn((size_t)Q * NN), tmp((size_t)NN * 3), tn(NN);
+        std::vector<double> dn((size_t)Q * NN * 3), sn((size_t)Q * NN), tmp((size_t)NN * 3), tn((size_t)Q * NN);
         for (int q = 0; q < Q; ++q) {
             // J and B: the reference evaluates QP q at (eta_q, xi_q, zeta_q)
             const double pe[3] = {eta[q], xi[q], zeta[q]};
-            shape(NN, pe, tn.data(), dn.data() + (size_t)q * NN * 3);
+            shape(NN, pe, tn.data() + (size_t)q * NN, dn.data() + (size_t)q * NN * 3);
             // N for body forces: at the table point (xi_q, eta_q, zeta_q)
             const double pt[3] = {xi[q], eta[q], zeta[q]};
             shape(NN, pt, sn.data() + (size_t)q * NN, tmp.data());
         }
         if ((rc = upload(&c->dNdxi, dn.data(), dn.size(), st))) return fail(rc);
         if ((rc = upload(&c->shapeN, sn.data(), sn.size(), st))) return fail(rc);
+        if ((rc = upload(&c->shapeNe, tn.data(), tn.size(), st))) return fail(rc);
         if ((rc = upload(&c->qpw, w.data(), w.size(), st))) return fail(rc);
         if (d->specimen_of_node) {
             // specimens: contiguous, non-decreasing node ranges; every element inside one specimen
@@ -257,7 +258,8 @@ void efe_ctx_destroy(efe_ctx* c) {
     efe_krylov_free(c);
     efe_gmres_free(c);
     efe_batched_free(c);
-    void* bufs[] = {c->coords, c->conn,  c->props,  c->dNdxi,  c->shapeN, c->qpw,    c->stateRef, c->stateTmp,
+    void* bufs[] = {c->shapeNe, c->geoElems,
+                    c->coords, c->conn,  c->props,  c->dNdxi,  c->shapeN, c->qpw,    c->stateRef, c->stateTmp,
                     c->Kscr,   c->Pscr,  c->bptr,   c->bcol,   c->coff,   c->clist,  c->nptr,     c->nlist,
                     c->diagk,  c->slice_off, c->scol, c->K64,  c->K32,    c->Minv64, c->Minv32,   c->dirflag,
                     c->status, c->red,   c->specOfNode, c->specNodeOff, c->specOfElem};
@@ -352,6 +354,24 @@ int efe_compute_elements(efe_ctx* c, const double* U, const double* dU, const do
 int efe_body_force(efe_ctx* c, const double* force3, double* PExt) {
     if (int rc = check(c)) return rc;
     return efe_launch_body_force(c, force3, PExt);
+}
+
+int efe_set_geostatic(efe_ctx* c, const double def[6], const int32_t* elements, int32_t n) {
+    if (int rc = check(c)) return rc;
+    if (!def || (elements && (n < 0 || n > c->E)) || def[3] == def[1]) {
+        efe_set_error("efe_set_geostatic: invalid arguments (h1 == h2 or element list)");
+        return EFE_ERR_ARG;
+    }
+    if (elements)
+        for (int32_t i = 0; i < n; ++i)
+            if (elements[i] < 0 || elements[i] >= c->E) {
+                efe_set_error("efe_set_geostatic: element index out of range");
+                return EFE_ERR_ARG;
+            }
+    int rc = efe_launch_geostatic(c, def, elements, elements ? n : c->E);
+    if (rc) return rc;
+    EFE_HIP(hipStreamSynchronize(c->stream));
+    return EFE_OK;
 }
 
 int efe_export_csr_values(efe_ctx* c, double* values) {

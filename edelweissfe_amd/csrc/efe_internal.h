@@ -72,6 +72,8 @@ struct efe_ctx {
     double* props = nullptr;     // [nprops][E] or [nprops]
     double* dNdxi = nullptr;     // [Q][NN][3] at the evaluation points
     double* shapeN = nullptr;    // [Q][NN] at the table points (body force)
+    double* shapeNe = nullptr;   // [Q][NN] at the evaluation points (QP coordinates)
+    int32_t* geoElems = nullptr; // [E] element subset of a geostatic initial state
     double* qpw = nullptr;       // [Q]
     double* stateRef = nullptr;  // [nstate][E][Q]
     double* stateTmp = nullptr;
@@ -119,6 +121,7 @@ int efe_launch_elements(efe_ctx* c, const double* dU, int precision);
 int efe_launch_assembly(efe_ctx* c, double* P, double* F, const double* dir_values, double* dir_correction,
                         int precision);
 int efe_launch_body_force(efe_ctx* c, const double* f3, double* PExt);
+int efe_launch_geostatic(efe_ctx* c, const double* def, const int32_t* elements, int32_t n);
 int efe_launch_export_csr(efe_ctx* c, double* values);
 
 // Krylov (krylov.hip)

@@ -450,6 +450,53 @@ int efe_launch_body_force(efe_ctx* c, const double* f3, double* PExt) {
     return EFE_OK;
 }
 
+// Geostatic initial stress at every QP of the listed elements (all when elems is NULL):
+// sigma_yy = p1 + (p2 - p1) (y - h1) / (h2 - h1) at the QP's y coordinate, sigma_xx =
+// xLateral sigma_yy, sigma_zz = zLateral sigma_yy; shear stresses and all other state stay.
+template <int NN>
+__global__ __launch_bounds__(256) void geostatic_kernel(int E, int Q, int n, int N, const int32_t* __restrict__ elems,
+                                                        const double* __restrict__ coords,
+                                                        const int32_t* __restrict__ conn,
+                                                        const double* __restrict__ shapeNe, double p1, double h1,
+                                                        double p2, double h2, double xl, double zl,
+                                                        double* __restrict__ stateRef) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)n * Q) return;
+    const int k = (int)(i / Q), q = (int)(i - (int64_t)k * Q);
+    const int e = elems ? elems[k] : k;
+    double y = 0.0;
+#pragma unroll
+    for (int a = 0; a < NN; ++a) y += shapeNe[q * NN + a] * coords[N + conn[(int64_t)a * E + e]];
+    const double sy = p1 + (p2 - p1) * (y - h1) / (h2 - h1);
+    const int64_t EQ = (int64_t)E * Q, idx = (int64_t)e * Q + q;
+    stateRef[idx] = xl * sy;
+    stateRef[EQ + idx] = sy;
+    stateRef[2 * EQ + idx] = zl * sy;
+}
+
+int efe_launch_geostatic(efe_ctx* c, const double* def, const int32_t* elements, int32_t n) {
+    const int32_t* de = nullptr;
+    if (elements) {
+        if (!c->geoElems) EFE_HIP(hipMalloc(&c->geoElems, (size_t)c->E * sizeof(int32_t)));
+        EFE_HIP(hipMemcpyAsync(c->geoElems, elements, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice,
+                               c->stream));
+        de = c->geoElems;
+    }
+    if (n == 0) return EFE_OK;
+    const int64_t work = (int64_t)n * c->Q;
+    const unsigned grid = (unsigned)((work + 255) / 256);
+    if (c->NN == 8)
+        geostatic_kernel<8><<<grid, 256, 0, c->stream>>>(c->E, c->Q, n, c->N, de, c->coords, c->conn, c->shapeNe,
+                                                          def[0], def[1], def[2], def[3], def[4], def[5],
+                                                          c->stateRef);
+    else
+        geostatic_kernel<20><<<grid, 256, 0, c->stream>>>(c->E, c->Q, n, c->N, de, c->coords, c->conn, c->shapeNe,
+                                                           def[0], def[1], def[2], def[3], def[4], def[5],
+                                                           c->stateRef);
+    EFE_HIP(hipGetLastError());
+    return EFE_OK;
+}
+
 int efe_launch_export_csr(efe_ctx* c, double* values) {
     const int grid = (c->N + 255) / 256;
     if (c->Kprec == EFE_FP32)

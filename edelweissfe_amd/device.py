@@ -105,6 +105,15 @@ class DeviceSystem:
         f = (ctypes.c_double * 3)(*[float(v) for v in force])
         check(self.lib.efe_body_force(self.ctx, f, self._ptr(PExt)), "efe_body_force")
 
+    def set_geostatic(self, definition, elements=None):
+        """Geostatic initial stress [p1, h1, p2, h2, xLateral, zLateral] on `elements` (all if None)."""
+        d = (ctypes.c_double * 6)(*[float(v) for v in definition])
+        if elements is None:
+            check(self.lib.efe_set_geostatic(self.ctx, d, None, 0), "efe_set_geostatic")
+        else:
+            el = np.ascontiguousarray(elements, dtype=np.int32)
+            check(self.lib.efe_set_geostatic(self.ctx, d, el.ctypes.data, int(el.size)), "efe_set_geostatic")
+
     def export_csr_values(self):
         out = torch.empty(self.nnz, dtype=torch.float64, device=self.device)
         check(self.lib.efe_export_csr_values(self.ctx, ctypes.c_void_p(out.data_ptr())), "efe_export_csr_values")

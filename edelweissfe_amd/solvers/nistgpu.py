@@ -174,15 +174,15 @@ class NISTGPU:
                 self.history.append(record)
         except (ReachedMaxIncrements, ReachedMinIncrementSize):
             raise StepFailed()
-        for group in ("dirichlet", "nodeforces", "bodyforce"):
-            for action in step.actions.get(group, {}).values():
-                action.applyAtStepEnd(model)
         torch.cuda.synchronize(dev)
         model.U = U.cpu().numpy()
         model.P = P.cpu().numpy()
         model.dU = dU.cpu().numpy()
         model._qpState = None
         model.deviceSystem = sysm
+        for group in ("dirichlet", "nodeforces", "bodyforce", "geostatic"):
+            for action in step.actions.get(group, {}).values():
+                action.applyAtStepEnd(model)
         return True, model
 
     # ---- one increment (nonlinearimplicitstatic.py:334-458) ---------------------------------------------------
@@ -213,7 +213,10 @@ class NISTGPU:
             dU = torch.zeros(D, dtype=torch.float64, device=dev)
         PExt = self._externalLoads(step, model, timeStep)
 
+        geostatics = list(step.actions.get("geostatic", {}).values())
         while True:
+            for g in geostatics:  # nonlinearimplicitstatic.py:413-414
+                g.applyAtIterationStart(sysm, model)
             first = iterationCounter == 0 and not extrapolated and bool(dirichlets)
             if first:
                 dirVals = torch.zeros(D, dtype=torch.float64, device=dev)

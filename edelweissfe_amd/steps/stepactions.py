@@ -7,6 +7,10 @@
               start + delta * amp(p).
 * BodyForce   edelweissfe/stepactions/bodyforce.py: force per volume on an element
               set, start + delta * amp(p).
+* Geostatic   edelweissfe/stepactions/geostatic.py:51-101: geostatic initial stress
+              [p1, h1, p2, h2, xLateral, zLateral] on an element set, set into the
+              reference state at every iteration start; displacements are reset at the
+              end of the step, after which the action is inactive.
 amp(t) = t unless an amplitude callable is given (the reference's f(t)).
 """
 
@@ -82,3 +86,23 @@ class BodyForce:
             self.start = self.start + self.delta * self.amplitude(1.0)
             self.delta = np.zeros(3)
             self.idle = True
+
+
+class Geostatic:
+    def __init__(self, name, p1, h1=1.0, p2=None, h2=-1.0, xLateral=1.0, zLateral=1.0, elSet="all"):
+        self.name = name
+        self.elSet = elSet
+        self.definition = np.array([p1, h1, p1 if p2 is None else p2, h2, xLateral, zLateral], dtype=float)
+        self.active = True
+
+    def applyAtIterationStart(self, system, model):
+        if not self.active:
+            return
+        els = model.elementSets.get(self.elSet) if self.elSet != "all" else None
+        system.set_geostatic(self.definition, els)
+
+    def applyAtStepEnd(self, model):
+        if not self.active:
+            return
+        model.U[:] = 0.0
+        self.active = False

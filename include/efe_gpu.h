@@ -134,6 +134,15 @@ int efe_compute_elements(efe_ctx* ctx, const double* U, const double* dU, const 
 /* External body-force vector (element.py:348-371), added into PExt (device [n_dof]). */
 int efe_body_force(efe_ctx* ctx, const double* force3, double* PExt);
 
+/* Geostatic initial stress (stepactions/geostatic.py:94-101 -> the element's
+ * setInitialCondition("geostatic stress", [p1, h1, p2, h2, xLateral, zLateral])):
+ * at every QP of the listed elements (host [n], or NULL for all elements) the reference
+ * state's sigma_yy = p1 + (p2 - p1) (y - h1) / (h2 - h1) at the QP's y coordinate,
+ * sigma_xx = xLateral sigma_yy, sigma_zz = zLateral sigma_yy; shear stresses and the
+ * remaining state are kept.  The NIST solver applies it at every iteration start while
+ * the geostatic step action is active (nonlinearimplicitstatic.py:413-414). */
+int efe_set_geostatic(efe_ctx* ctx, const double def[6], const int32_t* elements, int32_t n);
+
 /* The assembled K (before Dirichlet elimination only if no Dirichlet set is active)
  * in CSR value order (device [csr_nnz], FP64). */
 int efe_export_csr_values(efe_ctx* ctx, double* values);

@@ -162,6 +162,7 @@ class HexElements:
         self.nn = nodes_per_element(el_type)
         self.nDof = 3 * self.nn
         self.conn = np.asarray(conn)
+        self.coords = np.asarray(coords, dtype=float)
         X = coords[self.conn]  # (E, nn, 3)
         self.J, self.detJ, self.dNdx = geometry(self.el_type, X)
         self.B = b_operator(self.dNdx)
@@ -194,6 +195,11 @@ class HexElements:
         Pe = -np.einsum("eqki,eqk,eq->ei", B, st[:, :, 0:6], f)
         st[:, :, 6:12] += dEps
         return Ke, Pe, st
+
+    def qp_coordinates(self):
+        """Physical coordinates (E, Q, 3) of the QPs at the points where J and B are evaluated."""
+        N = shape_functions(self.nn, evaluation_points(self.el_type))  # (Q, nn)
+        return np.einsum("qa,eac->eqc", N, self.coords[self.conn])
 
     def body_force(self, load):
         """computeBodyForce for all elements (element.py:348-371)."""

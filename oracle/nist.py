@@ -117,6 +117,9 @@ class Step:
     dirichlets: list = field(default_factory=list)   # dict(nset=str, values={comp: value})
     nodeforces: list = field(default_factory=list)   # dict(nset=str, vector=(3,))
     bodyforces: list = field(default_factory=list)   # dict(elset=str, vector=(3,))
+    # dict(p1, h1, p2, h2, xLateral, zLateral, elset="all"): geostatic initial stress,
+    # stepactions/geostatic.py:41-101
+    geostatic: list = field(default_factory=list)
 
 
 class Model:
@@ -189,18 +192,35 @@ class NIST:
         return V, P, F, st
 
     def external_loads(self, model, step, ts):
+        """Loads start + vector * stepProgress; `start` (default 0) is the load a previous step
+        left (nodeforces.py:136-152, bodyforce.py)."""
         PExt = np.zeros(model.nDof)
         for nf in step.nodeforces:
             nodes = model.nodesets[nf["nset"]]
             idx = (3 * np.asarray(nodes)[:, None] + np.arange(3)[None, :]).ravel()
-            load = np.tile(np.asarray(nf["vector"], dtype=float), (len(nodes), 1)) * ts.stepProgress
-            PExt[idx] += load.ravel()
+            vec = np.asarray(nf.get("start", 0.0), dtype=float) + np.asarray(nf["vector"], dtype=float) * ts.stepProgress
+            PExt[idx] += np.tile(vec, (len(nodes), 1)).ravel()
         for bf in step.bodyforces:
             els = model.elsets[bf["elset"]]
-            force = np.asarray(bf["vector"], dtype=float) * ts.stepProgress
+            force = np.asarray(bf.get("start", 0.0), dtype=float) + np.asarray(bf["vector"], dtype=float) * ts.stepProgress
             Pe = model.elements.body_force(force)[els]
             np.add.at(PExt, self.dof_map[els].ravel(), Pe.ravel())
         return PExt
+
+    @staticmethod
+    def apply_geostatic(model, g):
+        """Geostatic initial stress into the reference state of the element set, at every
+        iteration start (nonlinearimplicitstatic.py:413-414 -> geostatic.py:94-101 ->
+        setInitialCondition, marmotelement/element.pyx:203-213: Temp = Ref, set, Ref = Temp).
+        The stress rule is the one of Marmot's displacement elements: at each QP,
+        sigma_yy = p1 + (p2 - p1) (y - h1) / (h2 - h1), sigma_xx = xLateral sigma_yy,
+        sigma_zz = zLateral sigma_yy; shear stresses and the other state stay."""
+        els = model.elsets[g.get("elset", "all")]
+        y = model.elements.qp_coordinates()[els, :, 1]
+        sy = g["p1"] + (g["p2"] - g["p1"]) * (y - g["h1"]) / (g["h2"] - g["h1"])
+        model.state[els, :, 0] = g.get("xLateral", 1.0) * sy
+        model.state[els, :, 1] = sy
+        model.state[els, :, 2] = g.get("zLateral", 1.0) * sy
 
     def apply_dirichlet_K(self, K, dir_idx):
         if dir_idx.size:
@@ -238,6 +258,8 @@ class NIST:
             extrapolated = False
             dU = np.zeros_like(dU)
         while True:
+            for g in step.geostatic:
+                self.apply_geostatic(model, g)
             U_np = U_n + dU
             V, P, F, st = self.compute_elements(model, dU)
             PExt = self.external_loads(model, step, ts)
@@ -296,4 +318,7 @@ class NIST:
         model.U = U
         self.P = P
         self.dU = dU
+        if step.geostatic:
+            # end of a geostatic step: displacements are reset (geostatic.py:81-92)
+            model.U = np.zeros_like(U)
         return U

@@ -370,3 +370,83 @@ def test_nist_j2_per_element_lognormal_yield_matches_oracle(precision):
     assert rel_err(pm.U, om.U) < tol
     assert rel_err(pm.qpState[..., 0:6], om.state[..., 0:6]) < 10 * tol
     assert rel_err(pm.qpState[..., 12], kap) < 100 * tol
+
+
+GEO = dict(p1=0.0, h1=1.0, p2=-10.0, h2=0.0, xLateral=0.5, zLateral=0.7)
+ROLLERS = [("bottom", "gen_bottom", {1: 0.0}), ("left", "gen_left", {0: 0.0}), ("right", "gen_right", {0: 0.0}),
+           ("back", "gen_back", {2: 0.0}), ("front", "gen_front", {2: 0.0})]
+
+
+@pytest.mark.parametrize("el_type", ["C3D8", "C3D20R", "C3D20"])
+def test_geostatic_initial_state_matches_oracle(el_type):
+    from oracle import nist as onist
+
+    n, l = (3, 2, 2), (1.0, 2.0, 1.0)
+    om = oracle_model(n, l, el_type, "druckerprager", DP, jitter=0.1, seed=5)
+    pm = device_model(n, l, el_type, "druckerprager", DP, jitter=0.1, seed=5)
+    sysm = _system(pm)
+    sub = np.array([0, 2, 5, 7])
+    om.elsets["sub"] = sub
+    onist.NIST.apply_geostatic(om, dict(GEO, h1=2.0, elset="sub"))
+    sysm.set_geostatic([GEO["p1"], 2.0, GEO["p2"], GEO["h2"], GEO["xLateral"], GEO["zLateral"]], sub)
+    got = sysm.qp_state(0)
+    assert np.abs(got[sub, :, 0:3]).min() > 0
+    np.testing.assert_allclose(got[..., 0:6], om.state[..., 0:6], rtol=1e-14, atol=1e-12)
+    onist.NIST.apply_geostatic(om, dict(GEO))
+    sysm.set_geostatic([GEO[k] for k in ("p1", "h1", "p2", "h2", "xLateral", "zLateral")])
+    np.testing.assert_allclose(sysm.qp_state(0)[..., 0:6], om.state[..., 0:6], rtol=1e-14, atol=1e-12)
+
+
+def _footing_nodes(coords, top):
+    x, z = coords[top, 0], coords[top, 2]
+    return top[(np.abs(x - 0.5) <= 0.25 + 1e-12) & (np.abs(z - 0.5) <= 0.25 + 1e-12)]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_geostatic_then_footing_drucker_prager_matches_oracle(precision):
+    """The non-symmetric workload's two steps at 3^3: geostatic prestress in equilibrium with
+    gravity (displacements reset afterwards), then a rigid footing pressed into the top-face
+    centre quarter; GMRES + ILU(0) on the device, SuperLU in the oracle."""
+    from edelweissfe_amd.solvers import NISTGPU
+    from edelweissfe_amd.steps import AdaptiveStep, BodyForce, Dirichlet, Geostatic
+    from oracle import nist as onist
+
+    n, l = (3, 3, 3), (1.0, 1.0, 1.0)
+    slope = (GEO["p2"] - GEO["p1"]) / (GEO["h2"] - GEO["h1"])
+    om = oracle_model(n, l, "C3D20R", "druckerprager", DP, jitter=0.1, seed=3)
+    pm = device_model(n, l, "C3D20R", "druckerprager", DP, jitter=0.1, seed=3)
+    foot = _footing_nodes(pm.coordinates, pm.nodeSets["gen_top"])
+    assert foot.size == 8
+    om.nodesets["footing"] = foot
+    pm.nodeSets["footing"] = foot
+    footing = [("footing", "footing", {1: -4e-3})]
+    kw1 = dict(maxInc=1.0, minInc=1e-6, maxNumInc=100, maxIter=15)
+    kw2 = dict(maxInc=0.25, minInc=1e-6, maxNumInc=100, maxIter=15)
+
+    so = onist.NIST(convention="colmajor")
+    st1 = oracle_step(ROLLERS, bodyforces=[("all", (0.0, 0.0, 0.0))], **kw1)
+    st1.bodyforces[0]["start"] = (0.0, -slope, 0.0)
+    st1.geostatic = [dict(GEO)]
+    so.solve_step(om, st1)
+    st2 = oracle_step(ROLLERS + footing, bodyforces=[("all", (0.0, 0.0, 0.0))], **kw2)
+    st2.bodyforces[0]["start"] = (0.0, -slope, 0.0)
+    so.solve_step(om, st2)
+
+    grav = BodyForce("gravity", "all", (0.0, -slope, 0.0), amplitude=lambda t: 1.0)
+    s = NISTGPU(precision=precision, linsolver="gmres")
+    a1 = {"dirichlet": {nm: Dirichlet(nm, ns, v) for nm, ns, v in ROLLERS}, "bodyforce": {"gravity": grav},
+          "geostatic": {"geo": Geostatic("geo", **GEO)}}
+    s.solveStep(AdaptiveStep(1, pm, a1, **kw1), pm)
+    assert np.abs(pm.U).max() == 0.0
+    a2 = {"dirichlet": {nm: Dirichlet(nm, ns, v) for nm, ns, v in ROLLERS + footing},
+          "bodyforce": {"gravity": grav}}
+    s.solveStep(AdaptiveStep(2, pm, a2, **kw2), pm)
+
+    assert (om.state[..., 12] > 0).any()
+    print([h[2] for h in so.history], [r["iters"] for r in s.history])
+    if precision == "fp64":
+        assert [h[2] for h in so.history] == [r["iters"] for r in s.history]
+        assert rel_err(pm.U, om.U) < 1e-9
+        assert rel_err(pm.qpState[..., 0:6], om.state[..., 0:6]) < 1e-8
+    else:
+        assert rel_err(pm.U, om.U) < 1e-8
