@@ -282,6 +282,149 @@ def bench_batched(args):
     print(json.dumps(result), flush=True)
 
 
+DP = [30000.0, 0.2, 30.0, 10.0, 10.0, 0.0]
+GEO = dict(p1=0.0, h1=1.0, p2=-10.0, h2=0.0, xLateral=0.5, zLateral=0.5)
+
+
+def footing_case(n, increments, settlement):
+    """n^3 C3D20R unit cube (10 % jitter, seed 0), non-associative Drucker-Prager (friction 30,
+    dilation 10, cohesion 10, E 30000, nu 0.2), rollers on the sides and the bottom.  Step 1:
+    geostatic prestress sigma_yy = -10 (1 - y), K0 = 0.5, with the matching gravity
+    b_y = -10 (displacements reset at its end).  Step 2: rigid footing, u_y = -settlement on
+    the top-face nodes with |x - 1/2| <= 1/4 and |z - 1/2| <= 1/4, in `increments` increments."""
+    from edelweissfe_amd.generators.boxgen import generateBox
+    from edelweissfe_amd.steps import AdaptiveStep, BodyForce, Dirichlet, Geostatic
+
+    model = generateBox(n, n, n, 1.0, 1.0, 1.0, elType="C3D20R", jitter=0.1, seed=0)
+    model.assignMaterial("druckerprager", DP)
+    top = model.nodeSets["gen_top"]
+    x, z = model.coordinates[top, 0], model.coordinates[top, 2]
+    model.nodeSets["footing"] = top[(np.abs(x - 0.5) <= 0.25 + 1e-12) & (np.abs(z - 0.5) <= 0.25 + 1e-12)]
+    rollers = [("bottom", "gen_bottom", {1: 0.0}), ("left", "gen_left", {0: 0.0}), ("right", "gen_right", {0: 0.0}),
+               ("back", "gen_back", {2: 0.0}), ("front", "gen_front", {2: 0.0})]
+    slope = (GEO["p2"] - GEO["p1"]) / (GEO["h2"] - GEO["h1"])
+
+    def steps_for(m):
+        grav = BodyForce("gravity", "all", (0.0, -slope, 0.0), amplitude=lambda t: 1.0)
+        a1 = {"dirichlet": {nm: Dirichlet(nm, ns, v) for nm, ns, v in rollers}, "bodyforce": {"gravity": grav},
+              "geostatic": {"geo": Geostatic("geo", **GEO)}}
+        a2 = {"dirichlet": {nm: Dirichlet(nm, ns, v) for nm, ns, v in
+                            rollers + [("footing", "footing", {1: -settlement})]},
+              "bodyforce": {"gravity": grav}}
+        kw = dict(minInc=1e-6, maxNumInc=1000, maxIter=25)
+        return [lambda: AdaptiveStep(1, m, a1, maxInc=1.0, **kw),
+                lambda: AdaptiveStep(2, m, a2, maxInc=1.0 / increments, **kw)]
+
+    return model, steps_for
+
+
+def run_footing(system, template, steps_for, precision, steps, warmup):
+    import torch
+
+    from edelweissfe_amd.solvers import NISTGPU
+
+    times, solvers, models = [], [], []
+    for k in range(warmup + steps):
+        m = fresh_model(template)
+        s = NISTGPU(precision=precision, linsolver="gmres", preconditioner="ilu0", gmresRestart=50)
+        s.attach(system)
+        makers = steps_for(m)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for make in makers:
+            s.solveStep(make(), m)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        log("[bench] footing %s %s: %.3f s, Newton %s, solves %d, GMRES its %d" % (
+            precision, "warmup" if k < warmup else "timed", dt, [r["iters"] for r in s.history], s.linearSolves,
+            s.krylovIterations))
+        if k >= warmup:
+            times.append(dt)
+            solvers.append(s)
+            models.append(m)
+    return times, solvers, models
+
+
+def bench_footing(args):
+    import torch
+
+    from edelweissfe_amd.device import DeviceSystem
+
+    n, inc, settle = args.n, args.increments, args.settlement
+    log("[bench] building %d^3 C3D20R Drucker-Prager footing model" % n)
+    template, steps_for = footing_case(n, inc, settle)
+    t0 = time.perf_counter()
+    system = DeviceSystem(template.coordinates, template.connectivity, "C3D20R", "druckerprager", DP)
+    torch.cuda.synchronize()
+    log("[bench] device system ready: %d DOFs, %d nnz (%.1f s)" % (system.n_dof, system.nnz, time.perf_counter() - t0))
+    t64, s64, m64 = run_footing(system, template, steps_for, "fp64", args.steps, args.warmup)
+    ph64 = system.phase_times()
+    q64 = m64[-1].qpState
+    spmv_ms, spmv_alg = spmv_roofline(system)
+    D = system.n_dof
+    wall64 = sum(t64) / len(t64)
+    nis64 = s64[-1].linearSolves / wall64
+    result = {
+        "metric": METRIC,
+        "value": nis64,
+        "unit": "Newton iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * wall64,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "%d^3 hexa20 C3D20R (2x2x2 Gauss) unit cube, 10%% jitter, non-associative "
+                               "Drucker-Prager (phi 30, psi 10, c 10, E 30000, nu 0.2); geostatic prestress step "
+                               "with gravity, then rigid footing u_y = -%g on the top-face centre quarter in %d "
+                               "increments; GMRES(50) + multicolor block ILU(0)" % (n, settle, inc),
+                   "elements": int(template.nElements), "dofs": int(D), "csr_nnz": int(system.nnz)},
+        "dof_updates_per_s": nis64 * D,
+        "newton_iterations_per_history": s64[-1].linearSolves,
+        "newton_iterations_per_increment": [r["iters"] for r in s64[-1].history],
+        "gmres_iterations_per_history": s64[-1].krylovIterations,
+        "phase_ms": ph64,
+    }
+    if not args.no_mixed:
+        t32, s32, m32 = run_footing(system, template, steps_for, "mixed", args.steps, args.warmup)
+        wall32 = sum(t32) / len(t32)
+        u64, u32 = m64[-1].U, m32[-1].U
+        q32 = m32[-1].qpState
+        result["mixed"] = {
+            "newton_iterations_per_s": s32[-1].linearSolves / wall32,
+            "dof_updates_per_s": s32[-1].linearSolves / wall32 * D,
+            "ms_per_history": 1000.0 * wall32,
+            "history_speedup_vs_fp64": wall64 / wall32,
+            "newton_iterations_per_history": s32[-1].linearSolves,
+            "newton_iterations_per_increment": [r["iters"] for r in s32[-1].history],
+            "gmres_iterations_per_history": s32[-1].krylovIterations,
+            "error_vs_fp64": {
+                "U_rel_inf": float(np.abs(u32 - u64).max() / np.abs(u64).max()),
+                "stress_rel_inf": float(np.abs(q32[..., 0:6] - q64[..., 0:6]).max() / np.abs(q64[..., 0:6]).max()),
+                "kappa_rel_inf": float(np.abs(q32[..., 12] - q64[..., 12]).max()
+                                       / max(np.abs(q64[..., 12]).max(), 1e-300)),
+            },
+            "phase_ms": system.phase_times(),
+        }
+        log("[bench] footing mixed: %.3f s (%.2fx), U rel err %.2e" % (
+            wall32, wall64 / wall32, result["mixed"]["error_vs_fp64"]["U_rel_inf"]))
+    result["roofline"] = {
+        "kernel": "spmv_kernel<double> (sliced-ELL 3x3 blocks)",
+        "bound": "hbm",
+        "achieved": spmv_alg / (spmv_ms * 1e-3) / 1e9,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": spmv_alg / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "traffic": None,
+        "algorithmic_bytes": spmv_alg,
+        "avg_launch_ms": spmv_ms,
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -294,9 +437,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--profile-spmv", action="store_true", help="counter-collection mode (no JSON line)")
-    ap.add_argument("--workload", choices=("cube", "lognormal", "batched"), default="cube",
+    ap.add_argument("--settlement", type=float, default=0.005)
+    ap.add_argument("--workload", choices=("cube", "lognormal", "batched", "footing"), default="cube",
                     help="cube: 64^3 J2 load history (default); lognormal: 96^3 J2 with per-element lognormal "
-                         "yield stress, 30 increments to 2 %% strain; batched: 512 uneven J2 specimens")
+                         "yield stress, 30 increments to 2 %% strain; batched: 512 uneven J2 specimens; "
+                         "footing: 48^3 C3D20R Drucker-Prager geostatic + rigid footing, GMRES + ILU(0)")
     ap.add_argument("--specimens", type=int, default=512)
     ap.add_argument("--batch-increments", type=int, default=10)
     args = ap.parse_args()
@@ -306,6 +451,13 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         bench_batched(args)
+        return
+    if args.workload == "footing":
+        if args.n == 64:
+            args.n = 48
+        if args.increments == 20:
+            args.increments = 25
+        bench_footing(args)
         return
     lognormal = args.workload == "lognormal"
     if lognormal:

@@ -106,48 +106,62 @@ __global__ __launch_bounds__(BS) void ilu_copy_kernel(int N, const int32_t* __re
 }
 
 // Rows [r0, r1) of one color: IKJ block ILU(0).  Rows k < i in elimination order are final.
-template <typename T>
+// A group of G lanes works on one row; the row's block at position p belongs to lane
+// (p - b0) % G, which alone reads and writes it.  For each k in the row's lower part
+// (in order) the owner of (i, k) forms L_ik = A_ik Dinv_k and broadcasts it; every lane
+// then updates its own blocks (i, j), j after k, for which (k, j) is in row k's upper part
+// (binary search: a row's columns are sorted in elimination order).
+template <typename T, int G>
 __global__ __launch_bounds__(BS) void ilu_factor_kernel(int r0, int r1, const int32_t* __restrict__ fptr,
                                                         const int32_t* __restrict__ fcol,
                                                         const int32_t* __restrict__ dslot, T* __restrict__ F,
                                                         T* __restrict__ Dinv) {
-    const int i = r0 + blockIdx.x * BS + threadIdx.x;
-    if (i >= r1) return;
+    const int tid = blockIdx.x * BS + threadIdx.x;
+    const int i = r0 + tid / G, lane = tid % G;
+    if (i >= r1) return;  // a whole group leaves together
     const int b0 = fptr[i], b1 = fptr[i + 1], d = b0 + dslot[i];
     for (int kb = b0; kb < d; ++kb) {
         const int k = fcol[kb];
-        T a[9], lik[9];
+        const int owner = (kb - b0) % G;
+        T lik[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (lane == owner) {
+            T a[9];
 #pragma unroll
-        for (int v = 0; v < 9; ++v) a[v] = F[(int64_t)kb * 9 + v];
-        mat3mul(a, Dinv + (int64_t)k * 9, lik);
+            for (int v = 0; v < 9; ++v) a[v] = F[(int64_t)kb * 9 + v];
+            mat3mul(a, Dinv + (int64_t)k * 9, lik);
 #pragma unroll
-        for (int v = 0; v < 9; ++v) F[(int64_t)kb * 9 + v] = lik[v];
-        int jb = kb + 1, kj = fptr[k] + dslot[k] + 1;
-        const int ke = fptr[k + 1];
-        while (jb < b1 && kj < ke) {
-            const int cj = fcol[jb], ck = fcol[kj];
-            if (cj == ck) {
+            for (int v = 0; v < 9; ++v) F[(int64_t)kb * 9 + v] = lik[v];
+        }
+#pragma unroll
+        for (int v = 0; v < 9; ++v) lik[v] = __shfl(lik[v], owner, G);
+        const int u0 = fptr[k] + dslot[k] + 1, u1 = fptr[k + 1];
+        const int first = kb + 1 + ((lane - (kb + 1 - b0) % G) % G + G) % G;  // first owned position > kb
+        for (int jb = first; jb < b1; jb += G) {
+            const int cj = fcol[jb];
+            int lo = u0, hi = u1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (fcol[mid] < cj) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo < u1 && fcol[lo] == cj) {
                 T u[9], p[9];
 #pragma unroll
-                for (int v = 0; v < 9; ++v) u[v] = F[(int64_t)kj * 9 + v];
+                for (int v = 0; v < 9; ++v) u[v] = F[(int64_t)lo * 9 + v];
                 mat3mul(lik, u, p);
 #pragma unroll
                 for (int v = 0; v < 9; ++v) F[(int64_t)jb * 9 + v] -= p[v];
-                ++jb;
-                ++kj;
-            } else if (cj < ck) {
-                ++jb;
-            } else {
-                ++kj;
             }
         }
     }
-    double dd[9], di[9];
+    if (lane == (d - b0) % G) {
+        double dd[9], di[9];
 #pragma unroll
-    for (int v = 0; v < 9; ++v) dd[v] = (double)F[(int64_t)d * 9 + v];
-    inv3d(dd, di);
+        for (int v = 0; v < 9; ++v) dd[v] = (double)F[(int64_t)d * 9 + v];
+        inv3d(dd, di);
 #pragma unroll
-    for (int v = 0; v < 9; ++v) Dinv[(int64_t)i * 9 + v] = (T)di[v];
+        for (int v = 0; v < 9; ++v) Dinv[(int64_t)i * 9 + v] = (T)di[v];
+    }
 }
 
 // t = perm(v): t[factor row i] = v[node perm[i]]
@@ -173,49 +187,74 @@ __global__ void permute_out_kernel(int N, const int32_t* __restrict__ perm, cons
     v[3 * n + 2] = t[3 * i + 2];
 }
 
-// forward sweep of one color: t_i -= sum_{k before i} L_ik t_k (unit block-lower)
-template <typename T>
+// Sum of the 3-vectors of a group of G lanes (fixed butterfly order).
+template <typename T, int G>
+__device__ __forceinline__ void group_sum3(T& a, T& b, T& c) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, G);
+        b += __shfl_xor(b, off, G);
+        c += __shfl_xor(c, off, G);
+    }
+}
+
+// forward sweep of one color: t_i -= sum_{k before i} L_ik t_k (unit block-lower);
+// G lanes per row, each over every G-th block of the row's lower part
+template <typename T, int G>
 __global__ __launch_bounds__(BS) void ilu_lower_kernel(int r0, int r1, const int32_t* __restrict__ fptr,
                                                        const int32_t* __restrict__ fcol,
                                                        const int32_t* __restrict__ dslot, const T* __restrict__ F,
                                                        T* __restrict__ t) {
-    const int i = r0 + blockIdx.x * BS + threadIdx.x;
+    const int tid = blockIdx.x * BS + threadIdx.x;
+    const int i = r0 + tid / G, lane = tid % G;
     if (i >= r1) return;
-    T y0 = t[3 * i], y1 = t[3 * i + 1], y2 = t[3 * i + 2];
-    for (int kb = fptr[i]; kb < fptr[i] + dslot[i]; ++kb) {
+    const int b0 = fptr[i], d = b0 + dslot[i];
+    T y0 = 0, y1 = 0, y2 = 0;
+    for (int kb = b0 + lane; kb < d; kb += G) {
         const int k = fcol[kb];
         const T* a = F + (int64_t)kb * 9;
         const T x0 = t[3 * k], x1 = t[3 * k + 1], x2 = t[3 * k + 2];
-        y0 -= a[0] * x0 + a[1] * x1 + a[2] * x2;
-        y1 -= a[3] * x0 + a[4] * x1 + a[5] * x2;
-        y2 -= a[6] * x0 + a[7] * x1 + a[8] * x2;
+        y0 += a[0] * x0 + a[1] * x1 + a[2] * x2;
+        y1 += a[3] * x0 + a[4] * x1 + a[5] * x2;
+        y2 += a[6] * x0 + a[7] * x1 + a[8] * x2;
     }
-    t[3 * i] = y0;
-    t[3 * i + 1] = y1;
-    t[3 * i + 2] = y2;
+    group_sum3<T, G>(y0, y1, y2);
+    if (lane == 0) {
+        t[3 * i] -= y0;
+        t[3 * i + 1] -= y1;
+        t[3 * i + 2] -= y2;
+    }
 }
 
 // backward sweep of one color: t_i = Dinv_i (t_i - sum_{j after i} U_ij t_j)
-template <typename T>
+template <typename T, int G>
 __global__ __launch_bounds__(BS) void ilu_upper_kernel(int r0, int r1, const int32_t* __restrict__ fptr,
                                                        const int32_t* __restrict__ fcol,
                                                        const int32_t* __restrict__ dslot, const T* __restrict__ F,
                                                        const T* __restrict__ Dinv, T* __restrict__ t) {
-    const int i = r0 + blockIdx.x * BS + threadIdx.x;
+    const int tid = blockIdx.x * BS + threadIdx.x;
+    const int i = r0 + tid / G, lane = tid % G;
     if (i >= r1) return;
-    T y0 = t[3 * i], y1 = t[3 * i + 1], y2 = t[3 * i + 2];
-    for (int jb = fptr[i] + dslot[i] + 1; jb < fptr[i + 1]; ++jb) {
+    const int e = fptr[i + 1];
+    T y0 = 0, y1 = 0, y2 = 0;
+    for (int jb = fptr[i] + dslot[i] + 1 + lane; jb < e; jb += G) {
         const int j = fcol[jb];
         const T* a = F + (int64_t)jb * 9;
         const T x0 = t[3 * j], x1 = t[3 * j + 1], x2 = t[3 * j + 2];
-        y0 -= a[0] * x0 + a[1] * x1 + a[2] * x2;
-        y1 -= a[3] * x0 + a[4] * x1 + a[5] * x2;
-        y2 -= a[6] * x0 + a[7] * x1 + a[8] * x2;
+        y0 += a[0] * x0 + a[1] * x1 + a[2] * x2;
+        y1 += a[3] * x0 + a[4] * x1 + a[5] * x2;
+        y2 += a[6] * x0 + a[7] * x1 + a[8] * x2;
     }
-    const T* m = Dinv + (int64_t)i * 9;
-    t[3 * i] = m[0] * y0 + m[1] * y1 + m[2] * y2;
-    t[3 * i + 1] = m[3] * y0 + m[4] * y1 + m[5] * y2;
-    t[3 * i + 2] = m[6] * y0 + m[7] * y1 + m[8] * y2;
+    group_sum3<T, G>(y0, y1, y2);
+    if (lane == 0) {
+        y0 = t[3 * i] - y0;
+        y1 = t[3 * i + 1] - y1;
+        y2 = t[3 * i + 2] - y2;
+        const T* m = Dinv + (int64_t)i * 9;
+        t[3 * i] = m[0] * y0 + m[1] * y1 + m[2] * y2;
+        t[3 * i + 1] = m[3] * y0 + m[4] * y1 + m[5] * y2;
+        t[3 * i + 2] = m[6] * y0 + m[7] * y1 + m[8] * y2;
+    }
 }
 
 // z = Minv v (block Jacobi)
@@ -380,7 +419,12 @@ int ilu_factor(efe_ctx* c, GmresWork* g, const T* K, T*& F, T*& Dinv, int pi, hi
     ilu_copy_kernel<T><<<grid_of(c->N), BS, 0, st>>>(c->N, c->bptr, c->slice_off, K, g->fidx, F);
     for (int col = 0; col < g->ncolors; ++col) {
         const int r0 = g->cstart[col], r1 = g->cstart[col + 1];
-        if (r1 > r0) ilu_factor_kernel<T><<<grid_of(r1 - r0), BS, 0, st>>>(r0, r1, g->fptr, g->fcol, g->dslot, F, Dinv);
+        if (r1 <= r0) continue;
+        if (c->NN == 8)
+            ilu_factor_kernel<T, 8><<<grid_of((r1 - r0) * 8), BS, 0, st>>>(r0, r1, g->fptr, g->fcol, g->dslot, F, Dinv);
+        else
+            ilu_factor_kernel<T, 16><<<grid_of((r1 - r0) * 16), BS, 0, st>>>(r0, r1, g->fptr, g->fcol, g->dslot, F,
+                                                                             Dinv);
     }
     EFE_HIP(hipGetLastError());
     g->factored_version[pi] = c->Kversion;
@@ -395,14 +439,25 @@ void apply_precond(efe_ctx* c, GmresWork* g, int precond, const T* F, const T* D
     if (precond == EFE_PRECOND_ILU0) {
         T* t = (T*)g->t;
         permute_in_kernel<T><<<grid_of(N), BS, 0, st>>>(N, g->perm, v, t);
+        const bool h8 = c->NN == 8;
         for (int col = 0; col < g->ncolors; ++col) {
             const int r0 = g->cstart[col], r1 = g->cstart[col + 1];
-            if (r1 > r0) ilu_lower_kernel<T><<<grid_of(r1 - r0), BS, 0, st>>>(r0, r1, g->fptr, g->fcol, g->dslot, F, t);
+            if (r1 <= r0) continue;
+            if (h8)
+                ilu_lower_kernel<T, 8><<<grid_of((r1 - r0) * 8), BS, 0, st>>>(r0, r1, g->fptr, g->fcol, g->dslot, F, t);
+            else
+                ilu_lower_kernel<T, 16><<<grid_of((r1 - r0) * 16), BS, 0, st>>>(r0, r1, g->fptr, g->fcol, g->dslot, F,
+                                                                                t);
         }
         for (int col = g->ncolors - 1; col >= 0; --col) {
             const int r0 = g->cstart[col], r1 = g->cstart[col + 1];
-            if (r1 > r0)
-                ilu_upper_kernel<T><<<grid_of(r1 - r0), BS, 0, st>>>(r0, r1, g->fptr, g->fcol, g->dslot, F, Dinv, t);
+            if (r1 <= r0) continue;
+            if (h8)
+                ilu_upper_kernel<T, 8><<<grid_of((r1 - r0) * 8), BS, 0, st>>>(r0, r1, g->fptr, g->fcol, g->dslot, F,
+                                                                              Dinv, t);
+            else
+                ilu_upper_kernel<T, 16><<<grid_of((r1 - r0) * 16), BS, 0, st>>>(r0, r1, g->fptr, g->fcol, g->dslot,
+                                                                                F, Dinv, t);
         }
         permute_out_kernel<T><<<grid_of(N), BS, 0, st>>>(N, g->perm, t, z);
     } else {
