@@ -422,6 +422,22 @@ def bench_footing(args):
         "algorithmic_bytes": spmv_alg,
         "avg_launch_ms": spmv_ms,
     }
+    if not args.no_cpu:
+        try:
+            from oracle import cpu_baseline
+
+            log("[bench] CPU baseline (bounded sample)")
+            P, F = system.zeros(), system.zeros()
+            system.compute_elements(system.zeros(), P, F)  # FP64 operator with the footing step's constraints
+            data = system.export_csr_values().cpu().numpy()
+            indptr, indices = system.csr_pattern()
+            counts = {"element_passes": s64[-1].elementPasses, "solves": s64[-1].linearSolves,
+                      "krylov_iterations": s64[-1].krylovIterations}
+            result["cpu_baseline"] = cpu_baseline.measure_footing(template.coordinates, template.connectivity, DP,
+                                                                  indptr, indices, data, counts,
+                                                                  budget_s=args.cpu_seconds, log=log)
+        except Exception as e:
+            result["cpu_baseline"] = {"error": repr(e)}
     print(json.dumps(result), flush=True)
 
 

@@ -26,6 +26,10 @@ def load():
         lib.oracle_x_map.argtypes = [ctypes.c_int, ctypes.c_int, _P, _P, _P, _P]
         lib.oracle_update_csr.restype = None
         lib.oracle_update_csr.argtypes = [ctypes.c_int64, _P, _P, _P, ctypes.c_int64]
+        lib.oracle_ilu0.restype = ctypes.c_int
+        lib.oracle_ilu0.argtypes = [ctypes.c_int, _P, _P, _P, _P, ctypes.c_double]
+        lib.oracle_gmres_phase_times.restype = None
+        lib.oracle_gmres_phase_times.argtypes = [ctypes.c_int, _P, _P, _P, _P, _P, ctypes.c_int, ctypes.c_int, _P]
         lib.oracle_gather_pf.restype = None
         lib.oracle_gather_pf.argtypes = [ctypes.c_int, ctypes.c_int, _P, _P, _P, _P, ctypes.c_int]
         lib.oracle_pcg.restype = None
@@ -109,3 +113,20 @@ def pcg(indptr, indices, data, b, rtol, atol, max_iter, budget_s):
 
 def threads():
     return load().oracle_threads()
+
+
+def ilu0(indptr, indices, data, budget_s):
+    """Scalar ILU(0) of the leading rows within budget_s: returns (lu, diag, rows done)."""
+    lib = load()
+    lu = np.array(data, dtype=np.float64)
+    diag = np.empty(indptr.size - 1, dtype=np.int32)
+    rows = lib.oracle_ilu0(indptr.size - 1, _p(indptr), _p(indices), _p(lu), _p(diag), budget_s)
+    return lu, diag, rows
+
+
+def gmres_phase_times(indptr, indices, data, lu, diag, j, reps):
+    lib = load()
+    out = np.zeros(4)
+    lib.oracle_gmres_phase_times(indptr.size - 1, _p(indptr), _p(indices), _p(data), _p(lu), _p(diag), j, reps,
+                                 _p(out))
+    return {"spmv": out[0], "ilu_apply": out[1], "gram_schmidt": out[2], "normalize": out[3]}

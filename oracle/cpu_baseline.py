@@ -137,3 +137,71 @@ def measure_batched(sizes, passes, solves, krylov, props, strain, increments, bu
                        "element passes, solves and Krylov iterations of the GPU run" % (
                            nref, r["threads"], r["its"], 1e3 * r["t_it"])),
             "phase_seconds": _phases(r), "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
+def measure_footing(coords, conn, props, indptr, indices, data, counts, restart=50, sample_elements=2048,
+                    budget_s=20.0, log=print):
+    """CPU estimate of the hexa20 Drucker-Prager footing history with GMRES(m) + ILU(0).
+
+    Phases timed on the host: the element pass (numpy restatement, DP, C3D20R) and the
+    COO->CSR scatter (C) on a sample of elements, scaled to all elements; the scalar ILU(0)
+    factorisation (C, serial) of the leading rows within a time budget, scaled to all rows;
+    the kernels of one GMRES iteration (C: OpenMP CSR SpMV, serial ILU(0) substitutions,
+    classical Gram-Schmidt against restart/2 basis vectors, normalisation) on the full
+    operator.  The operator is the device's assembled K (same pattern and values).  The
+    phases are weighted with the GPU run's element passes, solves (one factorisation each)
+    and GMRES iterations."""
+    from . import element as oel
+    from . import materials as omat
+
+    t0 = time.perf_counter()
+    rng = np.random.default_rng(0)
+    E = conn.shape[0]
+    S = min(sample_elements, E)
+    sel = np.sort(rng.choice(E, S, replace=False))
+    els = oel.HexElements("C3D20R", coords, conn[sel])
+    mat = omat.DruckerPrager(np.asarray(props, dtype=np.float64))
+    state = np.zeros((S, els.nQP, 13))
+    dofmap = els.dof_map.astype(np.int32)
+    dU = rng.uniform(-1e-4, 1e-4, 3 * coords.shape[0])
+    t = time.perf_counter()
+    Ke, Pe, _ = els.compute(dU[dofmap], state, mat)
+    t_el = (time.perf_counter() - t) * E / S
+    from . import assembly
+
+    V = np.ascontiguousarray(assembly.vij_values_colmajor(Ke))
+    x = clib.x_map(dofmap, indptr, indices)
+    t = time.perf_counter()
+    clib.update_csr(V, x, indices.size)
+    t_scatter = (time.perf_counter() - t) * E / S
+    n = indptr.size - 1
+    t = time.perf_counter()
+    lu, diag, rows = clib.ilu0(indptr, indices, data, budget_s * 0.5)
+    t_rows = time.perf_counter() - t
+    t_fac = t_rows * n / max(rows, 1)
+    if rows < n:
+        # substitutions timed on the factor's pattern with bounded values (unit diagonal,
+        # small off-diagonals): the same memory traffic and arithmetic as the real factor
+        rowlen = np.diff(indptr).astype(np.float64)
+        lu = np.repeat(0.5 / rowlen, np.diff(indptr))
+        lu[diag] = 1.0
+    ph = clib.gmres_phase_times(indptr, indices, data, lu, diag, restart // 2, 3)
+    t_it = ph["spmv"] + ph["ilu_apply"] + ph["gram_schmidt"] + ph["normalize"]
+    threads = clib.threads()
+    hist = counts["element_passes"] * (t_el + t_scatter) + counts["solves"] * t_fac + counts["krylov_iterations"] * t_it
+    log("[cpu] footing: elements %.2f s, scatter %.2f s per pass (sample %d), ILU(0) %.2f s (%d/%d rows timed), "
+        "GMRES iteration %.1f ms (SpMV %.1f, ILU apply %.1f, CGS %.1f), %d threads, setup+sample %.1f s" % (
+            t_el, t_scatter, S, t_fac, rows, n, 1e3 * t_it, 1e3 * ph["spmv"], 1e3 * ph["ilu_apply"],
+            1e3 * ph["gram_schmidt"], threads, time.perf_counter() - t0))
+    return {"value": counts["solves"] / hist, "unit": "Newton iterations/s", "cores": threads, "kind": "port",
+            "history_seconds_estimate": hist,
+            "sample": ("footing history phases on %d OpenMP threads (C/numpy restatement): element pass on %d "
+                       "sampled elements scaled to %d, scalar ILU(0) of %d of %d rows scaled, one GMRES(%d) iteration "
+                       "(SpMV, serial ILU(0) substitutions, CGS against %d vectors) on the device-assembled operator; "
+                       "weighted with the GPU run's %d element passes, %d factorisations, %d GMRES iterations; "
+                       "not the reference direct solver" % (threads, S, E, rows, n, restart, restart // 2,
+                                                            counts["element_passes"], counts["solves"],
+                                                            counts["krylov_iterations"])),
+            "phase_seconds": {"elements_per_pass": t_el, "scatter_per_pass": t_scatter, "ilu0_factor": t_fac,
+                              "gmres_iteration": t_it, **{"gmres_" + k: v for k, v in ph.items()}},
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}

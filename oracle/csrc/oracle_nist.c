@@ -349,3 +349,95 @@ void oracle_pcg(int n, const int32_t* ip, const int32_t* ix, const double* a, co
 }
 
 int oracle_threads(void) { return omp_get_max_threads(); }
+
+/* ---- GMRES(m) + ILU(0) phases on the CPU (timed baseline of the non-symmetric solve) ----
+ * Scalar ILU(0) in natural row order, IKJ variant (Saad, Iterative Methods for Sparse
+ * Linear Systems, 2nd ed., Algorithm 10.4), on a CSR matrix with sorted column indices
+ * and a nonzero diagonal; lu holds a copy of the values and is factorised in place.
+ * Rows are factorised in order until budget_s has elapsed; returns the rows done. */
+int oracle_ilu0(int n, const int32_t* ip, const int32_t* ix, double* lu, int32_t* diag, double budget_s) {
+    double t0 = omp_get_wtime();
+    for (int i = 0; i < n; ++i) {
+        diag[i] = -1;
+        for (int32_t p = ip[i]; p < ip[i + 1]; ++p)
+            if (ix[p] == i) diag[i] = p;
+    }
+    int i = 0;
+    for (; i < n; ++i) {
+        if ((i & 1023) == 0 && omp_get_wtime() - t0 > budget_s) break;
+        for (int32_t kb = ip[i]; kb < ip[i + 1] && ix[kb] < i; ++kb) {
+            const int k = ix[kb];
+            lu[kb] /= lu[diag[k]];
+            int32_t jb = kb + 1, kj = diag[k] + 1;
+            while (jb < ip[i + 1] && kj < ip[k + 1]) {
+                if (ix[jb] == ix[kj]) {
+                    lu[jb] -= lu[kb] * lu[kj];
+                    ++jb;
+                    ++kj;
+                } else if (ix[jb] < ix[kj]) {
+                    ++jb;
+                } else {
+                    ++kj;
+                }
+            }
+        }
+    }
+    return i;
+}
+
+/* z = (LU)^{-1} r: forward substitution with the unit lower factor, backward with the upper. */
+static void ilu0_apply(int n, const int32_t* ip, const int32_t* ix, const double* lu, const int32_t* diag,
+                       const double* r, double* z) {
+    for (int i = 0; i < n; ++i) {
+        double s = r[i];
+        for (int32_t p = ip[i]; p < diag[i]; ++p) s -= lu[p] * z[ix[p]];
+        z[i] = s;
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double s = z[i];
+        for (int32_t p = diag[i] + 1; p < ip[i + 1]; ++p) s -= lu[p] * z[ix[p]];
+        z[i] = s / lu[diag[i]];
+    }
+}
+
+/* Seconds per call of the kernels of one GMRES iteration, averaged over reps calls:
+ * out[0] y = A x (OpenMP CSR), out[1] ILU(0) application (serial substitutions),
+ * out[2] classical Gram-Schmidt of one vector against j basis vectors (OpenMP dots
+ * and one combined update), out[3] the norm and scaling of the new basis vector. */
+void oracle_gmres_phase_times(int n, const int32_t* ip, const int32_t* ix, const double* a, const double* lu,
+                              const int32_t* diag, int j, int reps, double* out) {
+    double* x = (double*)malloc(sizeof(double) * n);
+    double* y = (double*)malloc(sizeof(double) * n);
+    double* V = (double*)malloc(sizeof(double) * (size_t)n * (j + 1));
+    double* h = (double*)malloc(sizeof(double) * (j + 1));
+    for (int i = 0; i < n; ++i) x[i] = 1.0 / (1.0 + i % 7);
+    for (size_t i = 0; i < (size_t)n * (j + 1); ++i) V[i] = 1.0 / (1.0 + i % 11);
+    double t = omp_get_wtime();
+    for (int r = 0; r < reps; ++r) spmv(n, ip, ix, a, x, y);
+    out[0] = (omp_get_wtime() - t) / reps;
+    t = omp_get_wtime();
+    for (int r = 0; r < reps; ++r) ilu0_apply(n, ip, ix, lu, diag, y, x);
+    out[1] = (omp_get_wtime() - t) / reps;
+    t = omp_get_wtime();
+    for (int r = 0; r < reps; ++r) {
+        for (int q = 0; q < j; ++q) h[q] = dot(n, V + (size_t)q * n, y);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i) {
+            double s = y[i];
+            for (int q = 0; q < j; ++q) s -= h[q] * V[(size_t)q * n + i];
+            y[i] = s;
+        }
+    }
+    out[2] = (omp_get_wtime() - t) / reps;
+    t = omp_get_wtime();
+    for (int r = 0; r < reps; ++r) {
+        const double nr = sqrt(dot(n, y, y));
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i) V[(size_t)j * n + i] = y[i] / (nr > 0 ? nr : 1.0);
+    }
+    out[3] = (omp_get_wtime() - t) / reps;
+    free(x);
+    free(y);
+    free(V);
+    free(h);
+}

@@ -191,7 +191,7 @@ class HexElements:
         st[:, :, 12:] = mstate.reshape(E, Q, -1)
         C = C.reshape(E, Q, 6, 6)
         f = detJ * self.w[None, :]
-        Ke = np.einsum("eqki,eqkl,eqlj,eq->eij", B, C, B, f)
+        Ke = np.einsum("eqki,eqkl,eqlj,eq->eij", B, C, B, f, optimize=True)
         Pe = -np.einsum("eqki,eqk,eq->ei", B, st[:, :, 0:6], f)
         st[:, :, 6:12] += dEps
         return Ke, Pe, st
