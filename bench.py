@@ -241,7 +241,7 @@ def bench_batched(args):
     if not args.no_mixed:
         s32 = NISTBatched(precision="mixed")
         s32.attach(s64)
-        s32.tuning = autotune.tune(sysm, _lib.EFE_FP32)
+        s32.tuning = autotune.tune(sysm, _lib.EFE_FP32_MATFREE)
         t32, m32, _ = run_batched(s32, count, args.strain, inc, args.steps, args.warmup)
         wall32 = sum(t32) / len(t32)
         q32 = sysm.qp_state(0)

@@ -18,7 +18,7 @@ EFE_NOT_CONVERGED = 3
 
 EFE_C3D8, EFE_C3D20, EFE_C3D20R = 0, 1, 2
 EFE_LINEAR_ELASTIC, EFE_VONMISES, EFE_DRUCKER_PRAGER = 0, 1, 2
-EFE_FP64, EFE_FP32 = 0, 1
+EFE_FP64, EFE_FP32, EFE_FP32_MATFREE = 0, 1, 2
 EFE_SOLVER_CG, EFE_SOLVER_GMRES = 0, 1
 EFE_PRECOND_BLOCK_JACOBI, EFE_PRECOND_ILU0 = 0, 1
 

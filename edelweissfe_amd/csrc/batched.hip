@@ -25,6 +25,7 @@ struct BatchWork {
     hipEvent_t evA = nullptr, evB = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_batch = 0;
+    int graph_op = -1;  // precision code the captured graph was built for (operator choice)
 };
 
 namespace {
@@ -236,7 +237,11 @@ void launch_b_batch(efe_ctx* c, BatchWork* w, hipStream_t st, const T* K, const 
     const int N = c->N, S = c->nspec;
     T *x = (T*)w->x, *r = (T*)w->r, *z = (T*)w->z, *p = (T*)w->p, *q = (T*)w->q;
     for (int j = 0; j < batch; ++j) {
-        b_spmv_kernel<T><<<gridn(N), BS, 0, st>>>(N, c->specOfNode, w->active, c->slice_off, c->scol, K, p, q, w->nv0);
+        if (K)
+            b_spmv_kernel<T><<<gridn(N), BS, 0, st>>>(N, c->specOfNode, w->active, c->slice_off, c->scol, K, p, q,
+                                                      w->nv0);
+        else  // matrix-free FP32 operator
+            efe_mf_launch_operator_batched(c, st, (const float*)p, (float*)q, w->nv0, w->active);
         b_alpha_kernel<<<S, BS, 0, st>>>(c->specNodeOff, w->nv0, w->sc, w->active);
         b_update_kernel<T><<<gridn(N), BS, 0, st>>>(N, c->specOfNode, w->active, w->sc, Minv, x, r, z, p, q, w->nv0,
                                                      w->nv1);
@@ -304,7 +309,7 @@ int batched_pcg(efe_ctx* c, const double* b, double* xout, const efe_solve_opts*
     EFE_HIP(hipGetLastError());
     int batch = o->check_every > 0 ? o->check_every : 16;
     const int maxit = o->max_iter > 0 ? o->max_iter : 10000;
-    if (!w->graph || w->graph_batch != batch) {
+    if (!w->graph || w->graph_batch != batch || w->graph_op != o->precision) {
         if (w->graph) hipGraphExecDestroy(w->graph);
         w->graph = nullptr;
         hipGraph_t g;
@@ -314,6 +319,7 @@ int batched_pcg(efe_ctx* c, const double* b, double* xout, const efe_solve_opts*
         EFE_HIP(hipGraphInstantiate(&w->graph, g, nullptr, nullptr, 0));
         EFE_HIP(hipGraphDestroy(g));
         w->graph_batch = batch;
+        w->graph_op = o->precision;
     }
     int it = 0;
     info->status = EFE_OK;
@@ -367,8 +373,11 @@ int efe_batched_linsolve(efe_ctx* c, const double* b, double* x, const efe_solve
         efe_set_error("batched solves use block-Jacobi PCG");
         return EFE_ERR_ARG;
     }
-    int rc = ensure_batch(c, o->precision);
+    const int vp = o->precision == EFE_FP64 ? EFE_FP64 : EFE_FP32;  // vector precision
+    int rc = ensure_batch(c, vp);
     if (rc) return rc;
+    if (o->precision == EFE_FP32_MATFREE)
+        return batched_pcg<float>(c, b, x, o, spec_tol, spec_active, spec_iters, info, nullptr, c->Minv32);
     if (o->precision == EFE_FP32)
         return batched_pcg<float>(c, b, x, o, spec_tol, spec_active, spec_iters, info, c->K32, c->Minv32);
     return batched_pcg<double>(c, b, x, o, spec_tol, spec_active, spec_iters, info, c->K64, c->Minv64);

@@ -258,6 +258,7 @@ void efe_ctx_destroy(efe_ctx* c) {
     efe_krylov_free(c);
     efe_gmres_free(c);
     efe_batched_free(c);
+    efe_mf_free(c);
     void* bufs[] = {c->shapeNe, c->geoElems,
                     c->coords, c->conn,  c->props,  c->dNdxi,  c->shapeN, c->qpw,    c->stateRef, c->stateTmp,
                     c->Kscr,   c->Pscr,  c->bptr,   c->bcol,   c->coff,   c->clist,  c->nptr,     c->nlist,
@@ -319,7 +320,8 @@ int efe_compute_elements(efe_ctx* c, const double* U, const double* dU, const do
     (void)time;
     (void)dT;
     if (int rc = check(c)) return rc;
-    if (!dU || !P || !F || (dir_values && !dir_correction) || (precision != EFE_FP64 && precision != EFE_FP32)) {
+    if (!dU || !P || !F || (dir_values && !dir_correction) ||
+        (precision != EFE_FP64 && precision != EFE_FP32 && precision != EFE_FP32_MATFREE)) {
         efe_set_error("efe_compute_elements: invalid arguments");
         return EFE_ERR_ARG;
     }
@@ -336,11 +338,17 @@ int efe_compute_elements(efe_ctx* c, const double* U, const double* dU, const do
         EFE_HIP(hipMemsetAsync(c->K32, 0, nval * sizeof(float), c->stream));
         EFE_HIP(hipMalloc(&c->Minv32, (size_t)c->N * 9 * sizeof(float)));
     }
+    if (precision == EFE_FP32_MATFREE)
+        if (int rc = efe_mf_alloc(c)) return rc;
     EFE_HIP(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
     EFE_HIP(hipEventRecord(c->ev[0], c->stream));
     if (int rc = efe_launch_elements(c, dU, precision)) return rc;
     EFE_HIP(hipEventRecord(c->ev[1], c->stream));
-    if (int rc = efe_launch_assembly(c, P, F, dir_values, dir_correction, precision)) return rc;
+    if (precision == EFE_FP32_MATFREE) {
+        if (int rc = efe_launch_mf_assembly(c, P, F, dir_values, dir_correction)) return rc;
+    } else if (int rc = efe_launch_assembly(c, P, F, dir_values, dir_correction, precision)) {
+        return rc;
+    }
     EFE_HIP(hipEventRecord(c->ev[2], c->stream));
     int hs = 0;
     EFE_HIP(hipMemcpyAsync(&hs, c->status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -376,7 +384,7 @@ int efe_set_geostatic(efe_ctx* c, const double def[6], const int32_t* elements, 
 
 int efe_export_csr_values(efe_ctx* c, double* values) {
     if (int rc = check(c)) return rc;
-    if (c->Kprec < 0) {
+    if (c->Kprec < 0 || c->Kprec == EFE_FP32_MATFREE) {
         efe_set_error("no assembled operator");
         return EFE_ERR_STATE;
     }
@@ -385,6 +393,11 @@ int efe_export_csr_values(efe_ctx* c, double* values) {
 
 int efe_spmv(efe_ctx* c, const double* x, double* y) {
     if (int rc = check(c)) return rc;
+    if (!x || !y) {
+        efe_set_error("efe_spmv: invalid arguments");
+        return EFE_ERR_ARG;
+    }
+    if (c->Kprec == EFE_FP32_MATFREE) return efe_launch_spmv_mf(c, x, y);
     return efe_launch_spmv64(c, x, y);
 }
 

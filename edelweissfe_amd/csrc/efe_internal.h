@@ -73,6 +73,10 @@ struct efe_ctx {
     double* dNdxi = nullptr;     // [Q][NN][3] at the evaluation points
     double* shapeN = nullptr;    // [Q][NN] at the table points (body force)
     double* shapeNe = nullptr;   // [Q][NN] at the evaluation points (QP coordinates)
+    // matrix-free FP32 operator: [Q][NC][E] compact tangent, [E*NN][3] node-sorted force slots
+    float *mfC = nullptr, *mfF = nullptr, *dnxi32 = nullptr;
+    int32_t* mfPos = nullptr;        // [NN][E] slot of (e, a) in the node-sorted incidence list
+    bool mfPass = false;             // the current element launch is a matrix-free pass
     int32_t* geoElems = nullptr; // [E] element subset of a geostatic initial state
     double* qpw = nullptr;       // [Q]
     double* stateRef = nullptr;  // [nstate][E][Q]
@@ -122,6 +126,15 @@ int efe_launch_assembly(efe_ctx* c, double* P, double* F, const double* dir_valu
                         int precision);
 int efe_launch_body_force(efe_ctx* c, const double* f3, double* PExt);
 int efe_launch_geostatic(efe_ctx* c, const double* def, const int32_t* elements, int32_t n);
+// matrix-free FP32 operator (matfree.hip)
+int efe_mf_alloc(efe_ctx* c);
+int efe_launch_mf_assembly(efe_ctx* c, double* P, double* F, const double* dir_values, double* dir_correction);
+void efe_mf_launch_operator(efe_ctx* c, hipStream_t st, const float* p, float* q, double* part, const int* flags,
+                            int nwg, int bs);
+void efe_mf_free(efe_ctx* c);
+int efe_launch_spmv_mf(efe_ctx* c, const double* x, double* y);
+void efe_mf_launch_operator_batched(efe_ctx* c, hipStream_t st, const float* p, float* q, double* npq,
+                                    const int* active);
 int efe_launch_export_csr(efe_ctx* c, double* values);
 
 // Krylov (krylov.hip)

@@ -30,7 +30,10 @@ struct MatState {
     static constexpr int n = (MAT == EFE_LINEAR_ELASTIC) ? 0 : 1;
 };
 
-template <int NN, int Q, int MAT, typename KT, int EPW>
+// MF (matrix-free form, KT = float): phase 1 also stores the compact tangent
+// w detJ [A, B, cnn, (cn1, c1n,) n] per QP for the matrix-free operator, and phase 2 forms
+// only the diagonal blocks K_aa (for block Jacobi).
+template <int NN, int Q, int MAT, typename KT, int EPW, bool MF>
 __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double* __restrict__ coords,
                                                      const int32_t* __restrict__ conn,
                                                      const double* __restrict__ props, int propStride,
@@ -39,9 +42,10 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
                                                      const double* __restrict__ dU,
                                                      const double* __restrict__ sRef, double* __restrict__ sTmp,
                                                      KT* __restrict__ Kscr, double* __restrict__ Pscr,
-                                                     int* __restrict__ status) {
+                                                     int* __restrict__ status, float* __restrict__ mfC) {
     constexpr int NS = 12 + MatState<MAT>::n;
     constexpr int NP = (MAT == EFE_LINEAR_ELASTIC) ? 2 : 6;
+    constexpr int NC = (MAT == EFE_DRUCKER_PRAGER) ? 11 : 9;
     __shared__ double g_s[EPW][Q][NN][3];
     __shared__ KT fC_s[EPW][Q][36];
     __shared__ double fs_s[EPW][Q][6];
@@ -108,12 +112,20 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
         double p[NP];
 #pragma unroll
         for (int k = 0; k < NP; ++k) p[k] = propStride ? props[(int64_t)k * propStride + e] : props[k];
-        double C[36];
+        double C[36], cc[11];
         int st = 0;
-        if (MAT == EFE_LINEAR_ELASTIC) st = efe::le_update(p, s, de, C);
-        else if (MAT == EFE_VONMISES) st = efe::vm_update(p, s, ms, de, C);
-        else st = efe::dp_update(p, s, ms, de, C);
+        if (MAT == EFE_LINEAR_ELASTIC) st = efe::le_update(p, s, de, C, MF ? cc : nullptr);
+        else if (MAT == EFE_VONMISES) st = efe::vm_update(p, s, ms, de, C, MF ? cc : nullptr);
+        else st = efe::dp_update(p, s, ms, de, C, MF ? cc : nullptr);
         if (st) *status = 1;
+        if (MF) {
+            const double fq = det * qpw[q];
+            float* dc = mfC + (int64_t)q * NC * E + e;
+#pragma unroll
+            for (int k = 0; k < NC - 6; ++k) dc[(int64_t)k * E] = (float)(fq * cc[k]);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) dc[(int64_t)(NC - 6 + k) * E] = (float)cc[5 + k];
+        }
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             sTmp[c * EQ + sidx] = s[c];
@@ -129,9 +141,10 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
     }
     __syncthreads();
 
-    for (int t = lane; t < EPW * NN * NN; t += 64) {
-        const int el = t / (NN * NN);
-        const int ab = t - el * (NN * NN);
+    constexpr int NB = MF ? NN : NN * NN;  // blocks per element formed here
+    for (int t = lane; t < EPW * NB; t += 64) {
+        const int el = t / NB;
+        const int ab = MF ? (t - el * NB) * (NN + 1) : t - el * NB;
         const int a = ab / NN, b = ab - (ab / NN) * NN;
         const int e = e0 + el;
         if (e >= E) continue;
@@ -352,9 +365,14 @@ template <int NN, int Q, int MAT, typename KT, int EPW>
 void launch_elem_t(efe_ctx* c, const double* dU) {
     const int grid = (c->E + EPW - 1) / EPW;
     const int propStride = c->props_per_element ? c->E : 0;
-    element_kernel<NN, Q, MAT, KT, EPW><<<grid, 64, 0, c->stream>>>(
-        c->E, c->N, c->coords, c->conn, c->props, propStride, c->dNdxi, c->qpw, dU, c->stateRef, c->stateTmp,
-        (KT*)c->Kscr, c->Pscr, c->status);
+    if (c->mfPass)
+        element_kernel<NN, Q, MAT, float, EPW, true><<<grid, 64, 0, c->stream>>>(
+            c->E, c->N, c->coords, c->conn, c->props, propStride, c->dNdxi, c->qpw, dU, c->stateRef, c->stateTmp,
+            (float*)c->Kscr, c->Pscr, c->status, c->mfC);
+    else
+        element_kernel<NN, Q, MAT, KT, EPW, false><<<grid, 64, 0, c->stream>>>(
+            c->E, c->N, c->coords, c->conn, c->props, propStride, c->dNdxi, c->qpw, dU, c->stateRef, c->stateTmp,
+            (KT*)c->Kscr, c->Pscr, c->status, nullptr);
 }
 
 template <int NN, int Q, int MAT, typename KT>
@@ -402,7 +420,8 @@ int launch_elem_type(efe_ctx* c, const double* dU) {
 }  // namespace
 
 int efe_launch_elements(efe_ctx* c, const double* dU, int precision) {
-    int rc = (precision == EFE_FP32) ? launch_elem_type<float>(c, dU) : launch_elem_type<double>(c, dU);
+    c->mfPass = precision == EFE_FP32_MATFREE;
+    int rc = (precision == EFE_FP64) ? launch_elem_type<double>(c, dU) : launch_elem_type<float>(c, dU);
     if (rc != EFE_OK) return rc;
     EFE_HIP(hipGetLastError());
     return EFE_OK;

@@ -36,9 +36,9 @@ struct KrylovWork {
     double* hpart = nullptr; // pinned host scalars
     hipStream_t ks = nullptr;
     hipEvent_t evA = nullptr, evB = nullptr, t0 = nullptr, t1 = nullptr;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
-    int graph_batch[2] = {0, 0};
-    int graph_block[2] = {0, 0};
+    hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};  // FP64, FP32 assembled, FP32 matrix-free
+    int graph_batch[3] = {0, 0, 0};
+    int graph_block[3] = {0, 0, 0};
 };
 
 namespace {
@@ -403,7 +403,10 @@ template <typename T, int BS>
 void launch_iter(efe_ctx* c, KrylovWork* w, hipStream_t st, T* x, T* r, T* z, T* p, T* q, const T* K,
                  const T* Minv, int par, int j) {
     const int grid = w->nwg;
-    spmv_kernel<T, BS><<<grid, BS, 0, st>>>(c->N, c->slice_off, c->scol, K, p, q, w->part, w->flags);
+    if (K)
+        spmv_kernel<T, BS><<<grid, BS, 0, st>>>(c->N, c->slice_off, c->scol, K, p, q, w->part, w->flags);
+    else  // matrix-free FP32 operator
+        efe_mf_launch_operator(c, st, (const float*)p, (float*)q, w->part, w->flags, grid, BS);
     pcg_update_kernel<T, BS><<<grid, BS, 0, st>>>(c->N, Minv, x, r, z, p, q, w->part, grid, par, w->flags);
     pcg_direction_kernel<T, BS><<<grid, BS, 0, st>>>(c->N, z, p, w->part, grid, par, j, w->dscal, w->flags);
 }
@@ -461,7 +464,7 @@ int ensure_work(efe_ctx* c) {
         EFE_HIP(hipEventCreate(&w->t0));
         EFE_HIP(hipEventCreate(&w->t1));
     }
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < 3; ++i) {
         if (w->graph[i]) hipGraphExecDestroy(w->graph[i]);
         w->graph[i] = nullptr;
     }
@@ -564,6 +567,10 @@ int efe_krylov_solve(efe_ctx* c, const double* b, double* x, const efe_solve_opt
         efe_set_error("solve precision differs from the precision K was assembled in");
         return EFE_ERR_ARG;
     }
+    if (o->precision == EFE_FP32_MATFREE && (o->solver != EFE_SOLVER_CG || o->precond != EFE_PRECOND_BLOCK_JACOBI)) {
+        efe_set_error("the matrix-free operator runs with block-Jacobi CG");
+        return EFE_ERR_ARG;
+    }
     if (o->solver == EFE_SOLVER_GMRES) return efe_gmres_solve(c, b, x, o, info);
     if (o->solver != EFE_SOLVER_CG || o->precond != EFE_PRECOND_BLOCK_JACOBI) {
         efe_set_error("CG runs with the block-Jacobi preconditioner (ILU(0) is available with GMRES)");
@@ -572,6 +579,8 @@ int efe_krylov_solve(efe_ctx* c, const double* b, double* x, const efe_solve_opt
     int rc = ensure_work(c);
     if (rc != EFE_OK) return rc;
     KrylovWork* w = c->kw;
+    if (o->precision == EFE_FP32_MATFREE)
+        return pcg_solve<float>(c, b, x, o, info, w->x32, w->r32, w->z32, w->p32, w->q32, nullptr, c->Minv32, 2);
     if (o->precision == EFE_FP32)
         return pcg_solve<float>(c, b, x, o, info, w->x32, w->r32, w->z32, w->p32, w->q32, c->K32, c->Minv32, 1);
     return pcg_solve<double>(c, b, x, o, info, w->x64, w->r64, w->z64, w->p64, w->q64, c->K64, c->Minv64, 0);
@@ -604,7 +613,7 @@ int efe_launch_norms(efe_ctx* c, const double* R, const double* ddU, const doubl
 void efe_krylov_free(efe_ctx* c) {
     KrylovWork* w = c->kw;
     if (!w) return;
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 3; ++i)
         if (w->graph[i]) hipGraphExecDestroy(w->graph[i]);
     hipFree(w->x64); hipFree(w->r64); hipFree(w->z64); hipFree(w->p64); hipFree(w->q64);
     hipFree(w->x32); hipFree(w->r32); hipFree(w->z32); hipFree(w->p32); hipFree(w->q32);

@@ -1,5 +1,12 @@
 // Quadrature-point constitutive updates, FP64, Voigt order 11,22,33,12,13,23 with
 // engineering shear strains.
+//
+// Every tangent of these materials has the form
+//   C = A 1(x)1 + B Idev + cnn n(x)n + cn1 n(x)1 + c1n 1(x)n
+// with 1 = (1,1,1,0,0,0), Idev the Voigt deviatoric projector (diagonal 2/3, 2/3, 2/3,
+// 1/2, 1/2, 1/2, normal off-diagonal -1/3) and n a unit deviatoric direction in stress
+// components.  When cc is given, the update also returns [A, B, cnn, cn1, c1n, n0..n5]:
+// the 11 numbers from which the matrix-free operator rebuilds C.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -18,9 +25,19 @@ __device__ __forceinline__ void elastic_matrix(double E, double v, double C[36])
     C[21] = s; C[28] = s; C[35] = s;
 }
 
+__device__ __forceinline__ void compact_elastic(double E, double v, double* cc) {
+    if (!cc) return;
+    cc[0] = E / (3.0 * (1.0 - 2.0 * v));  // K
+    cc[1] = E / (1.0 + v);                // 2G
+#pragma unroll
+    for (int i = 2; i < 11; ++i) cc[i] = 0.0;
+}
+
 // LinearElasticMaterial.computeStress (linearelastic.py:185-210): C = Ei, stress += Ei dEps.
-__device__ __forceinline__ int le_update(const double* p, double s[6], const double de[6], double C[36]) {
+__device__ __forceinline__ int le_update(const double* p, double s[6], const double de[6], double C[36],
+                                         double* cc = nullptr) {
     elastic_matrix(p[0], p[1], C);
+    compact_elastic(p[0], p[1], cc);
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
         double acc = 0.0;
@@ -36,10 +53,11 @@ __device__ __forceinline__ int le_update(const double* p, double s[6], const dou
 // tolerance 1e-12, at most 15 steps, vonmises.py:55-56,229-236).  Returns 1 when
 // the local Newton fails (the reference raises CutbackRequest(.., 0.5)).
 __device__ __forceinline__ int vm_update(const double* p, double s[6], double& kappa, const double de[6],
-                                         double C[36]) {
+                                         double C[36], double* cc = nullptr) {
     const double E = p[0], v = p[1], fy0 = p[2], H = p[3], dfy = p[4], delta = p[5];
     const double G = E / (2.0 * (1.0 + v));
     elastic_matrix(E, v, C);
+    compact_elastic(E, v, cc);
     const double nde = sqrt(de[0] * de[0] + de[1] * de[1] + de[2] * de[2] + de[3] * de[3] + de[4] * de[4] +
                             de[5] * de[5]);
     if (nde < 1e-14) return 0;  // stress untouched, elastic tangent (vonmises.py:211-213)
@@ -89,6 +107,12 @@ __device__ __forceinline__ int vm_update(const double* p, double s[6], double& k
     const double hk = H + dfy * delta * exp(-delta * kappa);
     const double a = 2.0 * G * (1.0 / (1.0 + hk / (3.0 * G)) - 2.0 * G * dLambda / devNorm);
     const double b = 4.0 * G * G * dLambda / devNorm;
+    if (cc) {  // C = K 1(x)1 + (2G - b) Idev - a n(x)n
+        cc[1] = 2.0 * G - b;
+        cc[2] = -a;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) cc[5 + i] = n[i];
+    }
     const double idh[6] = {c23, c23, c23, 0.5, 0.5, 0.5};
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
@@ -108,7 +132,7 @@ __device__ __forceinline__ int vm_update(const double* p, double s[6], double& k
 // consistent tangents (Box 8.9 and the apex tangent).  p = tr(sigma)/3, tension positive.
 // props [E, nu, phi_deg, psi_deg, c0, H]; state: equivalent plastic strain e.
 __device__ __forceinline__ int dp_update(const double* p, double s[6], double& e, const double de[6],
-                                         double C[36]) {
+                                         double C[36], double* cc = nullptr) {
     const double E = p[0], v = p[1], c0 = p[4], H = p[5];
     const double pi = 3.14159265358979323846;
     const double ph = p[2] * pi / 180.0, ps = p[3] * pi / 180.0;
@@ -118,6 +142,7 @@ __device__ __forceinline__ int dp_update(const double* p, double s[6], double& e
     const double etab = 6.0 * sin(ps) / (r3 * (3.0 - sin(ps)));
     const double G = E / (2.0 * (1.0 + v)), K = E / (3.0 * (1.0 - 2.0 * v));
     elastic_matrix(E, v, C);
+    compact_elastic(E, v, cc);
     double t[6];
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
@@ -165,6 +190,15 @@ __device__ __forceinline__ int dp_update(const double* p, double s[6], double& e
                                t4 * one[i] * one[j];
             }
         }
+        if (cc) {
+            cc[0] = t4;
+            cc[1] = t1;
+            cc[2] = t2;
+            cc[3] = -t3 * eta;
+            cc[4] = -t3 * etab;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) cc[5 + i] = n[i];
+        }
     } else {
         const double alpha = xi / etab, beta = xi / eta;
         const double dv = (ptr - beta * coh) / (alpha * beta * H + K);
@@ -177,6 +211,10 @@ __device__ __forceinline__ int dp_update(const double* p, double s[6], double& e
         for (int i = 0; i < 6; ++i)
 #pragma unroll
             for (int j = 0; j < 6; ++j) C[i * 6 + j] = ka * one[i] * one[j];
+        if (cc) {
+            cc[0] = ka;
+            cc[1] = 0.0;
+        }
     }
     return 0;
 }

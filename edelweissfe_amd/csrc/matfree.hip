@@ -1,0 +1,408 @@
+// Matrix-free FP32 operator for the mixed-precision form (EFE_FP32_MATFREE).
+//
+// The element pass stores, per element and QP, the compact tangent
+// w detJ [A, B, cnn, (cn1, c1n,) n0..n5] (9 floats; 11 for Drucker-Prager) of
+// C = A 1(x)1 + B Idev + cnn n(x)n + cn1 n(x)1 + c1n 1(x)n, in [QP][field][element] order
+// so that consecutive lanes read consecutive elements.  y = K p is then, per element (one
+// thread):
+//   J   = sum_a dN_a/dxi (x) (x_a - x_0)              (rebuilt: cheaper than storing J^-1)
+//   H   = sum_a p_a (x) dN_a/dxi                      (3x3)
+//   eps = sym(H J^-1)                                 (engineering Voigt)
+//   sig = C eps                                       (compact form, ~30 flops)
+//   M_ik = sum_j sig_ij J^-1_jk
+//   f_a = M dN_a/dxi                                  -> node-sorted force slots
+// followed by a per-node sum of its slots in element order (the order of the assembled
+// path's row sums), so the product is deterministic.  Dirichlet DOFs are
+// eliminated symmetrically, as in the assembled operator: p is read as 0 on them and their
+// rows return p (unit diagonal).  The same two kernels with the prescribed values as p and
+// no elimination give the right-hand-side correction K[:, dir] dir_values.
+//
+// Algorithmic bytes per application: 36..44 per element-QP pair (tangent), 4 per
+// element-node (connectivity) and 2 x 4 per element-node (slot index, force slot written and
+// read), plus the node vectors; against 9 x 4 + 4 bytes per 3x3 block of the assembled FP32
+// operator.  It also removes the FP32 assembly from every Newton iteration.
+#include <vector>
+
+#include "efe_internal.h"
+
+namespace {
+
+__device__ __forceinline__ int xcd_block_mf() {
+    const int per = gridDim.x >> 3;
+    return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+
+template <int BS>
+__device__ __forceinline__ double block_sum_mf(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = BS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// One thread per element: element force f_e = K_e p_e.  J is rebuilt from the node
+// coordinates relative to the element's first node (formed in FP64, so that FP32 keeps the
+// element's own scale), the compact tangent is read per QP, and f_a is written to the
+// node-sorted slot pos[a][e] of the force list, so the gather reads contiguous memory.
+template <int NN, int Q, int NC, typename TP>
+__global__ __launch_bounds__(256) void mf_apply_kernel(int E, int N, const int32_t* __restrict__ conn,
+                                                       const double* __restrict__ coords,
+                                                       const float* __restrict__ dnxi,
+                                                       const float* __restrict__ mfC, const TP* __restrict__ p,
+                                                       const uint8_t* __restrict__ dirflag, int mask,
+                                                       const int* __restrict__ flags,
+                                                       const int32_t* __restrict__ pos, float* __restrict__ Fs,
+                                                       const int* __restrict__ active,
+                                                       const int32_t* __restrict__ specOfElem) {
+    if (flags && flags[0]) return;
+    const int e = xcd_block_mf() * 256 + threadIdx.x;
+    if (e >= E) return;
+    if (active && !active[specOfElem[e]]) return;  // frozen specimen of a batched solve
+    float pe[NN][3], fe[NN][3], xe[NN][3];
+    double x0[3];
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+        const int nd = conn[(int64_t)a * E + e];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            float v = (float)p[3 * nd + i];
+            if (mask && dirflag[3 * nd + i]) v = 0.0f;
+            pe[a][i] = v;
+            fe[a][i] = 0.0f;
+            const double xc = coords[(int64_t)i * N + nd];
+            if (a == 0) x0[i] = xc;
+            xe[a][i] = (float)(xc - x0[i]);
+        }
+    }
+    for (int q = 0; q < Q; ++q) {
+        const float* dn = dnxi + q * NN * 3;
+        float H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // H[i*3+k] = sum_a p_a,i dN_a/dxi_k
+        float J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // J[k*3+c] = sum_a dN_a/dxi_k x_a,c
+#pragma unroll
+        for (int a = 0; a < NN; ++a) {
+            const float d[3] = {dn[a * 3], dn[a * 3 + 1], dn[a * 3 + 2]};
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    H[i * 3 + k] += pe[a][i] * d[k];
+                    J[k * 3 + i] += d[k] * xe[a][i];
+                }
+        }
+        const float det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
+                          J[2] * (J[3] * J[7] - J[4] * J[6]);
+        const float id = 1.0f / det;
+        const float iJ[9] = {(J[4] * J[8] - J[5] * J[7]) * id, (J[2] * J[7] - J[1] * J[8]) * id,
+                             (J[1] * J[5] - J[2] * J[4]) * id, (J[5] * J[6] - J[3] * J[8]) * id,
+                             (J[0] * J[8] - J[2] * J[6]) * id, (J[2] * J[3] - J[0] * J[5]) * id,
+                             (J[3] * J[7] - J[4] * J[6]) * id, (J[1] * J[6] - J[0] * J[7]) * id,
+                             (J[0] * J[4] - J[1] * J[3]) * id};
+        // du_i/dx_j = sum_k H_ik iJ_jk
+        float g[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                g[i * 3 + j] = H[i * 3] * iJ[j * 3] + H[i * 3 + 1] * iJ[j * 3 + 1] + H[i * 3 + 2] * iJ[j * 3 + 2];
+        const float ep[6] = {g[0], g[4], g[8], g[1] + g[3], g[2] + g[6], g[5] + g[7]};
+        float cf[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) cf[k] = mfC[(int64_t)(q * NC + k) * E + e];
+        const float A = cf[0], B = cf[1], cnn = cf[2];
+        const float cn1 = NC == 11 ? cf[3] : 0.0f, c1n = NC == 11 ? cf[4] : 0.0f;
+        const float* n = cf + (NC == 11 ? 5 : 3);
+        const float tr = ep[0] + ep[1] + ep[2];
+        const float ne = n[0] * ep[0] + n[1] * ep[1] + n[2] * ep[2] + n[3] * ep[3] + n[4] * ep[4] + n[5] * ep[5];
+        const float t3 = tr * (1.0f / 3.0f);
+        const float sn = cnn * ne + cn1 * tr;     // coefficient of n
+        const float s1 = A * tr + c1n * ne;       // coefficient of 1
+        float sg[6];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) sg[i] = s1 + B * (ep[i] - t3) + sn * n[i];
+#pragma unroll
+        for (int i = 3; i < 6; ++i) sg[i] = 0.5f * B * ep[i] + sn * n[i];
+        // tensor sigma (Voigt 11,22,33,12,13,23); M_ik = sum_j sig_ij iJ_jk
+        const float S[9] = {sg[0], sg[3], sg[4], sg[3], sg[1], sg[5], sg[4], sg[5], sg[2]};
+        float M[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                M[i * 3 + k] = S[i * 3] * iJ[k] + S[i * 3 + 1] * iJ[3 + k] + S[i * 3 + 2] * iJ[6 + k];
+#pragma unroll
+        for (int a = 0; a < NN; ++a) {
+            const float d0 = dn[a * 3], d1 = dn[a * 3 + 1], d2 = dn[a * 3 + 2];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) fe[a][i] += M[i * 3] * d0 + M[i * 3 + 1] * d1 + M[i * 3 + 2] * d2;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+        float* dst = Fs + (int64_t)pos[(int64_t)a * E + e] * 3;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) dst[i] = fe[a][i];
+    }
+}
+
+// Batched solves: q = K p on the nodes of active specimens and the per-node p.q values
+// that the per-specimen reductions sum; nodes of frozen specimens are left alone (p.q 0).
+__global__ __launch_bounds__(256) void mf_gather_batched_kernel(int N, const int32_t* __restrict__ nptr,
+                                                                const float* __restrict__ Fe,
+                                                                const float* __restrict__ p,
+                                                                const uint8_t* __restrict__ dirflag,
+                                                                const int32_t* __restrict__ spec,
+                                                                const int* __restrict__ active, float* __restrict__ q,
+                                                                double* __restrict__ npq) {
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= N) return;
+    if (!active[spec[n]]) {
+        npq[n] = 0.0;
+        return;
+    }
+    float y0 = 0, y1 = 0, y2 = 0;
+    for (int c = nptr[n]; c < nptr[n + 1]; ++c) {
+        const float* src = Fe + (int64_t)c * 3;
+        y0 += src[0];
+        y1 += src[1];
+        y2 += src[2];
+    }
+    float yv[3] = {y0, y1, y2};
+    double dot = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float pv = p[3 * n + i];
+        if (dirflag[3 * n + i]) yv[i] = pv;
+        q[3 * n + i] = yv[i];
+        dot += (double)pv * yv[i];
+    }
+    npq[n] = dot;
+}
+
+// pos[a][e] = index of (e, a) in the node-sorted incidence list nlist (entries e*NN + a).
+__global__ void mf_pos_kernel(int64_t L, int E, int NN, const int32_t* __restrict__ nlist, int32_t* __restrict__ pos) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= L) return;
+    const int v = nlist[c];
+    const int e = v / NN, a = v - (v / NN) * NN;
+    pos[(int64_t)a * E + e] = (int32_t)c;
+}
+
+// MODE 0 (operator): y = sum of the node's element forces in element order on free DOFs,
+// y = p on Dirichlet DOFs, per-workgroup partials of p.y.  MODE 1 (correction): y = the
+// sum on free DOFs, 0 on Dirichlet DOFs.
+template <int BS, int MODE, typename TP, typename TY>
+__global__ __launch_bounds__(BS) void mf_gather_kernel(int N, const int32_t* __restrict__ nptr,
+                                                       const int32_t* __restrict__ nlist,
+                                                       const float* __restrict__ Fe, const TP* __restrict__ p,
+                                                       const uint8_t* __restrict__ dirflag,
+                                                       TY* __restrict__ y, double* __restrict__ part,
+                                                       const int* __restrict__ flags) {
+    __shared__ double sh[BS];
+    if (flags && flags[0]) return;
+    const int blk = xcd_block_mf();
+    const int n = blk * BS + threadIdx.x;
+    double dot = 0.0;
+    if (n < N) {
+        float y0 = 0, y1 = 0, y2 = 0;
+        for (int c = nptr[n]; c < nptr[n + 1]; ++c) {  // slots in element order
+            const float* src = Fe + (int64_t)c * 3;
+            y0 += src[0];
+            y1 += src[1];
+            y2 += src[2];
+        }
+        float yv[3] = {y0, y1, y2};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const bool d = dirflag[3 * n + i];  // always allocated; not ndir, which a captured graph would freeze
+            if (MODE == 0) {
+                const float pv = (float)p[3 * n + i];
+                if (d) yv[i] = pv;
+                dot += (double)pv * yv[i];
+            } else if (d) {
+                yv[i] = 0.0f;
+            }
+            y[3 * n + i] = (TY)yv[i];
+        }
+    }
+    if (MODE == 0 && part) {
+        const double s = block_sum_mf<BS>(dot, sh);
+        if (threadIdx.x == 0) part[blk] = s;
+    }
+}
+
+// Diagonal 3x3 blocks (element order), symmetric Dirichlet elimination inside the block,
+// block-Jacobi inverse, and the P / F gather of the element residuals.
+__global__ __launch_bounds__(256) void mf_diag_kernel(
+    int N, const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol, const int32_t* __restrict__ coff,
+    const int32_t* __restrict__ clist, const int32_t* __restrict__ nptr, const int32_t* __restrict__ nlist,
+    const float* __restrict__ Kscr, const double* __restrict__ Pscr, const uint8_t* __restrict__ dirflag, int ndir,
+    float* __restrict__ Minv, double* __restrict__ P, double* __restrict__ F) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double dg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = bptr[n]; b < bptr[n + 1]; ++b) {
+        if (bcol[b] != n) continue;
+        float acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int c = coff[b]; c < coff[b + 1]; ++c) {
+            const float* src = Kscr + (int64_t)clist[c] * 9;
+#pragma unroll
+            for (int v = 0; v < 9; ++v) acc[v] += src[v];
+        }
+#pragma unroll
+        for (int v = 0; v < 9; ++v) dg[v] = acc[v];
+    }
+    if (ndir) {
+        const bool fr[3] = {(bool)dirflag[3 * n], (bool)dirflag[3 * n + 1], (bool)dirflag[3 * n + 2]};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                if (fr[i] || fr[j]) dg[i * 3 + j] = (i == j) ? 1.0 : 0.0;
+    }
+    const double det = dg[0] * (dg[4] * dg[8] - dg[5] * dg[7]) - dg[1] * (dg[3] * dg[8] - dg[5] * dg[6]) +
+                       dg[2] * (dg[3] * dg[7] - dg[4] * dg[6]);
+    const double id = 1.0 / det;
+    const double iv[9] = {(dg[4] * dg[8] - dg[5] * dg[7]) * id, (dg[2] * dg[7] - dg[1] * dg[8]) * id,
+                          (dg[1] * dg[5] - dg[2] * dg[4]) * id, (dg[5] * dg[6] - dg[3] * dg[8]) * id,
+                          (dg[0] * dg[8] - dg[2] * dg[6]) * id, (dg[2] * dg[3] - dg[0] * dg[5]) * id,
+                          (dg[3] * dg[7] - dg[4] * dg[6]) * id, (dg[1] * dg[6] - dg[0] * dg[7]) * id,
+                          (dg[0] * dg[4] - dg[1] * dg[3]) * id};
+#pragma unroll
+    for (int v = 0; v < 9; ++v) Minv[(int64_t)n * 9 + v] = (float)iv[v];
+    double p0 = 0, p1 = 0, p2 = 0, f0 = 0, f1 = 0, f2 = 0;
+    for (int c = nptr[n]; c < nptr[n + 1]; ++c) {
+        const double* src = Pscr + (int64_t)nlist[c] * 3;
+        p0 += src[0];
+        p1 += src[1];
+        p2 += src[2];
+        f0 += fabs(src[0]);
+        f1 += fabs(src[1]);
+        f2 += fabs(src[2]);
+    }
+    P[3 * n] = p0;
+    P[3 * n + 1] = p1;
+    P[3 * n + 2] = p2;
+    F[3 * n] = f0;
+    F[3 * n + 1] = f1;
+    F[3 * n + 2] = f2;
+}
+
+inline int round8_mf(int n) { return (n + 7) & ~7; }
+
+#define MF_ARGS \
+    c->E, c->N, c->conn, c->coords, c->dnxi32, c->mfC, p, c->dirflag, mask, flags, c->mfPos, c->mfF, active, \
+        c->specOfElem
+
+template <typename TP>
+void launch_apply(efe_ctx* c, hipStream_t st, const TP* p, int mask, const int* flags,
+                  const int* active = nullptr) {
+    const int grid = round8_mf((c->E + 255) / 256);
+    const bool dp = c->material == EFE_DRUCKER_PRAGER;
+    switch (c->eltype) {
+        case EFE_C3D8:
+            if (dp) mf_apply_kernel<8, 8, 11, TP><<<grid, 256, 0, st>>>(MF_ARGS);
+            else mf_apply_kernel<8, 8, 9, TP><<<grid, 256, 0, st>>>(MF_ARGS);
+            break;
+        case EFE_C3D20:
+            if (dp) mf_apply_kernel<20, 27, 11, TP><<<grid, 256, 0, st>>>(MF_ARGS);
+            else mf_apply_kernel<20, 27, 9, TP><<<grid, 256, 0, st>>>(MF_ARGS);
+            break;
+        default:
+            if (dp) mf_apply_kernel<20, 8, 11, TP><<<grid, 256, 0, st>>>(MF_ARGS);
+            else mf_apply_kernel<20, 8, 9, TP><<<grid, 256, 0, st>>>(MF_ARGS);
+            break;
+    }
+}
+
+template <int BS>
+void launch_gather_op(efe_ctx* c, hipStream_t st, const float* p, float* q, double* part, const int* flags,
+                      int nwg) {
+    mf_gather_kernel<BS, 0, float, float><<<nwg, BS, 0, st>>>(c->N, c->nptr, c->nlist, c->mfF, p, c->dirflag, q,
+                                                              part, flags);
+}
+
+}  // namespace
+
+int efe_mf_alloc(efe_ctx* c) {
+    const int NC = c->material == EFE_DRUCKER_PRAGER ? 11 : 9;
+    if (c->mfC) return EFE_OK;
+    const size_t EQ = (size_t)c->E * c->Q;
+    const int64_t L = (int64_t)c->E * c->NN;
+    EFE_HIP(hipMalloc(&c->mfC, EQ * NC * sizeof(float)));
+    EFE_HIP(hipMalloc(&c->mfF, (size_t)L * 3 * sizeof(float)));
+    EFE_HIP(hipMalloc(&c->mfPos, (size_t)L * sizeof(int32_t)));
+    mf_pos_kernel<<<(unsigned)((L + 255) / 256), 256, 0, c->stream>>>(L, c->E, c->NN, c->nlist, c->mfPos);
+    EFE_HIP(hipGetLastError());
+    if (!c->Minv32) EFE_HIP(hipMalloc(&c->Minv32, (size_t)c->N * 9 * sizeof(float)));
+    // FP32 copy of the natural shape-function derivatives at the evaluation points
+    const size_t nd = (size_t)c->Q * c->NN * 3;
+    std::vector<double> h(nd);
+    std::vector<float> hf(nd);
+    EFE_HIP(hipMemcpy(h.data(), c->dNdxi, nd * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < nd; ++i) hf[i] = (float)h[i];
+    EFE_HIP(hipMalloc(&c->dnxi32, nd * sizeof(float)));
+    EFE_HIP(hipMemcpy(c->dnxi32, hf.data(), nd * sizeof(float), hipMemcpyHostToDevice));
+    return EFE_OK;
+}
+
+// Diagonal blocks, Minv, P, F and (optionally) the right-hand-side correction K[:, dir] dir_values.
+int efe_launch_mf_assembly(efe_ctx* c, double* P, double* F, const double* dir_values, double* dir_correction) {
+    const int grid = (c->N + 255) / 256;
+    mf_diag_kernel<<<grid, 256, 0, c->stream>>>(c->N, c->bptr, c->bcol, c->coff, c->clist, c->nptr, c->nlist,
+                                                (const float*)c->Kscr, c->Pscr, c->dirflag, c->ndir, c->Minv32, P, F);
+    EFE_HIP(hipGetLastError());
+    if (dir_values) {
+        launch_apply<double>(c, c->stream, dir_values, 0, nullptr);
+        mf_gather_kernel<256, 1, double, double><<<round8_mf(grid), 256, 0, c->stream>>>(
+            c->N, c->nptr, c->nlist, c->mfF, dir_values, c->dirflag, dir_correction, nullptr, nullptr);
+        EFE_HIP(hipGetLastError());
+    }
+    c->Kprec = EFE_FP32_MATFREE;
+    c->Kversion++;
+    return EFE_OK;
+}
+
+// q = K p (FP32, Dirichlet eliminated) with per-workgroup partials of p.q, for the PCG:
+// nwg workgroups of bs threads, the layout of the assembled SpMV's partials.
+void efe_mf_launch_operator(efe_ctx* c, hipStream_t st, const float* p, float* q, double* part, const int* flags,
+                            int nwg, int bs) {
+    launch_apply<float>(c, st, p, 1, flags);
+    switch (bs) {
+        case 64: launch_gather_op<64>(c, st, p, q, part, flags, nwg); break;
+        case 128: launch_gather_op<128>(c, st, p, q, part, flags, nwg); break;
+        case 512: launch_gather_op<512>(c, st, p, q, part, flags, nwg); break;
+        default: launch_gather_op<256>(c, st, p, q, part, flags, nwg); break;
+    }
+}
+
+// y = K x for FP64 vectors with the matrix-free FP32 operator (Dirichlet eliminated).
+int efe_launch_spmv_mf(efe_ctx* c, const double* x, double* y) {
+    launch_apply<double>(c, c->stream, x, 1, nullptr);
+    mf_gather_kernel<256, 0, double, double><<<round8_mf((c->N + 255) / 256), 256, 0, c->stream>>>(
+        c->N, c->nptr, c->nlist, c->mfF, x, c->dirflag, y, nullptr, nullptr);
+    EFE_HIP(hipGetLastError());
+    return EFE_OK;
+}
+
+// Batched PCG: q = K p for the active specimens, per-node p.q into npq.
+void efe_mf_launch_operator_batched(efe_ctx* c, hipStream_t st, const float* p, float* q, double* npq,
+                                    const int* active) {
+    launch_apply<float>(c, st, p, 1, nullptr, active);
+    mf_gather_batched_kernel<<<(c->N + 255) / 256, 256, 0, st>>>(c->N, c->nptr, c->mfF, p, c->dirflag, c->specOfNode,
+                                                                 active, q, npq);
+}
+
+void efe_mf_free(efe_ctx* c) {
+    hipFree(c->mfC);
+    hipFree(c->mfF);
+    hipFree(c->dnxi32);
+    hipFree(c->mfPos);
+    c->mfC = c->mfF = c->dnxi32 = nullptr;
+    c->mfPos = nullptr;
+}

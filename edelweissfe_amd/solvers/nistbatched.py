@@ -37,7 +37,7 @@ class NISTBatched:
     identification = "NISTBatchedSolver"
 
     options_default = {"extrapolation": "linear", "precision": "fp64", "krylovAtolFactor": 1e-2,
-                       "krylovRtol": 1e-13, "krylovRtolMixed": 1e-5, "krylovMaxIter": 50000,
+                       "krylovRtol": 1e-13, "krylovRtolMixed": 1e-4, "krylovMaxIter": 50000,
                        "krylovCheckEvery": 16, "autotune": True}
 
     def __init__(self, **kwargs):
@@ -82,7 +82,7 @@ class NISTBatched:
         if self.options["autotune"]:
             from .. import autotune
 
-            self.tuning = autotune.tune(self.system, _lib.EFE_FP32 if self._mixed else _lib.EFE_FP64)
+            self.tuning = autotune.tune(self.system, _lib.EFE_FP32_MATFREE if self._mixed else _lib.EFE_FP64)
         return self.system
 
     def attach(self, other):
@@ -104,7 +104,7 @@ class NISTBatched:
             raise ValueError("the batch was built for {} specimens".format(sysm.n_specimens))
         dev = sysm.device
         D = sysm.n_dof
-        prec = _lib.EFE_FP32 if self._mixed else _lib.EFE_FP64
+        prec = _lib.EFE_FP32_MATFREE if self._mixed else _lib.EFE_FP64
         rtol = self.options["krylovRtolMixed"] if self._mixed else self.options["krylovRtol"]
         nn = 8 if models[0].elementType == "C3D8" else 20
         nAcc = np.array([m.nElements * nn * 3 for m in models], dtype=np.float64)

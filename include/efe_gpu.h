@@ -52,6 +52,9 @@ extern "C" {
 /* precision of the stiffness contractions, the stored K and the Krylov solve */
 #define EFE_FP64 0
 #define EFE_FP32 1
+/* FP32 matrix-free operator: the element pass stores per-QP J^-1 and the compact tangent
+ * (FP32) and only the diagonal blocks; K p is applied element by element.  CG only. */
+#define EFE_FP32_MATFREE 2
 
 #define EFE_SOLVER_CG 0
 #define EFE_SOLVER_GMRES 1
@@ -147,7 +150,8 @@ int efe_set_geostatic(efe_ctx* ctx, const double def[6], const int32_t* elements
  * in CSR value order (device [csr_nnz], FP64). */
 int efe_export_csr_values(efe_ctx* ctx, double* values);
 
-/* y = K x with the context's K (device [n_dof], FP64 vectors). */
+/* y = K x with the context's K (device [n_dof], FP64 vectors): the FP64 assembled operator,
+ * or the FP32 matrix-free one after an EFE_FP32_MATFREE pass. */
 int efe_spmv(efe_ctx* ctx, const double* x, double* y);
 
 /* Solves K x = b with the context's (eliminated) K (device [n_dof]); nonlinearimplicitstatic.py:727-751.

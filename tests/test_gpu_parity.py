@@ -228,14 +228,15 @@ def test_nist_j2_load_history_matches_oracle():
     assert (om.state[..., 12] > 0).any()
 
 
-def test_mixed_precision_error_vs_fp64():
+@pytest.mark.parametrize("operator", ["matfree", "assembled"])
+def test_mixed_precision_error_vs_fp64(operator):
     n, l = (5, 5, 5), (1.0, 1.0, 1.0)
     bcs = uniaxial_bcs(0.01)
     kw = dict(maxInc=0.1, minInc=1e-6, maxNumInc=100, maxIter=15)
     p64 = device_model(n, l, "C3D8", "vonmises", J2, jitter=0.1)
     p32 = device_model(n, l, "C3D8", "vonmises", J2, jitter=0.1)
     _run_device(p64, kw, bcs)
-    _run_device(p32, kw, bcs, precision="mixed")
+    _run_device(p32, kw, bcs, precision="mixed", mixedOperator=operator)
     assert rel_err(p32.U, p64.U) < 1e-8
     assert rel_err(p32.qpState[..., 0:6], p64.qpState[..., 0:6]) < 1e-7
 
@@ -450,3 +451,45 @@ def test_geostatic_then_footing_drucker_prager_matches_oracle(precision):
         assert rel_err(pm.qpState[..., 0:6], om.state[..., 0:6]) < 1e-8
     else:
         assert rel_err(pm.U, om.U) < 1e-8
+
+
+@pytest.mark.parametrize("el_type,material,props,n", [
+    ("C3D8", "vonmises", J2, (4, 3, 3)),
+    ("C3D8", "druckerprager", DP, (3, 3, 2)),
+    ("C3D20R", "druckerprager", DP, (2, 3, 2)),
+    ("C3D20", "linearelastic", [210000.0, 0.3], (2, 2, 2)),
+    ("C3D20R", "vonmises", J2NL, (2, 2, 3)),
+])
+def test_matfree_operator_matches_assembled(el_type, material, props, n):
+    """The FP32 matrix-free operator (per-QP J^-1 and compact tangent) against the FP64
+    assembled, Dirichlet-eliminated operator at a plastic state; P and F from the FP64
+    residual path are identical; the right-hand-side correction K[:, dir] dir_values too."""
+    from edelweissfe_amd import _lib
+
+    l = (1.0, 1.0, 1.0)
+    pm = device_model(n, l, el_type, material, props, jitter=0.1, seed=2)
+    sysm = _system(pm)
+    dirs = np.r_[pm.dofsOfNodeSet("gen_bottom"), pm.dofsOfNodeSet("gen_top", [1])]
+    sysm.set_dirichlet(dirs)
+    rng = np.random.default_rng(11)
+    dU = torch.as_tensor(rng.uniform(-1, 1, sysm.n_dof) * 4e-3, device="cuda")
+    dv = torch.zeros(sysm.n_dof, dtype=torch.float64, device="cuda")
+    dv[torch.as_tensor(np.unique(dirs), device="cuda")] = torch.as_tensor(
+        rng.uniform(-1, 1, np.unique(dirs).size) * 1e-3, device="cuda")
+    x = torch.as_tensor(rng.standard_normal(sysm.n_dof), device="cuda")
+    P64, F64, c64 = sysm.zeros(), sysm.zeros(), sysm.zeros()
+    sysm.compute_elements(dU, P64, F64, dir_values=dv, dir_correction=c64)
+    y64 = sysm.zeros()
+    sysm.spmv(x, y64)
+    if material != "linearelastic":
+        assert (sysm.qp_state(1)[..., 12] > 0).any()
+    Pm, Fm, cm = sysm.zeros(), sysm.zeros(), sysm.zeros()
+    sysm.compute_elements(dU, Pm, Fm, dir_values=dv, dir_correction=cm, precision=_lib.EFE_FP32_MATFREE)
+    ym = sysm.zeros()
+    sysm.spmv(x, ym)
+    assert torch.equal(Pm, P64) and torch.equal(Fm, F64)
+    assert rel_err(ym.cpu().numpy(), y64.cpu().numpy()) < 3e-6
+    assert rel_err(cm.cpu().numpy(), c64.cpu().numpy()) < 3e-6
+    # Dirichlet rows are identity rows of the FP32 operator
+    d = torch.as_tensor(np.unique(dirs), device="cuda")
+    assert torch.equal(ym[d], x[d].float().double())
