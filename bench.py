@@ -550,6 +550,8 @@ def main():
             "newton_iterations_per_history": s32[-1].linearSolves,
             "newton_iterations_per_increment": [r["iters"] for r in s32[-1].history],
             "krylov_iterations_per_history": s32[-1].krylovIterations,
+            "operator_per_increment": [{1: "assembled", 2: "matfree"}.get(r.get("precision"), "fp64")
+                                       for r in s32[-1].history],
             "error_vs_fp64": err,
             "phase_ms": ph32,
         }

@@ -84,3 +84,43 @@ def tune(system, precision=_lib.EFE_FP64, reps=3, use_cache=True):
     cache[key] = best
     _store(cache)
     return best
+
+
+def mixed_operator_costs(system, reps=2, use_cache=True):
+    """Device time (ms) of one element pass (elements + assembly) and of one PCG iteration
+    with each FP32 operator of the mixed form: "matfree" (compact per-QP tangents) and
+    "assembled" (sliced-ELL FP32 operator).  Which one is faster for a whole load history
+    depends on the Krylov iterations per element pass; NISTGPU weighs the two with the ratio
+    it observes.  Cached like the launch shapes."""
+    cached = getattr(system, "_mixed_operator_costs", None)
+    if cached is not None:
+        return cached
+    cache = _load() if use_cache else {}
+    key = _key(system) + "|mixed-operators"
+    if key in cache:
+        system._mixed_operator_costs = cache[key]
+        return cache[key]
+    dU = torch.zeros(system.n_dof, dtype=torch.float64, device=system.device)
+    P, F = system.zeros(), system.zeros()
+    b = torch.randn(system.n_dof, dtype=torch.float64, device=system.device)
+    x = torch.empty_like(b)
+    out = {}
+    for name, prec in (("matfree", _lib.EFE_FP32_MATFREE), ("assembled", _lib.EFE_FP32)):
+        tp, ti = [], []
+        for _ in range(reps):
+            system.compute_elements(dU, P, F, precision=prec)
+            ph = system.phase_times()
+            tp.append(ph["elements"] + ph["assembly"])
+        for _ in range(reps):
+            system.linsolve(b, x, precision=prec, rtol=1e-30, max_iter=64, check_every=32)
+            ti.append(system.phase_times()["krylov_iteration"])
+        out[name] = {"pass_ms": float(np.min(tp)), "iteration_ms": float(np.min(ti))}
+    system._mixed_operator_costs = out
+    cache[key] = out
+    _store(cache)
+    return out
+
+
+def choose_mixed_operator(costs, iterations_per_pass):
+    """The FP32 operator with the smaller pass + iterations_per_pass x iteration time."""
+    return min(costs, key=lambda k: costs[k]["pass_ms"] + iterations_per_pass * costs[k]["iteration_ms"])

@@ -66,6 +66,9 @@ __global__ __launch_bounds__(256) void mf_apply_kernel(int E, int N, const int32
     if (active && !active[specOfElem[e]]) return;  // frozen specimen of a batched solve
     float pe[NN][3], fe[NN][3], xe[NN][3];
     double x0[3];
+    // the tangents are streamed once per product: non-temporal loads keep them from
+    // evicting the gathered p and coordinates from L2 (measured: 131 us instead of 151 us
+    // at 96^3; for the connectivity and slot indices plain loads are faster)
 #pragma unroll
     for (int a = 0; a < NN; ++a) {
         const int nd = conn[(int64_t)a * E + e];
@@ -113,7 +116,7 @@ __global__ __launch_bounds__(256) void mf_apply_kernel(int E, int N, const int32
         const float ep[6] = {g[0], g[4], g[8], g[1] + g[3], g[2] + g[6], g[5] + g[7]};
         float cf[NC];
 #pragma unroll
-        for (int k = 0; k < NC; ++k) cf[k] = mfC[(int64_t)(q * NC + k) * E + e];
+        for (int k = 0; k < NC; ++k) cf[k] = __builtin_nontemporal_load(mfC + (int64_t)(q * NC + k) * E + e);
         const float A = cf[0], B = cf[1], cnn = cf[2];
         const float cn1 = NC == 11 ? cf[3] : 0.0f, c1n = NC == 11 ? cf[4] : 0.0f;
         const float* n = cf + (NC == 11 ? 5 : 3);

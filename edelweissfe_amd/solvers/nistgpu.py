@@ -29,9 +29,13 @@ Device options (in addition to NISTOptions):
   linsolver       "pcg" (default, block-Jacobi PCG, symmetric tangents) or "gmres"
                   (GMRES(gmresRestart) for non-symmetric tangents);
   preconditioner  "auto" (pcg: block Jacobi, gmres: multicolor block ILU(0)), "bjacobi", "ilu0";
-  mixedOperator   "matfree" (default; the PCG of the mixed form applies K matrix-free from
-                  per-QP J^-1 and compact FP32 tangents, no FP32 assembly) or "assembled"
-                  (FP32 sliced-ELL operator; GMRES always uses the assembled one for ILU(0));
+  mixedOperator   FP32 operator of the mixed form's PCG: "matfree" (K applied element by
+                  element from compact per-QP FP32 tangents, no FP32 assembly), "assembled"
+                  (FP32 sliced-ELL operator) or "auto" (default: both are timed once per
+                  system, edelweissfe_amd.autotune.mixed_operator_costs, and every increment
+                  takes the one with the smaller element-pass + R x Krylov-iteration time,
+                  R = Krylov iterations per element pass observed so far).  GMRES always
+                  uses the assembled operator (ILU(0) needs it);
   gmresRestart    restart length m of GMRES(m) (default 50);
   krylovAtolFactor the Krylov solve stops once ||r||_2 <= krylovAtolFactor times the
                   Newton flux threshold max(tol * avgFlux, 1e-7) of the current
@@ -73,7 +77,7 @@ class NISTGPU:
         "preconditioner": "auto",
         "gmresRestart": 50,
         "precision": "fp64",
-        "mixedOperator": "matfree",
+        "mixedOperator": "auto",
         "krylovAtolFactor": 1e-2,
         "krylovRtol": 1e-13,
         "krylovRtolMixed": 1e-4,
@@ -154,7 +158,7 @@ class NISTGPU:
         try:
             for timeStep in step.getTimeStep():
                 record = {"inc": timeStep.number, "stepProgress": timeStep.stepProgress, "iters": None,
-                          "converged": False, "krylov": []}
+                          "converged": False, "krylov": [], "precision": self._precisionCode()}
                 try:
                     U_new, dU, P, iters = self.solveIncrement(U, dU, P, step, model, timeStep, prevTimeStep,
                                                               self.options["extrapolation"], maxIter,
@@ -199,7 +203,7 @@ class NISTGPU:
         dev = sysm.device
         D = sysm.n_dof
         dirichlets = list(step.actions["dirichlet"].values())
-        prec = self._precisionCode()
+        prec = record["precision"] if record is not None else self._precisionCode()
         iterationCounter = 0
         history = (0.0, 0)
         ddU = None
@@ -274,12 +278,18 @@ class NISTGPU:
     def _precisionCode(self):
         if not self._mixed:
             return _lib.EFE_FP64
-        mf = self.options["mixedOperator"].lower()
-        if mf not in ("matfree", "assembled"):
+        op = self.options["mixedOperator"].lower()
+        if op not in ("matfree", "assembled", "auto"):
             raise AttributeError("invalid mixedOperator {:}".format(self.options["mixedOperator"]))
-        if mf == "matfree" and self.options["linsolver"].lower() == "pcg":
-            return _lib.EFE_FP32_MATFREE
-        return _lib.EFE_FP32
+        if self.options["linsolver"].lower() != "pcg":
+            return _lib.EFE_FP32
+        if op == "auto":
+            from .. import autotune
+
+            costs = autotune.mixed_operator_costs(self.system)
+            ratio = self.krylovIterations / self.elementPasses if self.elementPasses else 100.0
+            op = autotune.choose_mixed_operator(costs, ratio)
+        return _lib.EFE_FP32_MATFREE if op == "matfree" else _lib.EFE_FP32
 
     def _linearSolve(self, b, x, prec, fluxThreshold):
         o = self.options

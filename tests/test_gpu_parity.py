@@ -228,7 +228,7 @@ def test_nist_j2_load_history_matches_oracle():
     assert (om.state[..., 12] > 0).any()
 
 
-@pytest.mark.parametrize("operator", ["matfree", "assembled"])
+@pytest.mark.parametrize("operator", ["matfree", "assembled", "auto"])
 def test_mixed_precision_error_vs_fp64(operator):
     n, l = (5, 5, 5), (1.0, 1.0, 1.0)
     bcs = uniaxial_bcs(0.01)
