@@ -130,7 +130,7 @@ int efe_launch_geostatic(efe_ctx* c, const double* def, const int32_t* elements,
 int efe_mf_alloc(efe_ctx* c);
 int efe_launch_mf_assembly(efe_ctx* c, double* P, double* F, const double* dir_values, double* dir_correction);
 void efe_mf_launch_operator(efe_ctx* c, hipStream_t st, const float* p, float* q, double* part, const int* flags,
-                            int nwg, int bs);
+                            int nwg, int bs, int kr);
 void efe_mf_free(efe_ctx* c);
 int efe_launch_spmv_mf(efe_ctx* c, const double* x, double* y);
 void efe_mf_launch_operator_batched(efe_ctx* c, hipStream_t st, const float* p, float* q, double* npq,

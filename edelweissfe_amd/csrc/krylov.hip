@@ -26,6 +26,9 @@
 
 struct KrylovWork {
     int D = 0, N = 0, nwg = 0, block = 0;
+    // rows per thread: a workgroup owns kr * block consecutive rows; [0] FP64, [1] FP32
+    int krp[2] = {1, 1}, nwgp[2] = {0, 0};
+    int kr = 1;  // of the precision being solved
     // FP64 and FP32 vector sets
     double *x64 = nullptr, *r64 = nullptr, *z64 = nullptr, *p64 = nullptr, *q64 = nullptr;
     float *x32 = nullptr, *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr;
@@ -81,13 +84,15 @@ template <typename T, int BS>
 __global__ __launch_bounds__(BS) void spmv_kernel(int N, const int32_t* __restrict__ slice_off,
                                                   const int32_t* __restrict__ scol, const T* __restrict__ K,
                                                   const T* __restrict__ x, T* __restrict__ y,
-                                                  double* __restrict__ part, const int* __restrict__ flags) {
+                                                  double* __restrict__ part, const int* __restrict__ flags,
+                                                  int kr) {
     __shared__ double sh[BS];
     if (flags && flags[0]) return;
     const int blk = xcd_block();
-    const int n = blk * BS + threadIdx.x;
     double dot = 0.0;
-    if (n < N) {
+    for (int it = 0; it < kr; ++it) {
+        const int n = (blk * kr + it) * BS + threadIdx.x;
+        if (n >= N) break;
         const int s = n / kSlice, l = n - (n / kSlice) * kSlice;
         const int o0 = slice_off[s], deg = (slice_off[s + 1] - o0) / kSlice;
         const int32_t* cp = scol + o0 + l;
@@ -110,7 +115,7 @@ __global__ __launch_bounds__(BS) void spmv_kernel(int N, const int32_t* __restri
         y[3 * n] = y0;
         y[3 * n + 1] = y1;
         y[3 * n + 2] = y2;
-        if (part) dot = (double)x[3 * n] * y0 + (double)x[3 * n + 1] * y1 + (double)x[3 * n + 2] * y2;
+        if (part) dot += (double)x[3 * n] * y0 + (double)x[3 * n + 1] * y1 + (double)x[3 * n + 2] * y2;
     }
     if (part) {
         double s = block_sum<BS>(dot, sh);
@@ -124,12 +129,13 @@ template <typename T, int BS>
 __global__ __launch_bounds__(BS) void pcg_init_kernel(int N, const double* __restrict__ b, const T* __restrict__ Minv,
                                                       T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
                                                       T* __restrict__ p, double* __restrict__ part, int nwg,
-                                                      int* __restrict__ flags) {
+                                                      int* __restrict__ flags, int kr) {
     __shared__ double sh[BS];
     const int blk = xcd_block();
-    const int n = blk * BS + threadIdx.x;
     double rz = 0, bb = 0;
-    if (n < N) {
+    for (int it = 0; it < kr; ++it) {
+        const int n = (blk * kr + it) * BS + threadIdx.x;
+        if (n >= N) break;
         T rv[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
@@ -165,16 +171,17 @@ __global__ __launch_bounds__(BS) void pcg_update_kernel(int N, const T* __restri
                                                         T* __restrict__ r, T* __restrict__ z,
                                                         const T* __restrict__ p, const T* __restrict__ q,
                                                         double* __restrict__ part, int nwg, int par,
-                                                        const int* __restrict__ flags) {
+                                                        const int* __restrict__ flags, int kr) {
     __shared__ double sh[BS];
     if (flags[0]) return;
     const double pq = reduce_partials<BS>(part, nwg, sh);
     const double rz = reduce_partials<BS>(part + (1 + par) * nwg, nwg, sh);
     const T alpha = (T)(rz / pq);
     const int blk = xcd_block();
-    const int n = blk * BS + threadIdx.x;
     double rzn = 0, rr = 0;
-    if (n < N) {
+    for (int it = 0; it < kr; ++it) {
+        const int n = (blk * kr + it) * BS + threadIdx.x;
+        if (n >= N) break;
         T rv[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
@@ -203,7 +210,7 @@ template <typename T, int BS>
 __global__ __launch_bounds__(BS) void pcg_direction_kernel(int N, const T* __restrict__ z, T* __restrict__ p,
                                                            const double* __restrict__ part, int nwg, int par,
                                                            int jlocal, const double* __restrict__ dtol2,
-                                                           int* __restrict__ flags) {
+                                                           int* __restrict__ flags, int kr) {
     __shared__ double sh[BS];
     if (flags[0]) return;
     const double rzo = reduce_partials<BS>(part + (1 + par) * nwg, nwg, sh);
@@ -220,8 +227,10 @@ __global__ __launch_bounds__(BS) void pcg_direction_kernel(int N, const T* __res
         return;
     }
     const T beta = (T)(rzn / rzo);
-    const int n = xcd_block() * BS + threadIdx.x;
-    if (n < N) {
+    const int blk = xcd_block();
+    for (int it = 0; it < kr; ++it) {
+        const int n = (blk * kr + it) * BS + threadIdx.x;
+        if (n >= N) break;
 #pragma unroll
         for (int i = 0; i < 3; ++i) p[3 * n + i] = z[3 * n + i] + beta * p[3 * n + i];
     }
@@ -403,12 +412,13 @@ template <typename T, int BS>
 void launch_iter(efe_ctx* c, KrylovWork* w, hipStream_t st, T* x, T* r, T* z, T* p, T* q, const T* K,
                  const T* Minv, int par, int j) {
     const int grid = w->nwg;
+    const int kr = w->kr;
     if (K)
-        spmv_kernel<T, BS><<<grid, BS, 0, st>>>(c->N, c->slice_off, c->scol, K, p, q, w->part, w->flags);
+        spmv_kernel<T, BS><<<grid, BS, 0, st>>>(c->N, c->slice_off, c->scol, K, p, q, w->part, w->flags, kr);
     else  // matrix-free FP32 operator
-        efe_mf_launch_operator(c, st, (const float*)p, (float*)q, w->part, w->flags, grid, BS);
-    pcg_update_kernel<T, BS><<<grid, BS, 0, st>>>(c->N, Minv, x, r, z, p, q, w->part, grid, par, w->flags);
-    pcg_direction_kernel<T, BS><<<grid, BS, 0, st>>>(c->N, z, p, w->part, grid, par, j, w->dscal, w->flags);
+        efe_mf_launch_operator(c, st, (const float*)p, (float*)q, w->part, w->flags, grid, BS, kr);
+    pcg_update_kernel<T, BS><<<grid, BS, 0, st>>>(c->N, Minv, x, r, z, p, q, w->part, grid, par, w->flags, kr);
+    pcg_direction_kernel<T, BS><<<grid, BS, 0, st>>>(c->N, z, p, w->part, grid, par, j, w->dscal, w->flags, kr);
 }
 
 template <typename T>
@@ -426,10 +436,10 @@ template <typename T>
 void launch_init_bs(efe_ctx* c, KrylovWork* w, hipStream_t st, const double* b, const T* Minv, T* x, T* r, T* z,
                     T* p) {
     switch (w->block) {
-        case 64: pcg_init_kernel<T, 64><<<w->nwg, 64, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->nwg, w->flags); break;
-        case 128: pcg_init_kernel<T, 128><<<w->nwg, 128, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->nwg, w->flags); break;
-        case 512: pcg_init_kernel<T, 512><<<w->nwg, 512, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->nwg, w->flags); break;
-        default: pcg_init_kernel<T, 256><<<w->nwg, 256, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->nwg, w->flags); break;
+        case 64: pcg_init_kernel<T, 64><<<w->nwg, 64, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->nwg, w->flags, w->kr); break;
+        case 128: pcg_init_kernel<T, 128><<<w->nwg, 128, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->nwg, w->flags, w->kr); break;
+        case 512: pcg_init_kernel<T, 512><<<w->nwg, 512, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->nwg, w->flags, w->kr); break;
+        default: pcg_init_kernel<T, 256><<<w->nwg, 256, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->nwg, w->flags, w->kr); break;
     }
 }
 
@@ -470,8 +480,19 @@ int ensure_work(efe_ctx* c) {
     }
     if (w->part) hipFree(w->part);
     w->block = block;
-    w->nwg = round8((c->N + block - 1) / block);
-    EFE_HIP(hipMalloc(&w->part, (size_t)5 * w->nwg * sizeof(double)));
+    // bounded workgroup counts: every workgroup re-reduces all partials in a fixed order,
+    // so their number must stay bounded at any mesh size.  Measured on MI355X: FP64 is
+    // fastest with at most ~2048 workgroups (fewer starve its SpMV of waves), the FP32
+    // operators with at most ~1024; uncapped, the reductions dominate at 96^3.
+    const int groups = (c->N + block - 1) / block;
+    const int cap[2] = {2048, 1024};
+    for (int i = 0; i < 2; ++i) {
+        w->krp[i] = (groups + cap[i] - 1) / cap[i];
+        w->nwgp[i] = round8((c->N + block * w->krp[i] - 1) / (block * w->krp[i]));
+    }
+    w->kr = w->krp[0];
+    w->nwg = w->nwgp[0];
+    EFE_HIP(hipMalloc(&w->part, (size_t)5 * (w->nwgp[0] > w->nwgp[1] ? w->nwgp[0] : w->nwgp[1]) * sizeof(double)));
     return EFE_OK;
 }
 
@@ -480,6 +501,8 @@ int pcg_solve(efe_ctx* c, const double* b, double* xout, const efe_solve_opts* o
               T* z, T* p, T* q, const T* K, const T* Minv, int gi) {
     KrylovWork* w = c->kw;
     hipStream_t st = w->ks;
+    w->kr = w->krp[sizeof(T) == 4 ? 1 : 0];
+    w->nwg = w->nwgp[sizeof(T) == 4 ? 1 : 0];
     // order after the caller's stream
     EFE_HIP(hipEventRecord(w->evA, c->stream));
     EFE_HIP(hipStreamWaitEvent(st, w->evA, 0));
@@ -592,7 +615,7 @@ int efe_launch_spmv64(efe_ctx* c, const double* x, double* y) {
         return EFE_ERR_STATE;
     }
     spmv_kernel<double, 256><<<round8((c->N + 255) / 256), 256, 0, c->stream>>>(c->N, c->slice_off, c->scol, c->K64,
-                                                                                 x, y, nullptr, nullptr);
+                                                                                 x, y, nullptr, nullptr, 1);
     EFE_HIP(hipGetLastError());
     return EFE_OK;
 }

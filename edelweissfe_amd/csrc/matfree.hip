@@ -205,13 +205,14 @@ __global__ __launch_bounds__(BS) void mf_gather_kernel(int N, const int32_t* __r
                                                        const float* __restrict__ Fe, const TP* __restrict__ p,
                                                        const uint8_t* __restrict__ dirflag,
                                                        TY* __restrict__ y, double* __restrict__ part,
-                                                       const int* __restrict__ flags) {
+                                                       const int* __restrict__ flags, int kr) {
     __shared__ double sh[BS];
     if (flags && flags[0]) return;
     const int blk = xcd_block_mf();
-    const int n = blk * BS + threadIdx.x;
     double dot = 0.0;
-    if (n < N) {
+    for (int it = 0; it < kr; ++it) {
+        const int n = (blk * kr + it) * BS + threadIdx.x;
+        if (n >= N) break;
         float y0 = 0, y1 = 0, y2 = 0;
         for (int c = nptr[n]; c < nptr[n + 1]; ++c) {  // slots in element order
             const float* src = Fe + (int64_t)c * 3;
@@ -325,9 +326,9 @@ void launch_apply(efe_ctx* c, hipStream_t st, const TP* p, int mask, const int* 
 
 template <int BS>
 void launch_gather_op(efe_ctx* c, hipStream_t st, const float* p, float* q, double* part, const int* flags,
-                      int nwg) {
+                      int nwg, int kr) {
     mf_gather_kernel<BS, 0, float, float><<<nwg, BS, 0, st>>>(c->N, c->nptr, c->nlist, c->mfF, p, c->dirflag, q,
-                                                              part, flags);
+                                                              part, flags, kr);
 }
 
 }  // namespace
@@ -363,7 +364,7 @@ int efe_launch_mf_assembly(efe_ctx* c, double* P, double* F, const double* dir_v
     if (dir_values) {
         launch_apply<double>(c, c->stream, dir_values, 0, nullptr);
         mf_gather_kernel<256, 1, double, double><<<round8_mf(grid), 256, 0, c->stream>>>(
-            c->N, c->nptr, c->nlist, c->mfF, dir_values, c->dirflag, dir_correction, nullptr, nullptr);
+            c->N, c->nptr, c->nlist, c->mfF, dir_values, c->dirflag, dir_correction, nullptr, nullptr, 1);
         EFE_HIP(hipGetLastError());
     }
     c->Kprec = EFE_FP32_MATFREE;
@@ -374,13 +375,13 @@ int efe_launch_mf_assembly(efe_ctx* c, double* P, double* F, const double* dir_v
 // q = K p (FP32, Dirichlet eliminated) with per-workgroup partials of p.q, for the PCG:
 // nwg workgroups of bs threads, the layout of the assembled SpMV's partials.
 void efe_mf_launch_operator(efe_ctx* c, hipStream_t st, const float* p, float* q, double* part, const int* flags,
-                            int nwg, int bs) {
+                            int nwg, int bs, int kr) {
     launch_apply<float>(c, st, p, 1, flags);
     switch (bs) {
-        case 64: launch_gather_op<64>(c, st, p, q, part, flags, nwg); break;
-        case 128: launch_gather_op<128>(c, st, p, q, part, flags, nwg); break;
-        case 512: launch_gather_op<512>(c, st, p, q, part, flags, nwg); break;
-        default: launch_gather_op<256>(c, st, p, q, part, flags, nwg); break;
+        case 64: launch_gather_op<64>(c, st, p, q, part, flags, nwg, kr); break;
+        case 128: launch_gather_op<128>(c, st, p, q, part, flags, nwg, kr); break;
+        case 512: launch_gather_op<512>(c, st, p, q, part, flags, nwg, kr); break;
+        default: launch_gather_op<256>(c, st, p, q, part, flags, nwg, kr); break;
     }
 }
 
@@ -388,7 +389,7 @@ void efe_mf_launch_operator(efe_ctx* c, hipStream_t st, const float* p, float* q
 int efe_launch_spmv_mf(efe_ctx* c, const double* x, double* y) {
     launch_apply<double>(c, c->stream, x, 1, nullptr);
     mf_gather_kernel<256, 0, double, double><<<round8_mf((c->N + 255) / 256), 256, 0, c->stream>>>(
-        c->N, c->nptr, c->nlist, c->mfF, x, c->dirflag, y, nullptr, nullptr);
+        c->N, c->nptr, c->nlist, c->mfF, x, c->dirflag, y, nullptr, nullptr, 1);
     EFE_HIP(hipGetLastError());
     return EFE_OK;
 }
