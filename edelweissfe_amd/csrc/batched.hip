@@ -107,16 +107,18 @@ template <typename T>
 __global__ __launch_bounds__(BS) void b_spmv_kernel(int N, const int32_t* __restrict__ spec,
                                                     const int* __restrict__ active,
                                                     const int32_t* __restrict__ slice_off,
-                                                    const int32_t* __restrict__ scol, const T* __restrict__ K,
+                                                    const int32_t* __restrict__ scol,
+                                                    const int32_t* __restrict__ slotNode, const T* __restrict__ K,
                                                     const T* __restrict__ p, T* __restrict__ q,
                                                     double* __restrict__ npq) {
-    const int n = blockIdx.x * BS + threadIdx.x;
-    if (n >= N) return;
+    const int ps = blockIdx.x * BS + threadIdx.x;  // row slot
+    if (ps >= N) return;
+    const int n = slotNode[ps];
     if (!active[spec[n]]) {
         npq[n] = 0.0;
         return;
     }
-    const int s = n / kSlice, l = n - (n / kSlice) * kSlice;
+    const int s = ps / kSlice, l = ps - (ps / kSlice) * kSlice;
     const int o0 = slice_off[s], deg = (slice_off[s + 1] - o0) / kSlice;
     const int32_t* cp = scol + o0 + l;
     const T* kp = K + (int64_t)o0 * 9 + l;
@@ -238,8 +240,8 @@ void launch_b_batch(efe_ctx* c, BatchWork* w, hipStream_t st, const T* K, const 
     T *x = (T*)w->x, *r = (T*)w->r, *z = (T*)w->z, *p = (T*)w->p, *q = (T*)w->q;
     for (int j = 0; j < batch; ++j) {
         if (K)
-            b_spmv_kernel<T><<<gridn(N), BS, 0, st>>>(N, c->specOfNode, w->active, c->slice_off, c->scol, K, p, q,
-                                                      w->nv0);
+            b_spmv_kernel<T><<<gridn(N), BS, 0, st>>>(N, c->specOfNode, w->active, c->slice_off, c->scol,
+                                                      c->slotNode, K, p, q, w->nv0);
         else  // matrix-free FP32 operator
             efe_mf_launch_operator_batched(c, st, (const float*)p, (float*)q, w->nv0, w->active);
         b_alpha_kernel<<<S, BS, 0, st>>>(c->specNodeOff, w->nv0, w->sc, w->active);

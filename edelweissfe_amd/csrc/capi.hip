@@ -228,6 +228,8 @@ int efe_ctx_create(const efe_mesh_desc* d, efe_ctx** out) {
     if ((rc = upload(&c->diagk, hp.diagk.data(), hp.diagk.size(), st))) return fail(rc);
     if ((rc = upload(&c->slice_off, hp.slice_off.data(), hp.slice_off.size(), st))) return fail(rc);
     if ((rc = upload(&c->scol, hp.scol.data(), hp.scol.size(), st))) return fail(rc);
+    if ((rc = upload(&c->slotNode, hp.slot_node.data(), hp.slot_node.size(), st))) return fail(rc);
+    if ((rc = upload(&c->nodeSlot, hp.node_slot.data(), hp.node_slot.size(), st))) return fail(rc);
     // no host copies of the large gather lists are needed after upload
     std::vector<int32_t>().swap(hp.clist);
     std::vector<int32_t>().swap(hp.coff);
@@ -263,6 +265,7 @@ void efe_ctx_destroy(efe_ctx* c) {
                     c->coords, c->conn,  c->props,  c->dNdxi,  c->shapeN, c->qpw,    c->stateRef, c->stateTmp,
                     c->Kscr,   c->Pscr,  c->bptr,   c->bcol,   c->coff,   c->clist,  c->nptr,     c->nlist,
                     c->diagk,  c->slice_off, c->scol, c->K64,  c->K32,    c->Minv64, c->Minv32,   c->dirflag,
+                    c->slotNode, c->nodeSlot,
                     c->status, c->red,   c->specOfNode, c->specNodeOff, c->specOfElem};
     for (void* p : bufs)
         if (p) hipFree(p);

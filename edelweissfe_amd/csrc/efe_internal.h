@@ -22,6 +22,10 @@ void efe_set_error(const std::string& msg);
 
 // Slice height of the sliced-ELL 3x3 block layout: one wavefront of row blocks.
 constexpr int kSlice = 64;
+// Rows of the sliced operator are ordered by decreasing length inside windows of this many
+// consecutive nodes, so that slices hold rows of equal length (hexa20 grids mix 81-block
+// corner rows with 51-block mid-side rows) while rows stay near their mesh neighbours.
+constexpr int kSortWindow = 1024;
 
 struct HostPattern {
     int N = 0, E = 0, NN = 0;
@@ -32,6 +36,8 @@ struct HostPattern {
     std::vector<int32_t> slice_off;       // [nslices+1] first padded block of each slice
     std::vector<int32_t> slice_deg;       // [nslices] padded row length of each slice
     std::vector<int32_t> scol;            // [padded blocks] block column (padding: the row itself)
+    std::vector<int32_t> slot_node;       // [N] node of operator row slot p (p = 64 s + l)
+    std::vector<int32_t> node_slot;       // [N] inverse
     int64_t padded_blocks = 0;
 };
 
@@ -92,6 +98,7 @@ struct efe_ctx {
     int32_t *bptr = nullptr, *bcol = nullptr, *coff = nullptr, *clist = nullptr;
     int32_t *nptr = nullptr, *nlist = nullptr, *diagk = nullptr;
     int32_t *slice_off = nullptr, *scol = nullptr;
+    int32_t *slotNode = nullptr, *nodeSlot = nullptr;  // operator row slot <-> node
     int nslices = 0;
 
     // assembled operator

@@ -214,10 +214,12 @@ __global__ __launch_bounds__(256) void assemble_kernel(
     const int32_t* __restrict__ clist, const int32_t* __restrict__ nptr, const int32_t* __restrict__ nlist,
     const int32_t* __restrict__ slice_off, const KT* __restrict__ Kscr, const double* __restrict__ Pscr,
     const uint8_t* __restrict__ dirflag, int ndir, const double* __restrict__ dirv, double* __restrict__ dircorr,
-    KT* __restrict__ Kval, KT* __restrict__ Minv, double* __restrict__ P, double* __restrict__ F) {
+    KT* __restrict__ Kval, KT* __restrict__ Minv, double* __restrict__ P, double* __restrict__ F,
+    const int32_t* __restrict__ nodeSlot) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
-    const int s = n / kSlice, l = n - (n / kSlice) * kSlice;
+    const int ps = nodeSlot[n];  // row slot of node n in the sliced layout
+    const int s = ps / kSlice, l = ps - (ps / kSlice) * kSlice;
     const int64_t vbase = (int64_t)slice_off[s] * 9 + l;
     bool fr[3] = {false, false, false};
     if (ndir) {
@@ -348,10 +350,12 @@ __global__ void gather_add_kernel(int N, const int32_t* __restrict__ nptr, const
 
 template <typename KT>
 __global__ void export_csr_kernel(int N, const int32_t* __restrict__ bptr, const int32_t* __restrict__ slice_off,
-                                  const KT* __restrict__ Kval, double* __restrict__ out) {
+                                  const int32_t* __restrict__ nodeSlot, const KT* __restrict__ Kval,
+                                  double* __restrict__ out) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
-    const int s = n / kSlice, l = n - (n / kSlice) * kSlice;
+    const int ps = nodeSlot[n];
+    const int s = ps / kSlice, l = ps - (ps / kSlice) * kSlice;
     const int64_t vbase = (int64_t)slice_off[s] * 9 + l;
     const int deg = bptr[n + 1] - bptr[n];
     const int64_t r0 = (int64_t)9 * bptr[n];
@@ -433,11 +437,11 @@ int efe_launch_assembly(efe_ctx* c, double* P, double* F, const double* dir_valu
     if (precision == EFE_FP32) {
         assemble_kernel<float><<<grid, bs, 0, c->stream>>>(
             c->N, c->bptr, c->bcol, c->coff, c->clist, c->nptr, c->nlist, c->slice_off, (const float*)c->Kscr,
-            c->Pscr, c->dirflag, c->ndir, dir_values, dir_correction, c->K32, c->Minv32, P, F);
+            c->Pscr, c->dirflag, c->ndir, dir_values, dir_correction, c->K32, c->Minv32, P, F, c->nodeSlot);
     } else {
         assemble_kernel<double><<<grid, bs, 0, c->stream>>>(
             c->N, c->bptr, c->bcol, c->coff, c->clist, c->nptr, c->nlist, c->slice_off, (const double*)c->Kscr,
-            c->Pscr, c->dirflag, c->ndir, dir_values, dir_correction, c->K64, c->Minv64, P, F);
+            c->Pscr, c->dirflag, c->ndir, dir_values, dir_correction, c->K64, c->Minv64, P, F, c->nodeSlot);
     }
     EFE_HIP(hipGetLastError());
     c->Kprec = precision;
@@ -519,9 +523,11 @@ int efe_launch_geostatic(efe_ctx* c, const double* def, const int32_t* elements,
 int efe_launch_export_csr(efe_ctx* c, double* values) {
     const int grid = (c->N + 255) / 256;
     if (c->Kprec == EFE_FP32)
-        export_csr_kernel<float><<<grid, 256, 0, c->stream>>>(c->N, c->bptr, c->slice_off, c->K32, values);
+        export_csr_kernel<float><<<grid, 256, 0, c->stream>>>(c->N, c->bptr, c->slice_off, c->nodeSlot, c->K32,
+                                                               values);
     else
-        export_csr_kernel<double><<<grid, 256, 0, c->stream>>>(c->N, c->bptr, c->slice_off, c->K64, values);
+        export_csr_kernel<double><<<grid, 256, 0, c->stream>>>(c->N, c->bptr, c->slice_off, c->nodeSlot, c->K64,
+                                                                values);
     EFE_HIP(hipGetLastError());
     return EFE_OK;
 }

@@ -92,11 +92,13 @@ __device__ __forceinline__ void inv3d(const double m[9], double o[9]) {
 template <typename T>
 __global__ __launch_bounds__(BS) void ilu_copy_kernel(int N, const int32_t* __restrict__ bptr,
                                                       const int32_t* __restrict__ slice_off,
+                                                      const int32_t* __restrict__ nodeSlot,
                                                       const T* __restrict__ K, const int32_t* __restrict__ fidx,
                                                       T* __restrict__ F) {
     const int n = blockIdx.x * BS + threadIdx.x;
     if (n >= N) return;
-    const int s = n / kSlice, l = n - (n / kSlice) * kSlice;
+    const int ps = nodeSlot[n];
+    const int s = ps / kSlice, l = ps - (ps / kSlice) * kSlice;
     const int64_t vb = (int64_t)slice_off[s] * 9 + l;
     for (int k = 0, b = bptr[n]; b < bptr[n + 1]; ++k, ++b) {
         T* dst = F + (int64_t)fidx[b] * 9;
@@ -272,12 +274,14 @@ __global__ void bjacobi_kernel(int N, const T* __restrict__ Minv, const T* __res
 // y = K x (sliced ELL), optionally y = b - K x
 template <typename T>
 __global__ __launch_bounds__(BS) void gm_spmv_kernel(int N, const int32_t* __restrict__ slice_off,
-                                                     const int32_t* __restrict__ scol, const T* __restrict__ K,
+                                                     const int32_t* __restrict__ scol,
+                                                     const int32_t* __restrict__ slotNode, const T* __restrict__ K,
                                                      const T* __restrict__ x, T* __restrict__ y,
                                                      const double* __restrict__ b) {
-    const int n = xcd_block_g() * BS + threadIdx.x;
-    if (n >= N) return;
-    const int s = n / kSlice, l = n - (n / kSlice) * kSlice;
+    const int ps = xcd_block_g() * BS + threadIdx.x;  // row slot
+    if (ps >= N) return;
+    const int n = slotNode[ps];
+    const int s = ps / kSlice, l = ps - (ps / kSlice) * kSlice;
     const int o0 = slice_off[s], deg = (slice_off[s + 1] - o0) / kSlice;
     const int32_t* cp = scol + o0 + l;
     const T* kp = K + (int64_t)o0 * 9 + l;
@@ -416,7 +420,7 @@ int ilu_factor(efe_ctx* c, GmresWork* g, const T* K, T*& F, T*& Dinv, int pi, hi
         EFE_HIP(hipMalloc(&Dinv, (size_t)c->N * 9 * sizeof(T)));
     }
     if (g->factored_version[pi] == c->Kversion) return EFE_OK;
-    ilu_copy_kernel<T><<<grid_of(c->N), BS, 0, st>>>(c->N, c->bptr, c->slice_off, K, g->fidx, F);
+    ilu_copy_kernel<T><<<grid_of(c->N), BS, 0, st>>>(c->N, c->bptr, c->slice_off, c->nodeSlot, K, g->fidx, F);
     for (int col = 0; col < g->ncolors; ++col) {
         const int r0 = g->cstart[col], r1 = g->cstart[col + 1];
         if (r1 <= r0) continue;
@@ -494,7 +498,7 @@ int gmres_solve(efe_ctx* c, GmresWork* g, const double* b, double* xout, const e
     double* hh = g->hhost;
     // ||b||
     zero_kernel<T><<<gd, BS, 0, st>>>(D, x);
-    gm_spmv_kernel<T><<<gs, BS, 0, st>>>(N, c->slice_off, c->scol, K, x, w, b);
+    gm_spmv_kernel<T><<<gs, BS, 0, st>>>(N, c->slice_off, c->scol, c->slotNode, K, x, w, b);
     mdot_kernel<T><<<dim3(g->nwg, 1), BS, 0, st>>>(D, w, w, g->part, g->nwg);
     mdot_reduce_kernel<<<1, BS, 0, st>>>(g->part, g->nwg, g->hdev + 2 * (m + 2), 0);
     EFE_HIP(hipMemcpyAsync(hh, g->hdev + 2 * (m + 2), sizeof(double), hipMemcpyDeviceToHost, st));
@@ -519,7 +523,7 @@ int gmres_solve(efe_ctx* c, GmresWork* g, const double* b, double* xout, const e
         for (int j = 0; j < m; ++j) {
             T* Vj = V + (int64_t)j * D;
             apply_precond<T>(c, g, o->precond, F, Dinv, Minv, Vj, z, st);
-            gm_spmv_kernel<T><<<gs, BS, 0, st>>>(N, c->slice_off, c->scol, K, z, w, nullptr);
+            gm_spmv_kernel<T><<<gs, BS, 0, st>>>(N, c->slice_off, c->scol, c->slotNode, K, z, w, nullptr);
             // classical Gram-Schmidt, twice
             for (int pass = 0; pass < 2; ++pass) {
                 mdot_kernel<T><<<dim3(g->nwg, j + 1), BS, 0, st>>>(D, V, w, g->part, g->nwg);
@@ -573,7 +577,7 @@ int gmres_solve(efe_ctx* c, GmresWork* g, const double* b, double* xout, const e
         axpy_kernel<T><<<gd, BS, 0, st>>>(D, z, x);
         if (done || it >= maxit) break;
         // restart: w = b - K x
-        gm_spmv_kernel<T><<<gs, BS, 0, st>>>(N, c->slice_off, c->scol, K, x, w, b);
+        gm_spmv_kernel<T><<<gs, BS, 0, st>>>(N, c->slice_off, c->scol, c->slotNode, K, x, w, b);
         mdot_kernel<T><<<dim3(g->nwg, 1), BS, 0, st>>>(D, w, w, g->part, g->nwg);
         mdot_reduce_kernel<<<1, BS, 0, st>>>(g->part, g->nwg, g->hdev + 2 * (m + 2), 0);
         EFE_HIP(hipMemcpyAsync(hh, g->hdev + 2 * (m + 2), sizeof(double), hipMemcpyDeviceToHost, st));

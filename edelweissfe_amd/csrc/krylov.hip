@@ -1,7 +1,8 @@
 // Krylov solvers for the assembled tangent on gfx950.
 //
 // Operator layout: sliced ELL of 3x3 blocks, slice height 64 (one wavefront).
-// Thread (slice s, lane l) owns node row n = 64 s + l; block slot k of that row
+// Thread (slice s, lane l) owns row slot p = 64 s + l, the row of node slotNode[p] (rows
+// sorted by length inside windows of nodes); block slot k of that row
 // stores its 9 values at Kval[slice_off[s]*9 + (9k+v)*64 + l], so every value
 // load of a wavefront is one contiguous 512 B (FP64) / 256 B (FP32) segment and
 // the block column index (1 int per 9 values) is shared by the three rows of a
@@ -82,7 +83,8 @@ __device__ __forceinline__ double reduce_partials(const double* __restrict__ par
 
 template <typename T, int BS>
 __global__ __launch_bounds__(BS) void spmv_kernel(int N, const int32_t* __restrict__ slice_off,
-                                                  const int32_t* __restrict__ scol, const T* __restrict__ K,
+                                                  const int32_t* __restrict__ scol,
+                                                  const int32_t* __restrict__ slotNode, const T* __restrict__ K,
                                                   const T* __restrict__ x, T* __restrict__ y,
                                                   double* __restrict__ part, const int* __restrict__ flags,
                                                   int kr) {
@@ -91,9 +93,10 @@ __global__ __launch_bounds__(BS) void spmv_kernel(int N, const int32_t* __restri
     const int blk = xcd_block();
     double dot = 0.0;
     for (int it = 0; it < kr; ++it) {
-        const int n = (blk * kr + it) * BS + threadIdx.x;
-        if (n >= N) break;
-        const int s = n / kSlice, l = n - (n / kSlice) * kSlice;
+        const int ps = (blk * kr + it) * BS + threadIdx.x;  // row slot
+        if (ps >= N) break;
+        const int n = slotNode[ps];
+        const int s = ps / kSlice, l = ps - (ps / kSlice) * kSlice;
         const int o0 = slice_off[s], deg = (slice_off[s + 1] - o0) / kSlice;
         const int32_t* cp = scol + o0 + l;
         const T* kp = K + (int64_t)o0 * 9 + l;
@@ -414,7 +417,8 @@ void launch_iter(efe_ctx* c, KrylovWork* w, hipStream_t st, T* x, T* r, T* z, T*
     const int grid = w->nwg;
     const int kr = w->kr;
     if (K)
-        spmv_kernel<T, BS><<<grid, BS, 0, st>>>(c->N, c->slice_off, c->scol, K, p, q, w->part, w->flags, kr);
+        spmv_kernel<T, BS><<<grid, BS, 0, st>>>(c->N, c->slice_off, c->scol, c->slotNode, K, p, q, w->part, w->flags,
+                                                kr);
     else  // matrix-free FP32 operator
         efe_mf_launch_operator(c, st, (const float*)p, (float*)q, w->part, w->flags, grid, BS, kr);
     pcg_update_kernel<T, BS><<<grid, BS, 0, st>>>(c->N, Minv, x, r, z, p, q, w->part, grid, par, w->flags, kr);
@@ -614,8 +618,8 @@ int efe_launch_spmv64(efe_ctx* c, const double* x, double* y) {
         efe_set_error("efe_spmv needs an FP64 operator");
         return EFE_ERR_STATE;
     }
-    spmv_kernel<double, 256><<<round8((c->N + 255) / 256), 256, 0, c->stream>>>(c->N, c->slice_off, c->scol, c->K64,
-                                                                                 x, y, nullptr, nullptr, 1);
+    spmv_kernel<double, 256><<<round8((c->N + 255) / 256), 256, 0, c->stream>>>(
+        c->N, c->slice_off, c->scol, c->slotNode, c->K64, x, y, nullptr, nullptr, 1);
     EFE_HIP(hipGetLastError());
     return EFE_OK;
 }

@@ -88,15 +88,29 @@ bool efe_build_pattern(HostPattern& hp, int N, int E, int NN, const int32_t* con
         std::vector<int32_t> fill(hp.coff.begin(), hp.coff.end() - 1);
         for (size_t k = 0; k < slot.size(); ++k) hp.clist[fill[slot[k]]++] = (int32_t)k;
     }
-    // sliced-ELL layout of the block rows
+    // row slots: inside each window of kSortWindow nodes, rows by decreasing length (stable)
+    hp.slot_node.resize(N);
+    hp.node_slot.resize(N);
+    for (int w0 = 0; w0 < N; w0 += kSortWindow) {
+        const int w1 = std::min(N, w0 + kSortWindow);
+        for (int n = w0; n < w1; ++n) hp.slot_node[n] = n;
+        std::stable_sort(hp.slot_node.begin() + w0, hp.slot_node.begin() + w1, [&](int32_t a, int32_t b) {
+            return hp.bptr[a + 1] - hp.bptr[a] > hp.bptr[b + 1] - hp.bptr[b];
+        });
+    }
+    for (int p = 0; p < N; ++p) hp.node_slot[hp.slot_node[p]] = p;
+    // sliced-ELL layout of the block rows, in slot order
     int ns = (N + kSlice - 1) / kSlice;
     hp.slice_off.assign(ns + 1, 0);
     hp.slice_deg.assign(ns, 0);
     for (int s = 0; s < ns; ++s) {
         int d = 0;
         for (int l = 0; l < kSlice; ++l) {
-            int n = s * kSlice + l;
-            if (n < N) d = std::max(d, hp.bptr[n + 1] - hp.bptr[n]);
+            const int p = s * kSlice + l;
+            if (p < N) {
+                const int n = hp.slot_node[p];
+                d = std::max(d, hp.bptr[n + 1] - hp.bptr[n]);
+            }
         }
         hp.slice_deg[s] = d;
         hp.slice_off[s + 1] = hp.slice_off[s] + d * kSlice;
@@ -105,10 +119,11 @@ bool efe_build_pattern(HostPattern& hp, int N, int E, int NN, const int32_t* con
     hp.scol.assign(hp.padded_blocks, 0);
     for (int s = 0; s < ns; ++s) {
         for (int l = 0; l < kSlice; ++l) {
-            int n = s * kSlice + l;
+            const int p = s * kSlice + l;
+            const int n = p < N ? hp.slot_node[p] : -1;
             for (int k = 0; k < hp.slice_deg[s]; ++k) {
                 int32_t col = 0;
-                if (n < N) col = (k < hp.bptr[n + 1] - hp.bptr[n]) ? hp.bcol[hp.bptr[n] + k] : n;
+                if (n >= 0) col = (k < hp.bptr[n + 1] - hp.bptr[n]) ? hp.bcol[hp.bptr[n] + k] : n;
                 hp.scol[hp.slice_off[s] + (int64_t)k * kSlice + l] = col;
             }
         }
@@ -173,7 +188,7 @@ bool efe_build_ilu_pattern(const HostPattern& hp, IluPattern& ip, std::string& e
         for (int32_t b = hp.bptr[n]; b < hp.bptr[n + 1]; ++b) row.emplace_back(ip.iperm[hp.bcol[b]], b - hp.bptr[n]);
         std::sort(row.begin(), row.end());
         const int s = i / kSlice, l = i % kSlice;
-        const int ks = n / kSlice, kl = n % kSlice;
+        const int ks = hp.node_slot[n] / kSlice, kl = hp.node_slot[n] % kSlice;
         ip.rowlen[i] = (int32_t)row.size();
         for (size_t k = 0; k < row.size(); ++k) {
             ip.scol[ip.slice_off[s] + (int64_t)k * kSlice + l] = row[k].first;
