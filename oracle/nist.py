@@ -125,13 +125,16 @@ class Step:
 class Model:
     """Nodes, one homogeneous hexahedral element set with its material, sets."""
 
-    def __init__(self, coords, el_type, conn, material, nodesets, elsets):
+    def __init__(self, coords, el_type, conn, material, nodesets, elsets, elements=None):
+        """elements: an element set object (default: HexElements of el_type); its `dim` is the
+        number of displacement DOFs per node (3 unless given)."""
         self.coords = np.asarray(coords, dtype=float)
-        self.elements = HexElements(el_type, self.coords, conn)
+        self.elements = elements if elements is not None else HexElements(el_type, self.coords, conn)
+        self.dim = getattr(self.elements, "dim", 3)
         self.material = material
         self.nodesets = nodesets
         self.elsets = elsets
-        self.nDof = 3 * self.coords.shape[0]
+        self.nDof = self.dim * self.coords.shape[0]
         E, Q = self.elements.conn.shape[0], self.elements.nQP
         self.state = np.zeros((E, Q, 12 + material.n_state))
         self.U = np.zeros(self.nDof)
@@ -162,12 +165,12 @@ class NIST:
         self.indptr, self.indices = assembly.csr_pattern(I, J, model.nDof)
         self.x = assembly.x_map(I, J, self.indptr, self.indices)
         self.nnz = self.indices.size
-        self.nAccFlux = el.conn.shape[0] * el.nn * 3
+        self.nAccFlux = el.conn.shape[0] * el.nn * model.dim
 
     def dirichlet_indices(self, model, d):
         nodes = model.nodesets[d["nset"]]
         comps = sorted(d["values"].keys())
-        idx = (3 * np.asarray(nodes)[:, None] + np.arange(3)[None, :])[:, comps]
+        idx = (model.dim * np.asarray(nodes)[:, None] + np.arange(model.dim)[None, :])[:, comps]
         return idx.ravel()
 
     def dirichlet_delta(self, model, d, ts):
@@ -197,7 +200,7 @@ class NIST:
         PExt = np.zeros(model.nDof)
         for nf in step.nodeforces:
             nodes = model.nodesets[nf["nset"]]
-            idx = (3 * np.asarray(nodes)[:, None] + np.arange(3)[None, :]).ravel()
+            idx = (model.dim * np.asarray(nodes)[:, None] + np.arange(model.dim)[None, :]).ravel()
             vec = np.asarray(nf.get("start", 0.0), dtype=float) + np.asarray(nf["vector"], dtype=float) * ts.stepProgress
             PExt[idx] += np.tile(vec, (len(nodes), 1)).ravel()
         for bf in step.bodyforces:

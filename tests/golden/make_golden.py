@@ -24,6 +24,7 @@ UREF_TESTS = [
     "WallShearHexa8",
     "WallShearHexa20",
     "CantileverBeamHexa8",
+    "CantileverBeamVonMisesPE",  # plane strain CPE8, von Mises with nonlinear hardening
 ]
 
 

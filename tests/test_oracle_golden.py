@@ -66,3 +66,31 @@ def test_oracle_reproduces_mesh_file_cantilever(golden):
     U = nist.NIST().solve_step(model, step)
     ref = golden["uref_CantileverBeamHexa8"]
     assert np.abs(U - ref).max() < 1e-9 * np.abs(ref).max()
+
+
+def test_oracle_von_mises_nonlinear_hardening_reproduces_plane_strain_cantilever(golden):
+    """testfiles/CantileverBeamVonMisesPE: 75 x 1 CPE8 (thickness 40) on 10000 x 100, von Mises
+    [210000, 0.15, 355, 1000, 200, 1400] (nonlinear isotropic hardening), clamped left end,
+    node force (0, -4000) on the top right node and body force (0, -7.7e-5), step length 100
+    with maxInc 0.2, minInc 1e-8, maxIter 25.  Pins the oracle's VonMises return mapping and
+    tangent (and so the device kernel, which matches the oracle to 1e-12) against the
+    reference's own plastic load history, with the reference's criterion max|dU| < 1e-6."""
+    from oracle import materials as omat
+    from oracle import nist as onist
+    from oracle import plane
+
+    m = plane.plane_rect_quad(75, 1, 10000.0, 100.0)
+    els = plane.QuadElements8PE(m["coords"], m["conn"], thickness=40.0)
+    mat = omat.VonMises([210000.0, 0.15, 355.0, 1000.0, 200.0, 1400.0])
+    om = onist.Model(m["coords"], "CPE8", m["conn"], mat, m["nodesets"], m["elsets"], elements=els)
+    step = onist.Step(maxInc=0.2, minInc=1e-8, maxNumInc=1000, maxIter=25, stepLength=100.0,
+                      dirichlets=[dict(nset="gen_left", values={0: 0.0, 1: 0.0})],
+                      nodeforces=[dict(nset="gen_rightTop", vector=(0.0, -4000.0))],
+                      bodyforces=[dict(elset="all", vector=(0.0, -0.000077))])
+    so = onist.NIST()
+    so.solve_step(om, step)
+    ref = golden["uref_CantileverBeamVonMisesPE"]
+    assert om.U.shape == ref.shape
+    assert (om.state[..., 12] > 0).any()          # the beam yields
+    assert np.abs(om.U - ref).max() < 1e-6         # _run_tests_edelweissfe.py:101-106
+    assert np.abs(om.U - ref).max() / np.abs(ref).max() < 1e-9
