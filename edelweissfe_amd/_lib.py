@@ -87,6 +87,7 @@ SIGNATURES = {
     "efe_linsolve_batched": (_I, [_P, _P, _P, ctypes.POINTER(SolveOpts), _P, _P, _P, ctypes.POINTER(SolveInfo)]),
     "efe_accept_state_batched": (_I, [_P, _P]),
     "efe_get_specimen_count": (_I, [_P]),
+    "efe_get_specimen_status": (_I, [_P, _P]),
     "efe_accept_state": (_I, [_P]),
     "efe_get_qp_state": (_I, [_P, _I, _P]),
     "efe_set_qp_state": (_I, [_P, _P]),

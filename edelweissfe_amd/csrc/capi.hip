@@ -217,6 +217,8 @@ int efe_ctx_create(const efe_mesh_desc* d, efe_ctx** out) {
             if ((rc = upload(&c->specOfNode, d->specimen_of_node, (size_t)c->N, st))) return fail(rc);
             if ((rc = upload(&c->specNodeOff, off.data(), off.size(), st))) return fail(rc);
             if ((rc = upload(&c->specOfElem, se.data(), se.size(), st))) return fail(rc);
+            EFE_HIP(hipMalloc(&c->specStatus, (size_t)c->nspec * sizeof(int)));
+            EFE_HIP(hipMemsetAsync(c->specStatus, 0, (size_t)c->nspec * sizeof(int), st));
         }
     }
     if ((rc = upload(&c->bptr, hp.bptr.data(), hp.bptr.size(), st))) return fail(rc);
@@ -265,7 +267,7 @@ void efe_ctx_destroy(efe_ctx* c) {
                     c->coords, c->conn,  c->props,  c->dNdxi,  c->shapeN, c->qpw,    c->stateRef, c->stateTmp,
                     c->Kscr,   c->Pscr,  c->bptr,   c->bcol,   c->coff,   c->clist,  c->nptr,     c->nlist,
                     c->diagk,  c->slice_off, c->scol, c->K64,  c->K32,    c->Minv64, c->Minv32,   c->dirflag,
-                    c->slotNode, c->nodeSlot,
+                    c->slotNode, c->nodeSlot, c->specStatus,
                     c->status, c->red,   c->specOfNode, c->specNodeOff, c->specOfElem};
     for (void* p : bufs)
         if (p) hipFree(p);
@@ -344,6 +346,7 @@ int efe_compute_elements(efe_ctx* c, const double* U, const double* dU, const do
     if (precision == EFE_FP32_MATFREE)
         if (int rc = efe_mf_alloc(c)) return rc;
     EFE_HIP(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
+    if (c->specStatus) EFE_HIP(hipMemsetAsync(c->specStatus, 0, (size_t)c->nspec * sizeof(int), c->stream));
     EFE_HIP(hipEventRecord(c->ev[0], c->stream));
     if (int rc = efe_launch_elements(c, dU, precision)) return rc;
     EFE_HIP(hipEventRecord(c->ev[1], c->stream));
@@ -445,6 +448,23 @@ int efe_accept_state_batched(efe_ctx* c, const uint8_t* accept) {
         return EFE_ERR_STATE;
     }
     return efe_launch_accept_batched(c, accept);
+}
+
+int efe_get_specimen_status(efe_ctx* c, uint8_t* out) {
+    if (int rc = check(c)) return rc;
+    if (!out) {
+        efe_set_error("efe_get_specimen_status: out is NULL");
+        return EFE_ERR_ARG;
+    }
+    if (!c->specStatus) {
+        efe_set_error("efe_get_specimen_status needs a context created with specimens");
+        return EFE_ERR_STATE;
+    }
+    std::vector<int> h(c->nspec);
+    EFE_HIP(hipMemcpyAsync(h.data(), c->specStatus, h.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    EFE_HIP(hipStreamSynchronize(c->stream));
+    for (int s = 0; s < c->nspec; ++s) out[s] = h[s] ? 1 : 0;
+    return EFE_OK;
 }
 
 int efe_get_specimen_count(efe_ctx* c) {

@@ -110,6 +110,7 @@ struct efe_ctx {
     uint8_t* dirflag = nullptr;  // [D]
     int ndir = 0;
     int* status = nullptr;       // device status word (cutback / NaN)
+    int* specStatus = nullptr;   // [nspec] cutback request per specimen (batched contexts)
     double* red = nullptr;       // reduction scratch
     int redn = 0;
 

@@ -42,7 +42,9 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
                                                      const double* __restrict__ dU,
                                                      const double* __restrict__ sRef, double* __restrict__ sTmp,
                                                      KT* __restrict__ Kscr, double* __restrict__ Pscr,
-                                                     int* __restrict__ status, float* __restrict__ mfC) {
+                                                     int* __restrict__ status, float* __restrict__ mfC,
+                                                     int* __restrict__ specStatus,
+                                                     const int32_t* __restrict__ specOfElem) {
     constexpr int NS = 12 + MatState<MAT>::n;
     constexpr int NP = (MAT == EFE_LINEAR_ELASTIC) ? 2 : 6;
     constexpr int NC = (MAT == EFE_DRUCKER_PRAGER) ? 11 : 9;
@@ -117,7 +119,10 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
         if (MAT == EFE_LINEAR_ELASTIC) st = efe::le_update(p, s, de, C, MF ? cc : nullptr);
         else if (MAT == EFE_VONMISES) st = efe::vm_update(p, s, ms, de, C, MF ? cc : nullptr);
         else st = efe::dp_update(p, s, ms, de, C, MF ? cc : nullptr);
-        if (st) *status = 1;
+        if (st) {
+            *status = 1;
+            if (specStatus) specStatus[specOfElem[e]] = 1;  // which specimen asked (batched contexts)
+        }
         if (MF) {
             const double fq = det * qpw[q];
             float* dc = mfC + (int64_t)q * NC * E + e;
@@ -372,11 +377,11 @@ void launch_elem_t(efe_ctx* c, const double* dU) {
     if (c->mfPass)
         element_kernel<NN, Q, MAT, float, EPW, true><<<grid, 64, 0, c->stream>>>(
             c->E, c->N, c->coords, c->conn, c->props, propStride, c->dNdxi, c->qpw, dU, c->stateRef, c->stateTmp,
-            (float*)c->Kscr, c->Pscr, c->status, c->mfC);
+            (float*)c->Kscr, c->Pscr, c->status, c->mfC, c->specStatus, c->specOfElem);
     else
         element_kernel<NN, Q, MAT, KT, EPW, false><<<grid, 64, 0, c->stream>>>(
             c->E, c->N, c->coords, c->conn, c->props, propStride, c->dNdxi, c->qpw, dU, c->stateRef, c->stateTmp,
-            (KT*)c->Kscr, c->Pscr, c->status, nullptr);
+            (KT*)c->Kscr, c->Pscr, c->status, nullptr, c->specStatus, c->specOfElem);
 }
 
 template <int NN, int Q, int MAT, typename KT>

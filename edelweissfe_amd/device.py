@@ -158,6 +158,12 @@ class DeviceSystem:
                                                      self._ptr(F), out.ctypes.data), "efe_convergence_norms_batched")
         return out
 
+    def specimen_status(self):
+        """Per specimen: True where the last element pass requested a cutback."""
+        out = np.zeros(self.n_specimens, dtype=np.uint8)
+        check(self.lib.efe_get_specimen_status(self.ctx, out.ctypes.data), "efe_get_specimen_status")
+        return out.astype(bool)
+
     def accept_state_batched(self, accept):
         a = np.ascontiguousarray(accept, dtype=np.uint8)
         check(self.lib.efe_accept_state_batched(self.ctx, a.ctypes.data), "efe_accept_state_batched")

@@ -15,8 +15,11 @@ correction is zero, its temporary state is recomputed unchanged) until the slowe
 specimen of the round has converged or failed; a failed specimen cuts back by its step's
 cutback factor and retries in the next round.
 
-Restriction: the device status word is shared by the batch, so a material cutback request
-(CutbackRequest) fails the whole batch; J2 with linear hardening never requests one.
+A material cutback request (CutbackRequest, vonmises.py:230-231) is reported per specimen
+(efe_get_specimen_status): that specimen's increment fails and is cut back by
+max(0.5, cutbackFactor), as nonlinearimplicitstatic.py:247-249 does for one model.  A
+specimen whose time stepper gives up (StepFailed, :317-319) stops with the state and U of
+its last accepted increment and is listed in failedSpecimens; the others go on.
 """
 
 import numpy as np
@@ -24,7 +27,7 @@ import torch
 
 from .. import _lib
 from ..device import DeviceSystem
-from ..exceptions import ReachedMaxIncrements, ReachedMinIncrementSize, StepFailed
+from ..exceptions import ReachedMaxIncrements, ReachedMinIncrementSize
 
 FIELD_CORRECTION_TOL = 1e-8      # phenomena.py fieldCorrectionTolerance["displacement"]
 FLUX_RESIDUAL_TOL = 1e-8         # phenomena.py fluxResidualTolerance["displacement"]
@@ -93,9 +96,11 @@ class NISTBatched:
 
     # ---- public interface ------------------------------------------------------------------------------
     def solveSteps(self, models, steps):
-        """Solves steps[s] on models[s] for every specimen s; returns (True, models) with
-        model.U and model.time updated.  Raises StepFailed when a specimen's increment
-        control gives up (ReachedMaxIncrements / ReachedMinIncrementSize)."""
+        """Solves steps[s] on models[s] for every specimen s; model.U and model.time are
+        updated.  A specimen whose increment control gives up (ReachedMaxIncrements /
+        ReachedMinIncrementSize, i.e. StepFailed of nonlinearimplicitstatic.py:330-331 for
+        that model) stops there while the others go on; its index is listed in
+        self.failedSpecimens.  Returns (True if no specimen failed, models)."""
         S = len(models)
         if len(steps) != S:
             raise ValueError("one step per model")
@@ -131,59 +136,71 @@ class NISTBatched:
 
         gens = [st.getTimeStep() for st in steps]
         finished = np.zeros(S, dtype=bool)
+        self.failedSpecimens = []
         prevTS = [None] * S
         U = torch.as_tensor(np.concatenate([m.U for m in models]), dtype=torch.float64, device=dev)
         dU = torch.zeros(D, dtype=torch.float64, device=dev)
         P = torch.zeros(D, dtype=torch.float64, device=dev)
         F = torch.zeros(D, dtype=torch.float64, device=dev)
         self.history = [[] for _ in range(S)]
+        # per specimen, why each failed increment failed: "cutback" (material request), "nan",
+        # "iterations" (maxIter or maxGrowIter) or "krylov" (linear solve diverged)
+        self.failures = [[] for _ in range(S)]
         # per-specimen work counts: element passes, linear solves and Krylov iterations the
         # specimen would need on its own
         self.specimenPasses = np.zeros(S, dtype=np.int64)
         self.specimenSolves = np.zeros(S, dtype=np.int64)
         self.specimenKrylov = np.zeros(S, dtype=np.int64)
-        try:
-            while True:
-                ts = [None] * S
-                for s in range(S):
-                    if not finished[s]:
+        while True:
+            ts = [None] * S
+            for s in range(S):
+                if not finished[s]:
+                    try:
                         ts[s] = next(gens[s], None)
-                        finished[s] = ts[s] is None
-                live = ~finished
-                if not live.any():
-                    break
-                self.rounds += 1
-                dU, PExt, deltaT, extrap = self._startIncrements(models, steps, ts, prevTS, live, owners, dU, isDir,
-                                                                 perDof)
-                state, it = self._newtonRound(sysm, steps, live, extrap, hasDir, dU, P, F, PExt, deltaT, dirT,
-                                              perDof, nAcc, prec, rtol)
-                dU = self._lastDU
-                ok = state == CONVERGED
-                sysm.accept_state_batched(ok)
-                U = torch.where(perDof(ok), U + dU, U)
-                for s in np.flatnonzero(live):
-                    if ok[s]:
-                        prevTS[s] = ts[s]
-                        if it[s] >= steps[s].criticalIter:
-                            steps[s].preventIncrementIncrease()
-                        models[s].time = ts[s].totalTime
-                        self.history[s].append(int(it[s]))
-                    else:
-                        steps[s].discardAndChangeIncrement(steps[s].cutbackFactor)
-                        prevTS[s] = None
-                        self.history[s].append(np.inf)
-                dU = torch.where(perDof(~ok), torch.zeros_like(dU), dU)
-        except (ReachedMaxIncrements, ReachedMinIncrementSize):
-            raise StepFailed()
+                    except (ReachedMaxIncrements, ReachedMinIncrementSize):
+                        ts[s] = None
+                        self.failedSpecimens.append(s)
+                    finished[s] = ts[s] is None
+            live = ~finished
+            if not live.any():
+                break
+            self.rounds += 1
+            dU, PExt, deltaT, extrap = self._startIncrements(models, steps, ts, prevTS, live, owners, dU, isDir,
+                                                             perDof)
+            self._cutbackSize = np.zeros(S)
+            state, it = self._newtonRound(sysm, steps, live, extrap, hasDir, dU, P, F, PExt, deltaT, dirT,
+                                          perDof, nAcc, prec, rtol)
+            dU = self._lastDU
+            ok = state == CONVERGED
+            sysm.accept_state_batched(ok)
+            U = torch.where(perDof(ok), U + dU, U)
+            for s in np.flatnonzero(live):
+                if ok[s]:
+                    prevTS[s] = ts[s]
+                    if it[s] >= steps[s].criticalIter:
+                        steps[s].preventIncrementIncrease()
+                    models[s].time = ts[s].totalTime
+                    self.history[s].append(int(it[s]))
+                else:
+                    prevTS[s] = None
+                    try:
+                        steps[s].discardAndChangeIncrement(max(self._cutbackSize[s], steps[s].cutbackFactor))
+                    except (ReachedMaxIncrements, ReachedMinIncrementSize):
+                        finished[s] = True
+                        self.failedSpecimens.append(s)
+                        continue
+                    self.history[s].append(np.inf)
+            dU = torch.where(perDof(~ok), torch.zeros_like(dU), dU)
         for s, st in enumerate(steps):
             for group in ("dirichlet", "nodeforces"):
                 for action in st.actions.get(group, {}).values():
                     action.applyAtStepEnd(models[s])
-        torch.cuda.synchronize(dev)
+        if dev.type == "cuda":
+            torch.cuda.synchronize(dev)
         Uh = U.cpu().numpy()
         for s, m in enumerate(models):
             m.U = Uh[self.dofOffset[s]:self.dofOffset[s + 1]].copy()
-        return True, models
+        return not self.failedSpecimens, models
 
     # ---- one lockstep round ------------------------------------------------------------------------------
     def _startIncrements(self, models, steps, ts, prevTS, live, owners, dU, isDir, perDof):
@@ -240,7 +257,12 @@ class NISTBatched:
             self.elementPasses += 1
             self.specimenPasses += state == ITERATING
             if st == _lib.EFE_CUTBACK:
-                raise StepFailed("a material requested a cutback inside a batch")
+                # only the specimens still iterating count: the others keep results of a finished pass
+                asked = sysm.specimen_status() & (state == ITERATING)
+                state[asked] = FAILED
+                for s in np.flatnonzero(asked):
+                    self.failures[s].append("cutback")
+                self._cutbackSize[asked] = 0.5
             R = P + PExt if PExt is not None else P.clone()
             if dirT.numel():
                 R[dirT] = dirVals[dirT] if dirVals is not None else 0.0
@@ -255,10 +277,12 @@ class NISTBatched:
                 grow[s] = (flux, nGrew + (1 if flux > prev else 0))
                 if nrm[s, 3]:
                     state[s] = FAILED
+                    self.failures[s].append("nan")
                 elif correction < FIELD_CORRECTION_TOL and flux <= fluxThreshold[s]:
                     state[s] = CONVERGED
                 elif grow[s][1] > steps[s].maxGrowIter or it[s] == steps[s].maxIter:
                     state[s] = FAILED
+                    self.failures[s].append("iterations")
             go = state == ITERATING
             if not go.any():
                 break
@@ -277,6 +301,8 @@ class NISTBatched:
             if info.status == _lib.EFE_NAN:
                 bad |= go & ~np.isfinite(sysm.norms_batched(ddU, None, ddU)[:, 0])
             state[bad] = FAILED
+            for s in np.flatnonzero(bad):
+                self.failures[s].append("krylov")
             dU = dU + ddU
             it[go] += 1
         self._lastDU = dU

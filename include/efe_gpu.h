@@ -180,6 +180,11 @@ int efe_linsolve_batched(efe_ctx* ctx, const double* b, double* x, const efe_sol
  * femodel.py:245-258 per specimen model. */
 int efe_accept_state_batched(efe_ctx* ctx, const uint8_t* accept);
 
+/* After an efe_compute_elements that returned EFE_CUTBACK on a batched context: out[s] = 1
+ * for the specimens whose material asked for a smaller increment (host [n_specimens]);
+ * the CutbackRequest of vonmises.py:230-231 raised per specimen model. */
+int efe_get_specimen_status(efe_ctx* ctx, uint8_t* out);
+
 /* Number of specimens of the context (1 for an ordinary context). */
 int efe_get_specimen_count(efe_ctx* ctx);
 

@@ -1,0 +1,133 @@
+"""A stand-in for DeviceSystem under NISTBatched, computed with the oracle (CPU only).
+
+It lets the batched driver's control flow (lockstep rounds, per-specimen convergence,
+material cutback requests, step failures, state acceptance) be checked on the CPU against
+each specimen's own oracle NIST run.  Element routines are the oracle's; linear solves are
+direct, with the prescribed DOFs eliminated as on the device: rows and columns of the
+Dirichlet DOFs zeroed with a unit diagonal, and the prescribed values' column contribution
+returned as dir_correction on the first iteration of an increment.
+"""
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+import torch
+
+from edelweissfe_amd import _lib
+from oracle import assembly
+from oracle.exceptions import CutbackRequest
+from oracle.nist import NIST
+
+
+class OracleBatchSystem:
+    def __init__(self, oracle_models):
+        self.models = oracle_models
+        self.n_specimens = len(oracle_models)
+        self.nist = []
+        for om in oracle_models:
+            so = NIST()
+            so._setup(om)
+            self.nist.append(so)
+        self.dofOffset = np.r_[0, np.cumsum([om.nDof for om in oracle_models])]
+        self.n_dof = int(self.dofOffset[-1])
+        self.device = torch.device("cpu")
+        self.ref = [om.state.copy() for om in oracle_models]
+        self.tmp = [om.state.copy() for om in oracle_models]
+        self.status = np.zeros(self.n_specimens, dtype=bool)
+        self.K = [None] * self.n_specimens
+        self.dir = np.zeros(0, dtype=np.int64)
+        self.passes = 0
+
+    def _sl(self, s):
+        return slice(int(self.dofOffset[s]), int(self.dofOffset[s + 1]))
+
+    def set_dirichlet(self, dofs):
+        self.dir = np.unique(np.asarray(dofs, dtype=np.int64))
+
+    def _eliminated(self, s):
+        sl = self._sl(s)
+        d = self.dir[(self.dir >= sl.start) & (self.dir < sl.stop)] - sl.start
+        keep = np.ones(self.K[s].shape[0])
+        keep[d] = 0.0
+        D = sp.diags(keep)
+        return (D @ self.K[s] @ D + sp.diags(1.0 - keep)).tocsc(), d
+
+    def compute_elements(self, dU, P, F, dir_values=None, dir_correction=None, precision=_lib.EFE_FP64, **kw):
+        self.passes += 1
+        dUh = dU.numpy()
+        Ph, Fh = np.zeros(self.n_dof), np.zeros(self.n_dof)
+        corr = np.zeros(self.n_dof)
+        dv = dir_values.numpy() if dir_values is not None else None
+        for s, (om, so) in enumerate(zip(self.models, self.nist)):
+            sl = self._sl(s)
+            om.state = self.ref[s]
+            try:
+                V, p, f, st = so.compute_elements(om, dUh[sl])
+            except CutbackRequest:
+                self.status[s] = True
+                continue
+            self.status[s] = False
+            self.tmp[s] = st
+            self.K[s] = assembly.csr_matrix(assembly.update_csr(V, so.x, so.nnz), so.indptr, so.indices)
+            Ph[sl], Fh[sl] = p, f
+            if dv is not None:
+                d = self.dir[(self.dir >= sl.start) & (self.dir < sl.stop)] - sl.start
+                delta = np.zeros(om.nDof)
+                delta[d] = dv[sl][d]
+                c = self.K[s] @ delta
+                c[d] = 0.0
+                corr[sl] = c
+        P.copy_(torch.from_numpy(Ph))
+        F.copy_(torch.from_numpy(Fh))
+        if dir_correction is not None:
+            dir_correction.copy_(torch.from_numpy(corr))
+        return (_lib.EFE_CUTBACK if self.status.any() else _lib.EFE_OK), (0.5 if self.status.any() else 1.0)
+
+    def specimen_status(self):
+        return self.status.copy()
+
+    def norms_batched(self, R, ddU, F):
+        out = np.zeros((self.n_specimens, 4))
+        Rh, Fh = R.numpy(), F.numpy()
+        dh = ddU.numpy() if ddU is not None else None
+        for s in range(self.n_specimens):
+            sl = self._sl(s)
+            out[s, 0] = np.abs(Rh[sl]).max()
+            out[s, 1] = np.abs(dh[sl]).max() if dh is not None else 0.0
+            out[s, 2] = np.abs(Fh[sl]).sum()
+            out[s, 3] = float(not np.isfinite(Rh[sl]).all() or (dh is not None and not np.isfinite(dh[sl]).all()))
+        return out
+
+    def linsolve_batched(self, b, x, spec_atol, spec_active=None, precision=_lib.EFE_FP64, rtol=0.0, max_iter=0,
+                         check_every=0):
+        bh = b.numpy()
+        xh = np.zeros(self.n_dof)
+        its = np.zeros(self.n_specimens, dtype=np.int32)
+        for s in range(self.n_specimens):
+            if spec_active is not None and not spec_active[s]:
+                continue
+            A, _ = self._eliminated(s)
+            xh[self._sl(s)] = spla.spsolve(A, bh[self._sl(s)])
+            its[s] = 1
+        x.copy_(torch.from_numpy(xh))
+        info = _lib.SolveInfo()
+        info.status = _lib.EFE_OK
+        return info, its
+
+    def accept_state_batched(self, accept):
+        for s, a in enumerate(accept):
+            if a:
+                self.ref[s] = self.tmp[s].copy()
+
+    def qp_state(self, which=0):
+        return np.concatenate(self.ref if which == 0 else self.tmp)
+
+
+def attach_mock(solver, device_models, oracle_models):
+    """Points a NISTBatched at an OracleBatchSystem built from `oracle_models` (the same
+    meshes as `device_models`)."""
+    solver.system = OracleBatchSystem(oracle_models)
+    solver.nodeOffset = list(np.r_[0, np.cumsum([m.nNodes for m in device_models])])
+    solver.elementOffset = list(np.r_[0, np.cumsum([m.nElements for m in device_models])])
+    solver.dofOffset = [3 * o for o in solver.nodeOffset]
+    return solver.system

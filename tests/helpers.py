@@ -4,6 +4,7 @@ import numpy as np
 
 from edelweissfe_amd.generators.boxgen import generateBox
 from edelweissfe_amd.steps import AdaptiveStep, BodyForce, Dirichlet, NodeForces
+from oracle import exceptions as oexc
 from oracle import materials as omat
 from oracle import mesh as omesh
 from oracle import nist as onist
@@ -54,3 +55,38 @@ def rel_err(a, b):
     b = np.asarray(b)
     scale = max(np.abs(b).max(), 1e-300)
     return float(np.abs(a - b).max() / scale)
+
+
+# exponential hardening fy = fy0 + H k + dfy (1 - exp(-delta k)) with dfy, delta < 0: the
+# local Newton of the return mapping starts below the root, overshoots and then comes back
+# by about 1/|delta| a step, so in a full increment it fails with |R| ~ 0.6 left after 15
+# steps, far from the 1e-12 tolerance, while the halved increments converge in at most 11
+# steps.  Stresses are kept small (E = 2100, fy0 = 2.5) so that a converged local Newton
+# ends at |R| <= 3e-14, well below the absolute 1e-12 tolerance: at steel-like magnitudes
+# (fy0 = 250) the last steps stall at |R| ~ 1e-12 from rounding alone and whether a QP
+# "fails" depends on the last bits of its input.  The overshoot also grows exponentially
+# with the strain increment, so the outcome of a later increment depends on the global
+# Newton path: a batched solve that stops its Krylov iterations at the default 1e-2 of the
+# flux tolerance takes another path than the oracle's direct solves and cuts back
+# elsewhere; runs compared with the oracle on these specimens set krylovAtolFactor = 1e-8.
+# Specimen 0 asks for cutbacks and then completes; specimen 2, the same material with
+# maxNumInc = 1, fails with ReachedMaxIncrements (StepFailed of its oracle run) after its
+# cutbacks; specimen 1 is a plain J2 specimen.
+EXP_HARDENING = [2100.0, 0.3, 2.5, 1.0, -1e-8, -1e4]
+CUTBACK_SPECIMENS = [
+    ((2, 2, 2), EXP_HARDENING, 0.005, 1.0, 200, 1),
+    ((3, 3, 3), [180000.0, 0.3, 300.0, 1000.0, 0.0, 0.0], 0.020, 0.5, 200, 2),
+    ((2, 3, 2), EXP_HARDENING, 0.005, 1.0, 1, 4),
+]
+
+
+def cutback_oracle_run(n, props, strain, maxInc, maxNumInc, seed):
+    om = oracle_model(n, (1.0, 1.0, 1.0), "C3D8", "vonmises", props, jitter=0.1, seed=seed)
+    so = onist.NIST()
+    try:
+        so.solve_step(om, oracle_step(uniaxial_bcs(strain), maxInc=maxInc, minInc=1e-6, maxNumInc=maxNumInc,
+                                      maxIter=15))
+        failed = False
+    except oexc.StepFailed:
+        failed = True
+    return so, om, failed

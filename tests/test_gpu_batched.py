@@ -4,7 +4,8 @@ own oracle NIST run (increments, cutbacks, Newton counts, U and QP stress)."""
 import numpy as np
 import pytest
 
-from helpers import device_model, device_step, oracle_model, oracle_step, rel_err, uniaxial_bcs
+from helpers import (CUTBACK_SPECIMENS, cutback_oracle_run, device_model, device_step, oracle_model, oracle_step,
+                     rel_err, uniaxial_bcs)
 from oracle.nist import NIST
 
 torch = pytest.importorskip("torch")
@@ -124,3 +125,30 @@ def test_batched_norms_solve_and_selective_accept():
     assert not ref_state[:ea].any()
     assert np.array_equal(ref_state[ea:], tmp_state[ea:])
     assert np.abs(tmp_state[ea:, :, 0:6]).max() > 0
+
+
+def test_material_cutbacks_and_step_failure_stay_per_specimen():
+    from edelweissfe_amd.solvers.nistbatched import NISTBatched
+
+    models, steps = [], []
+    for n, props, strain, maxInc, maxNumInc, seed in CUTBACK_SPECIMENS:
+        pm = device_model(n, (1.0, 1.0, 1.0), "C3D8", "vonmises", props, jitter=0.1, seed=seed)
+        models.append(pm)
+        steps.append(device_step(pm, uniaxial_bcs(strain), maxInc=maxInc, minInc=1e-6, maxNumInc=maxNumInc,
+                                 maxIter=15))
+    solver = NISTBatched(precision="fp64", autotune=False, krylovAtolFactor=1e-8)
+    ok, _ = solver.solveSteps(models, steps)
+    states = solver.qpStates(models)
+    for s, pm in enumerate(models):
+        so, om, failed = cutback_oracle_run(*CUTBACK_SPECIMENS[s])
+        want = [np.inf if h[2] is None else h[2] for h in so.history]
+        print(s, solver.history[s], want, solver.failures[s], flush=True)
+        assert failed == (s == 2)
+        assert solver.history[s] == want
+        # the state and time of the last accepted increment
+        assert rel_err(states[s][..., 0:6], om.state[..., 0:6]) < 1e-9
+        assert pm.time == pytest.approx(om.time)
+        if not failed:
+            assert rel_err(pm.U, om.U) < 1e-9
+    assert not ok and solver.failedSpecimens == [2]
+    assert np.inf in solver.history[0] and solver.history[0][-1] != np.inf
