@@ -58,27 +58,64 @@ __device__ __forceinline__ int xcd_block() {
 
 __host__ __forceinline__ int round8(int n) { return (n + 7) & ~7; }
 
-template <int BS>
-__device__ __forceinline__ double block_sum(double v, double* sh) {
-    // deterministic tree: fixed pairing order
-    sh[threadIdx.x] = v;
-    __syncthreads();
+// Deterministic workgroup sums.  Within a wavefront an xor butterfly: the two lanes of a
+// pair add the same two values (a + b == b + a exactly), so after every step all lanes
+// hold bitwise identical sums.  Across the BS/64 wavefronts the per-wave sums are added in
+// wave order.  One barrier pair per call, however many values are summed together.
+__device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
-    for (int s = BS / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    double r = sh[0];
-    __syncthreads();
-    return r;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
 
-// Sum of a partial array in a fixed order, computed identically by every workgroup.
+template <int BS, int K>
+__device__ __forceinline__ void block_sums(double (&v)[K], double* sh) {
+    constexpr int NW = BS / 64;
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+    if (NW == 1) return;
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) sh[k * NW + w] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double r = sh[k * NW];
+        for (int i = 1; i < NW; ++i) r += sh[k * NW + i];
+        v[k] = r;
+    }
+    __syncthreads();
+}
+
+template <int BS>
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    double a[1] = {v};
+    block_sums<BS, 1>(a, sh);
+    return a[0];
+}
+
+// Sums of K partial arrays (each n long, at part + offs[k]) in a fixed order, computed
+// identically by every workgroup.
+template <int BS, int K>
+__device__ __forceinline__ void reduce_partials_n(const double* __restrict__ part, const int (&offs)[K], int n,
+                                                  double (&out)[K], double* sh) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = 0.0;
+    for (int i = threadIdx.x; i < n; i += BS) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) out[k] += part[offs[k] + i];
+    }
+    block_sums<BS, K>(out, sh);
+}
+
 template <int BS>
 __device__ __forceinline__ double reduce_partials(const double* __restrict__ part, int n, double* sh) {
-    double v = 0.0;
-    for (int i = threadIdx.x; i < n; i += BS) v += part[i];
-    return block_sum<BS>(v, sh);
+    const int offs[1] = {0};
+    double out[1];
+    reduce_partials_n<BS, 1>(part, offs, n, out, sh);
+    return out[0];
 }
 
 template <typename T, int BS>
@@ -156,8 +193,9 @@ __global__ __launch_bounds__(BS) void pcg_init_kernel(int N, const double* __res
             rz += (double)rv[i] * zi;
         }
     }
-    double s1 = block_sum<BS>(rz, sh);
-    double s2 = block_sum<BS>(bb, sh);
+    double sums[2] = {rz, bb};
+    block_sums<BS, 2>(sums, sh);
+    const double s1 = sums[0], s2 = sums[1];
     if (threadIdx.x == 0) {
         part[1 * nwg + blk] = s1;
         part[4 * nwg + blk] = s2;
@@ -177,8 +215,10 @@ __global__ __launch_bounds__(BS) void pcg_update_kernel(int N, const T* __restri
                                                         const int* __restrict__ flags, int kr) {
     __shared__ double sh[BS];
     if (flags[0]) return;
-    const double pq = reduce_partials<BS>(part, nwg, sh);
-    const double rz = reduce_partials<BS>(part + (1 + par) * nwg, nwg, sh);
+    const int offs[2] = {0, (1 + par) * nwg};
+    double sc[2];
+    reduce_partials_n<BS, 2>(part, offs, nwg, sc, sh);
+    const double pq = sc[0], rz = sc[1];
     const T alpha = (T)(rz / pq);
     const int blk = xcd_block();
     double rzn = 0, rr = 0;
@@ -201,8 +241,9 @@ __global__ __launch_bounds__(BS) void pcg_update_kernel(int N, const T* __restri
             rzn += (double)rv[i] * zi;
         }
     }
-    double s1 = block_sum<BS>(rzn, sh);
-    double s2 = block_sum<BS>(rr, sh);
+    double sums[2] = {rzn, rr};
+    block_sums<BS, 2>(sums, sh);
+    const double s1 = sums[0], s2 = sums[1];
     if (threadIdx.x == 0) {
         part[(2 - par) * nwg + blk] = s1;
         part[3 * nwg + blk] = s2;
@@ -216,9 +257,10 @@ __global__ __launch_bounds__(BS) void pcg_direction_kernel(int N, const T* __res
                                                            int* __restrict__ flags, int kr) {
     __shared__ double sh[BS];
     if (flags[0]) return;
-    const double rzo = reduce_partials<BS>(part + (1 + par) * nwg, nwg, sh);
-    const double rzn = reduce_partials<BS>(part + (2 - par) * nwg, nwg, sh);
-    const double rr = reduce_partials<BS>(part + 3 * nwg, nwg, sh);
+    const int offs[3] = {(1 + par) * nwg, (2 - par) * nwg, 3 * nwg};
+    double sc[3];
+    reduce_partials_n<BS, 3>(part, offs, nwg, sc, sh);
+    const double rzo = sc[0], rzn = sc[1], rr = sc[2];
     const bool bad = !(rr == rr) || !(rzn == rzn);
     const bool done = bad || rr <= dtol2[0];
     if (done) {

@@ -27,21 +27,27 @@
 
 namespace {
 
+// slots staged per chunk by the gather: 2048 x 12 B (+1/8 skew) = 27 KiB of LDS, one chunk
+// for the 256 nodes of a workgroup of an interior hexa8 region
+constexpr int kGatherChunk = 2048;
+
 __device__ __forceinline__ int xcd_block_mf() {
     const int per = gridDim.x >> 3;
     return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
 }
 
+// Deterministic workgroup sum: xor butterfly inside each wavefront (all lanes end with
+// bitwise identical sums), then the per-wave sums in wave order.
 template <int BS>
 __device__ __forceinline__ double block_sum_mf(double v, double* sh) {
-    sh[threadIdx.x] = v;
-    __syncthreads();
 #pragma unroll
-    for (int s = BS / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    const double r = sh[0];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    constexpr int NW = BS / 64;
+    if (NW == 1) return v;
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = sh[0];
+    for (int i = 1; i < NW; ++i) r += sh[i];
     __syncthreads();
     return r;
 }
@@ -206,20 +212,58 @@ __global__ __launch_bounds__(BS) void mf_gather_kernel(int N, const int32_t* __r
                                                        const uint8_t* __restrict__ dirflag,
                                                        TY* __restrict__ y, double* __restrict__ part,
                                                        const int* __restrict__ flags, int kr) {
-    __shared__ double sh[BS];
+    // The slots of the workgroup's BS consecutive nodes are one contiguous range of the
+    // node-sorted force list: it is staged through LDS in chunks with coalesced loads
+    // (consecutive lanes, consecutive words), then every lane sums its own node's slots in
+    // slot order, the same order as a direct per-node loop.
+    // component-major with a skew of one word per 8 slots: lane l starts near slot 8 l, so
+    // the skewed word index 9 l + .. falls into distinct banks
+    // (+1 so that the three components of one slot, stored by neighbouring lanes, differ in
+    // bank too)
+    constexpr int CH = kGatherChunk, CHP = kGatherChunk + kGatherChunk / 8 + 1;
+    constexpr int R = (3 * CH + BS - 1) / BS;  // staged words per lane
+    __shared__ float buf[3 * CHP];
+    __shared__ double sh[BS / 64];
     if (flags && flags[0]) return;
     const int blk = xcd_block_mf();
     double dot = 0.0;
     for (int it = 0; it < kr; ++it) {
-        const int n = (blk * kr + it) * BS + threadIdx.x;
-        if (n >= N) break;
+        const int n0 = (blk * kr + it) * BS;
+        if (n0 >= N) break;
+        const int n = n0 + threadIdx.x;
+        const bool valid = n < N;
+        const int s0 = nptr[n0], s1 = nptr[min(n0 + BS, N)];
+        const int mb = valid ? nptr[n] : 0, me = valid ? nptr[n + 1] : 0;
         float y0 = 0, y1 = 0, y2 = 0;
-        for (int c = nptr[n]; c < nptr[n + 1]; ++c) {  // slots in element order
-            const float* src = Fe + (int64_t)c * 3;
-            y0 += src[0];
-            y1 += src[1];
-            y2 += src[2];
+        for (int c0 = s0; c0 < s1; c0 += CH) {
+            const int cnt = min(CH, s1 - c0);
+            const float* src = Fe + (int64_t)c0 * 3;
+            // all loads first (independent, so they are in flight together), then the stores
+            float tmp[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = threadIdx.x + r * BS;
+                tmp[r] = i < 3 * cnt ? src[i] : 0.0f;
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = threadIdx.x + r * BS;
+                if (i < 3 * cnt) {
+                    const int c = i / 3, k = i - 3 * (i / 3);
+                    buf[k * CHP + c + (c >> 3)] = tmp[r];
+                }
+            }
+            __syncthreads();
+            const int lo = max(mb, c0), hi = min(me, c0 + cnt);
+            for (int c = lo; c < hi; ++c) {
+                const int w = (c - c0) + ((c - c0) >> 3);
+                y0 += buf[w];
+                y1 += buf[CHP + w];
+                y2 += buf[2 * CHP + w];
+            }
+            __syncthreads();
         }
+        if (!valid) continue;
         float yv[3] = {y0, y1, y2};
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
