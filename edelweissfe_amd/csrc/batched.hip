@@ -33,23 +33,58 @@ namespace {
 constexpr int BS = 256;
 enum { RZ = 0, ALPHA = 1, RR = 2, TOL2 = 3, BETA = 4, NSC = 5 };
 
-__device__ __forceinline__ double wsum(double v, double* sh) {
-    sh[threadIdx.x] = v;
+// Fixed-order workgroup sums: an xor butterfly inside each wavefront (all lanes end with
+// bitwise identical sums), then the per-wave sums in wave order.
+template <int K>
+__device__ __forceinline__ void wsums(double (&v)[K], double* sh) {
+    constexpr int NW = BS / 64;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k) sh[k * NW + (threadIdx.x >> 6)] = v[k];
     __syncthreads();
-    for (int s = BS / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double r = sh[k * NW];
+        for (int i = 1; i < NW; ++i) r += sh[k * NW + i];
+        v[k] = r;
     }
-    double r = sh[0];
     __syncthreads();
-    return r;
 }
 
-// fixed-order sum of a per-node array over [n0, n1)
+// Fixed-order sums of K per-node arrays over [n0, n1).  The order depends only on the
+// positions relative to n0, so a specimen sums the same way alone as inside a batch.
+// Four independent accumulators per lane keep four loads of each array in flight.
+template <int K>
+__device__ __forceinline__ void range_sums(const double* const (&v)[K], int n0, int n1, double (&out)[K],
+                                           double* sh) {
+    double a[4][K];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int k = 0; k < K; ++k) a[j][k] = 0.0;
+    for (int n = n0 + threadIdx.x; n < n1; n += 4 * BS) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int m = n + j * BS;
+            if (m < n1)
+#pragma unroll
+                for (int k = 0; k < K; ++k) a[j][k] += v[k][m];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = (a[0][k] + a[1][k]) + (a[2][k] + a[3][k]);
+    wsums<K>(out, sh);
+}
+
 __device__ __forceinline__ double range_sum(const double* __restrict__ v, int n0, int n1, double* sh) {
-    double s = 0.0;
-    for (int n = n0 + threadIdx.x; n < n1; n += BS) s += v[n];
-    return wsum(s, sh);
+    const double* const arr[1] = {v};
+    double out[1];
+    range_sums<1>(arr, n0, n1, out, sh);
+    return out[0];
 }
 
 template <typename T>
@@ -90,8 +125,10 @@ __global__ __launch_bounds__(BS) void b_init_reduce_kernel(const int64_t* __rest
                                                            int* __restrict__ iters) {
     __shared__ double sh[BS];
     const int s = blockIdx.x;
-    const double rz = range_sum(nrz, (int)noff[s], (int)noff[s + 1], sh);
-    const double rr = range_sum(nrr, (int)noff[s], (int)noff[s + 1], sh);
+    const double* const arr[2] = {nrz, nrr};
+    double sums[2];
+    range_sums<2>(arr, (int)noff[s], (int)noff[s + 1], sums, sh);
+    const double rz = sums[0], rr = sums[1];
     if (threadIdx.x == 0) {
         // stop rule ||r_s|| <= max(atol_s, rtol ||b_s||)
         const double t2 = fmax(sc[s * NSC + TOL2], rtol2 * rr);
@@ -192,8 +229,10 @@ __global__ __launch_bounds__(BS) void b_beta_kernel(const int64_t* __restrict__ 
     __shared__ double sh[BS];
     const int s = blockIdx.x;
     if (!active[s]) return;
-    const double rz = range_sum(nrz, (int)noff[s], (int)noff[s + 1], sh);
-    const double rr = range_sum(nrr, (int)noff[s], (int)noff[s + 1], sh);
+    const double* const arr[2] = {nrz, nrr};
+    double sums[2];
+    range_sums<2>(arr, (int)noff[s], (int)noff[s + 1], sums, sh);
+    const double rz = sums[0], rr = sums[1];
     if (threadIdx.x == 0) {
         sc[s * NSC + BETA] = rz / sc[s * NSC + RZ];
         sc[s * NSC + RZ] = rz;
@@ -420,8 +459,9 @@ __global__ __launch_bounds__(BS) void b_norms_kernel(const int64_t* __restrict__
     }
     const double dmax = sh[0];
     __syncthreads();
-    const double fs = wsum(sf, sh);
-    const double nn = wsum(nan, sh);
+    double sums[2] = {sf, nan};
+    wsums<2>(sums, sh);
+    const double fs = sums[0], nn = sums[1];
     if (threadIdx.x == 0) {
         out[s * 4 + 0] = rmax;
         out[s * 4 + 1] = dmax;

@@ -27,11 +27,29 @@ constexpr int kSlice = 64;
 // corner rows with 51-block mid-side rows) while rows stay near their mesh neighbours.
 constexpr int kSortWindow = 1024;
 
+// Element scratch: one 3x3 block (9 values) per contribution.  (Padding the blocks to 16-byte
+// vectors was measured slower on MI355X: the extra bytes cost more than the wider accesses
+// save.)
+constexpr int kBlockPitch = 9;
+
+template <typename KT>
+__device__ __forceinline__ void store_block(KT* dst, const KT (&a)[9]) {
+#pragma unroll
+    for (int v = 0; v < 9; ++v) dst[v] = a[v];
+}
+
+template <typename KT>
+__device__ __forceinline__ void add_block(const KT* src, KT (&acc)[9]) {
+#pragma unroll
+    for (int v = 0; v < 9; ++v) acc[v] += src[v];
+}
+
 struct HostPattern {
     int N = 0, E = 0, NN = 0;
     std::vector<int32_t> bptr, bcol;      // node adjacency (block CSR), columns ascending
     std::vector<int32_t> diagk;           // position of the diagonal block in each row
-    std::vector<int32_t> coff, clist;     // per block: contributing element blocks, element order
+    std::vector<int32_t> coff, cpos;      // per block slot: its contribution range; per element block: its position
+    std::vector<int32_t> epos;            // [a][e]: index of (e, a) in nlist
     std::vector<int32_t> nptr, nlist;     // per node: (e*NN + a) incidences, element order
     std::vector<int32_t> slice_off;       // [nslices+1] first padded block of each slice
     std::vector<int32_t> slice_deg;       // [nslices] padded row length of each slice
@@ -81,7 +99,9 @@ struct efe_ctx {
     double* shapeNe = nullptr;   // [Q][NN] at the evaluation points (QP coordinates)
     // matrix-free FP32 operator: [Q][NC][E] compact tangent, [E*NN][3] node-sorted force slots
     float *mfC = nullptr, *mfF = nullptr, *dnxi32 = nullptr;
-    int32_t* mfPos = nullptr;        // [NN][E] slot of (e, a) in the node-sorted incidence list
+    int32_t* elemPos = nullptr;  // [NN][E] index of (e, a) in the node-sorted incidence list nlist
+                                 // (force slot of the matrix-free operator; block-row (e, a) of the
+                                 // element scratch starts at block elemPos * NN)
     bool mfPass = false;             // the current element launch is a matrix-free pass
     int32_t* geoElems = nullptr; // [E] element subset of a geostatic initial state
     double* qpw = nullptr;       // [Q]
@@ -95,7 +115,7 @@ struct efe_ctx {
     int32_t* specOfElem = nullptr;   // [E]
 
     // device pattern
-    int32_t *bptr = nullptr, *bcol = nullptr, *coff = nullptr, *clist = nullptr;
+    int32_t *bptr = nullptr, *bcol = nullptr, *coff = nullptr, *cpos = nullptr;
     int32_t *nptr = nullptr, *nlist = nullptr, *diagk = nullptr;
     int32_t *slice_off = nullptr, *scol = nullptr;
     int32_t *slotNode = nullptr, *nodeSlot = nullptr;  // operator row slot <-> node

@@ -6,15 +6,18 @@
 //            evaluation point, dN/dx, strain increment from the gathered dU,
 //            FP64 material update of the reference state into the temporary
 //            state, and (w detJ) C and (w detJ) sigma staged in LDS;
-//   phase 2, one lane per (element, node a, node b): the 3x3 block
-//            K_ab = sum_q B_a^T (w detJ C_q) B_b in KT precision, written to the
-//            element scratch [e][a][b][9];
+//   phase 2, one lane per (element, node a, node b), b >= a for symmetric
+//            tangents: the 3x3 block
+//            K_ab = sum_q B_a^T (w detJ C_q) B_b in KT precision (and K_ba = K_ab^T), written to the
+//            element scratch at position cpos[e][a][b] of the contribution order
+//            (block slot, then element), so that a row's contributions are
+//            contiguous for the assembly;
 //   phase 3, one lane per (element, node a): Pe_a = -sum_q B_a^T (w detJ sigma_q).
 //   Follows DisplacementElement.computeYourself
 //   (edelweissfe/elements/displacementelement/element.py:290-346).
 //
 // assemble_kernel: one thread per node row-block.  For every block slot of its
-//   row it sums the element blocks listed for that slot in element order (the
+//   row it sums the element blocks stored for that slot, in element order (the
 //   order of updateCSR, csrgenerator.pyx:112-113), eliminates Dirichlet rows and
 //   columns symmetrically, writes the sliced-ELL value layout, inverts its
 //   diagonal block for block Jacobi, and gathers P = sum Pe and F = sum |Pe|
@@ -44,7 +47,8 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
                                                      KT* __restrict__ Kscr, double* __restrict__ Pscr,
                                                      int* __restrict__ status, float* __restrict__ mfC,
                                                      int* __restrict__ specStatus,
-                                                     const int32_t* __restrict__ specOfElem) {
+                                                     const int32_t* __restrict__ specOfElem,
+                                                     const int32_t* __restrict__ cpos) {
     constexpr int NS = 12 + MatState<MAT>::n;
     constexpr int NP = (MAT == EFE_LINEAR_ELASTIC) ? 2 : 6;
     constexpr int NC = (MAT == EFE_DRUCKER_PRAGER) ? 11 : 9;
@@ -146,11 +150,27 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
     }
     __syncthreads();
 
-    constexpr int NB = MF ? NN : NN * NN;  // blocks per element formed here
+    // Symmetric tangents (linear elastic, J2): only the blocks a <= b are formed; K_ba is
+    // stored as K_ab^T.  Drucker-Prager (non-associative in general) forms all NN^2 blocks.
+    constexpr bool SYM = !MF && MAT != EFE_DRUCKER_PRAGER;
+    constexpr int NB = MF ? NN : (SYM ? NN * (NN + 1) / 2 : NN * NN);  // blocks per element formed here
     for (int t = lane; t < EPW * NB; t += 64) {
-        const int el = t / NB;
-        const int ab = MF ? (t - el * NB) * (NN + 1) : t - el * NB;
-        const int a = ab / NN, b = ab - (ab / NN) * NN;
+        const int el = t / NB, r0 = t - (t / NB) * NB;
+        int a, b;
+        if (MF) {
+            a = b = r0;
+        } else if (SYM) {  // r0 -> (a, b), upper triangle row by row
+            int r = r0;
+            a = 0;
+            while (r >= NN - a) {
+                r -= NN - a;
+                ++a;
+            }
+            b = a + r;
+        } else {
+            a = r0 / NN;
+            b = r0 - a * NN;
+        }
         const int e = e0 + el;
         if (e >= E) continue;
         KT acc[9];
@@ -174,9 +194,12 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
                 acc[6 + j] += az * CB[2][j] + ax * CB[4][j] + ay * CB[5][j];
             }
         }
-        KT* dst = Kscr + ((int64_t)e * NN * NN + ab) * 9;
-#pragma unroll
-        for (int k = 0; k < 9; ++k) dst[k] = acc[k];
+        const int32_t* cp = cpos + (int64_t)e * NN * NN;
+        store_block(Kscr + (int64_t)cp[a * NN + b] * kBlockPitch, acc);
+        if (SYM && a != b) {
+            const KT tr[9] = {acc[0], acc[3], acc[6], acc[1], acc[4], acc[7], acc[2], acc[5], acc[8]};
+            store_block(Kscr + (int64_t)cp[b * NN + a] * kBlockPitch, tr);
+        }
     }
 
     for (int t = lane; t < EPW * NN; t += 64) {
@@ -216,14 +239,19 @@ __device__ __forceinline__ void inv3(const double m[9], double o[9]) {
 template <typename KT>
 __global__ __launch_bounds__(256) void assemble_kernel(
     int N, const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol, const int32_t* __restrict__ coff,
-    const int32_t* __restrict__ clist, const int32_t* __restrict__ nptr, const int32_t* __restrict__ nlist,
+    const int32_t* __restrict__ nptr, const int32_t* __restrict__ nlist,
     const int32_t* __restrict__ slice_off, const KT* __restrict__ Kscr, const double* __restrict__ Pscr,
     const uint8_t* __restrict__ dirflag, int ndir, const double* __restrict__ dirv, double* __restrict__ dircorr,
     KT* __restrict__ Kval, KT* __restrict__ Minv, double* __restrict__ P, double* __restrict__ F,
-    const int32_t* __restrict__ nodeSlot) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    const int ps = nodeSlot[n];  // row slot of node n in the sliced layout
+    const int32_t* __restrict__ slotNode) {
+    // Threads walk the row slots of the sliced layout (so the value stores of a wavefront
+    // are contiguous), and the workgroups of one XCD own one contiguous slot range (so the
+    // element blocks that neighbouring rows share are read through the same L2).
+    const int per = gridDim.x >> 3;
+    const int blk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int ps = blk * blockDim.x + threadIdx.x;
+    if (ps >= N) return;
+    const int n = slotNode[ps];
     const int s = ps / kSlice, l = ps - (ps / kSlice) * kSlice;
     const int64_t vbase = (int64_t)slice_off[s] * 9 + l;
     bool fr[3] = {false, false, false};
@@ -240,11 +268,10 @@ __global__ __launch_bounds__(256) void assemble_kernel(
         KT acc[9];
 #pragma unroll
         for (int v = 0; v < 9; ++v) acc[v] = (KT)0;
-        for (int c = coff[b]; c < coff[b + 1]; ++c) {
-            const KT* src = Kscr + (int64_t)clist[c] * 9;
-#pragma unroll
-            for (int v = 0; v < 9; ++v) acc[v] += src[v];
-        }
+        // the slot's contributions are contiguous and in element order; unrolled so that
+        // the loads of several contributions are in flight together
+#pragma unroll 4
+        for (int c = coff[b]; c < coff[b + 1]; ++c) add_block(Kscr + (int64_t)c * kBlockPitch, acc);
         if (dirv) {
             const double d0 = dirv[3 * m], d1 = dirv[3 * m + 1], d2 = dirv[3 * m + 2];
             t0 += (double)acc[0] * d0 + (double)acc[1] * d1 + (double)acc[2] * d2;
@@ -377,11 +404,11 @@ void launch_elem_t(efe_ctx* c, const double* dU) {
     if (c->mfPass)
         element_kernel<NN, Q, MAT, float, EPW, true><<<grid, 64, 0, c->stream>>>(
             c->E, c->N, c->coords, c->conn, c->props, propStride, c->dNdxi, c->qpw, dU, c->stateRef, c->stateTmp,
-            (float*)c->Kscr, c->Pscr, c->status, c->mfC, c->specStatus, c->specOfElem);
+            (float*)c->Kscr, c->Pscr, c->status, c->mfC, c->specStatus, c->specOfElem, c->cpos);
     else
         element_kernel<NN, Q, MAT, KT, EPW, false><<<grid, 64, 0, c->stream>>>(
             c->E, c->N, c->coords, c->conn, c->props, propStride, c->dNdxi, c->qpw, dU, c->stateRef, c->stateTmp,
-            (KT*)c->Kscr, c->Pscr, c->status, nullptr, c->specStatus, c->specOfElem);
+            (KT*)c->Kscr, c->Pscr, c->status, nullptr, c->specStatus, c->specOfElem, c->cpos);
 }
 
 template <int NN, int Q, int MAT, typename KT>
@@ -438,15 +465,15 @@ int efe_launch_elements(efe_ctx* c, const double* dU, int precision) {
 
 int efe_launch_assembly(efe_ctx* c, double* P, double* F, const double* dir_values, double* dir_correction,
                         int precision) {
-    const int bs = 256, grid = (c->N + bs - 1) / bs;
+    const int bs = 256, grid = (((c->N + bs - 1) / bs) + 7) & ~7;  // a multiple of 8 (XCD mapping)
     if (precision == EFE_FP32) {
         assemble_kernel<float><<<grid, bs, 0, c->stream>>>(
-            c->N, c->bptr, c->bcol, c->coff, c->clist, c->nptr, c->nlist, c->slice_off, (const float*)c->Kscr,
-            c->Pscr, c->dirflag, c->ndir, dir_values, dir_correction, c->K32, c->Minv32, P, F, c->nodeSlot);
+            c->N, c->bptr, c->bcol, c->coff, c->nptr, c->nlist, c->slice_off, (const float*)c->Kscr,
+            c->Pscr, c->dirflag, c->ndir, dir_values, dir_correction, c->K32, c->Minv32, P, F, c->slotNode);
     } else {
         assemble_kernel<double><<<grid, bs, 0, c->stream>>>(
-            c->N, c->bptr, c->bcol, c->coff, c->clist, c->nptr, c->nlist, c->slice_off, (const double*)c->Kscr,
-            c->Pscr, c->dirflag, c->ndir, dir_values, dir_correction, c->K64, c->Minv64, P, F, c->nodeSlot);
+            c->N, c->bptr, c->bcol, c->coff, c->nptr, c->nlist, c->slice_off, (const double*)c->Kscr,
+            c->Pscr, c->dirflag, c->ndir, dir_values, dir_correction, c->K64, c->Minv64, P, F, c->slotNode);
     }
     EFE_HIP(hipGetLastError());
     c->Kprec = precision;

@@ -52,6 +52,57 @@ __device__ __forceinline__ double block_sum_mf(double v, double* sh) {
     return r;
 }
 
+// Sum of the force slots of node n0 + threadIdx.x, for the BS consecutive nodes
+// n0 .. n0 + BS - 1 of a workgroup (block-uniform call).  Their slots are one contiguous
+// range of the node-sorted force list; it is staged through LDS in chunks with coalesced
+// loads (consecutive lanes, consecutive words), then every lane sums its own node's slots
+// in slot order, the same order as a direct per-node loop.  LDS layout: component-major
+// with a skew of one word per 8 slots (lane l starts near slot 8 l, so the skewed word
+// index 9 l + .. falls into distinct banks) and a row pitch of CHP words (CHP = 1 mod 64,
+// so the three components of one slot, stored by neighbouring lanes, differ in bank too).
+// A workgroup in which no lane wants its sum (frozen specimens) reads nothing.
+template <int BS>
+__device__ __forceinline__ void gather_slots(int n0, int N, const int32_t* __restrict__ nptr,
+                                             const float* __restrict__ Fe, bool want, float* buf, float (&y)[3]) {
+    constexpr int CH = kGatherChunk, CHP = kGatherChunk + kGatherChunk / 8 + 1;
+    constexpr int R = (3 * CH + BS - 1) / BS;  // staged words per lane
+    y[0] = y[1] = y[2] = 0.0f;
+    if (!__syncthreads_or(want)) return;
+    const int n = n0 + threadIdx.x;
+    const int s0 = nptr[n0], s1 = nptr[min(n0 + BS, N)];
+    const int mb = want ? nptr[n] : 0, me = want ? nptr[n + 1] : 0;
+    for (int c0 = s0; c0 < s1; c0 += CH) {
+        const int cnt = min(CH, s1 - c0);
+        const float* src = Fe + (int64_t)c0 * 3;
+        // all loads first (independent, so they are in flight together), then the stores
+        float tmp[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = threadIdx.x + r * BS;
+            tmp[r] = i < 3 * cnt ? src[i] : 0.0f;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = threadIdx.x + r * BS;
+            if (i < 3 * cnt) {
+                const int c = i / 3, k = i - 3 * (i / 3);
+                buf[k * CHP + c + (c >> 3)] = tmp[r];
+            }
+        }
+        __syncthreads();
+        const int lo = max(mb, c0), hi = min(me, c0 + cnt);
+        for (int c = lo; c < hi; ++c) {
+            const int w = (c - c0) + ((c - c0) >> 3);
+            y[0] += buf[w];
+            y[1] += buf[CHP + w];
+            y[2] += buf[2 * CHP + w];
+        }
+        __syncthreads();
+    }
+}
+
+constexpr int kGatherLds = 3 * (kGatherChunk + kGatherChunk / 8 + 1);
+
 // One thread per element: element force f_e = K_e p_e.  J is rebuilt from the node
 // coordinates relative to the element's first node (formed in FP64, so that FP32 keeps the
 // element's own scale), the compact tangent is read per QP, and f_a is written to the
@@ -168,20 +219,17 @@ __global__ __launch_bounds__(256) void mf_gather_batched_kernel(int N, const int
                                                                 const int32_t* __restrict__ spec,
                                                                 const int* __restrict__ active, float* __restrict__ q,
                                                                 double* __restrict__ npq) {
-    const int n = blockIdx.x * 256 + threadIdx.x;
-    if (n >= N) return;
-    if (!active[spec[n]]) {
+    __shared__ float buf[kGatherLds];
+    const int n0 = blockIdx.x * 256, n = n0 + threadIdx.x;
+    const bool valid = n < N;
+    const bool act = valid && active[spec[n]];
+    float yv[3];
+    gather_slots<256>(n0, N, nptr, Fe, act, buf, yv);
+    if (!valid) return;
+    if (!act) {
         npq[n] = 0.0;
         return;
     }
-    float y0 = 0, y1 = 0, y2 = 0;
-    for (int c = nptr[n]; c < nptr[n + 1]; ++c) {
-        const float* src = Fe + (int64_t)c * 3;
-        y0 += src[0];
-        y1 += src[1];
-        y2 += src[2];
-    }
-    float yv[3] = {y0, y1, y2};
     double dot = 0.0;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -191,15 +239,6 @@ __global__ __launch_bounds__(256) void mf_gather_batched_kernel(int N, const int
         dot += (double)pv * yv[i];
     }
     npq[n] = dot;
-}
-
-// pos[a][e] = index of (e, a) in the node-sorted incidence list nlist (entries e*NN + a).
-__global__ void mf_pos_kernel(int64_t L, int E, int NN, const int32_t* __restrict__ nlist, int32_t* __restrict__ pos) {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= L) return;
-    const int v = nlist[c];
-    const int e = v / NN, a = v - (v / NN) * NN;
-    pos[(int64_t)a * E + e] = (int32_t)c;
 }
 
 // MODE 0 (operator): y = sum of the node's element forces in element order on free DOFs,
@@ -212,17 +251,7 @@ __global__ __launch_bounds__(BS) void mf_gather_kernel(int N, const int32_t* __r
                                                        const uint8_t* __restrict__ dirflag,
                                                        TY* __restrict__ y, double* __restrict__ part,
                                                        const int* __restrict__ flags, int kr) {
-    // The slots of the workgroup's BS consecutive nodes are one contiguous range of the
-    // node-sorted force list: it is staged through LDS in chunks with coalesced loads
-    // (consecutive lanes, consecutive words), then every lane sums its own node's slots in
-    // slot order, the same order as a direct per-node loop.
-    // component-major with a skew of one word per 8 slots: lane l starts near slot 8 l, so
-    // the skewed word index 9 l + .. falls into distinct banks
-    // (+1 so that the three components of one slot, stored by neighbouring lanes, differ in
-    // bank too)
-    constexpr int CH = kGatherChunk, CHP = kGatherChunk + kGatherChunk / 8 + 1;
-    constexpr int R = (3 * CH + BS - 1) / BS;  // staged words per lane
-    __shared__ float buf[3 * CHP];
+    __shared__ float buf[kGatherLds];
     __shared__ double sh[BS / 64];
     if (flags && flags[0]) return;
     const int blk = xcd_block_mf();
@@ -232,39 +261,9 @@ __global__ __launch_bounds__(BS) void mf_gather_kernel(int N, const int32_t* __r
         if (n0 >= N) break;
         const int n = n0 + threadIdx.x;
         const bool valid = n < N;
-        const int s0 = nptr[n0], s1 = nptr[min(n0 + BS, N)];
-        const int mb = valid ? nptr[n] : 0, me = valid ? nptr[n + 1] : 0;
-        float y0 = 0, y1 = 0, y2 = 0;
-        for (int c0 = s0; c0 < s1; c0 += CH) {
-            const int cnt = min(CH, s1 - c0);
-            const float* src = Fe + (int64_t)c0 * 3;
-            // all loads first (independent, so they are in flight together), then the stores
-            float tmp[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int i = threadIdx.x + r * BS;
-                tmp[r] = i < 3 * cnt ? src[i] : 0.0f;
-            }
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int i = threadIdx.x + r * BS;
-                if (i < 3 * cnt) {
-                    const int c = i / 3, k = i - 3 * (i / 3);
-                    buf[k * CHP + c + (c >> 3)] = tmp[r];
-                }
-            }
-            __syncthreads();
-            const int lo = max(mb, c0), hi = min(me, c0 + cnt);
-            for (int c = lo; c < hi; ++c) {
-                const int w = (c - c0) + ((c - c0) >> 3);
-                y0 += buf[w];
-                y1 += buf[CHP + w];
-                y2 += buf[2 * CHP + w];
-            }
-            __syncthreads();
-        }
+        float yv[3];
+        gather_slots<BS>(n0, N, nptr, Fe, valid, buf, yv);
         if (!valid) continue;
-        float yv[3] = {y0, y1, y2};
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             const bool d = dirflag[3 * n + i];  // always allocated; not ndir, which a captured graph would freeze
@@ -288,7 +287,7 @@ __global__ __launch_bounds__(BS) void mf_gather_kernel(int N, const int32_t* __r
 // block-Jacobi inverse, and the P / F gather of the element residuals.
 __global__ __launch_bounds__(256) void mf_diag_kernel(
     int N, const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol, const int32_t* __restrict__ coff,
-    const int32_t* __restrict__ clist, const int32_t* __restrict__ nptr, const int32_t* __restrict__ nlist,
+    const int32_t* __restrict__ nptr, const int32_t* __restrict__ nlist,
     const float* __restrict__ Kscr, const double* __restrict__ Pscr, const uint8_t* __restrict__ dirflag, int ndir,
     float* __restrict__ Minv, double* __restrict__ P, double* __restrict__ F) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -297,11 +296,7 @@ __global__ __launch_bounds__(256) void mf_diag_kernel(
     for (int b = bptr[n]; b < bptr[n + 1]; ++b) {
         if (bcol[b] != n) continue;
         float acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int c = coff[b]; c < coff[b + 1]; ++c) {
-            const float* src = Kscr + (int64_t)clist[c] * 9;
-#pragma unroll
-            for (int v = 0; v < 9; ++v) acc[v] += src[v];
-        }
+        for (int c = coff[b]; c < coff[b + 1]; ++c) add_block(Kscr + (int64_t)c * kBlockPitch, acc);
 #pragma unroll
         for (int v = 0; v < 9; ++v) dg[v] = acc[v];
     }
@@ -344,7 +339,7 @@ __global__ __launch_bounds__(256) void mf_diag_kernel(
 inline int round8_mf(int n) { return (n + 7) & ~7; }
 
 #define MF_ARGS \
-    c->E, c->N, c->conn, c->coords, c->dnxi32, c->mfC, p, c->dirflag, mask, flags, c->mfPos, c->mfF, active, \
+    c->E, c->N, c->conn, c->coords, c->dnxi32, c->mfC, p, c->dirflag, mask, flags, c->elemPos, c->mfF, active, \
         c->specOfElem
 
 template <typename TP>
@@ -384,9 +379,6 @@ int efe_mf_alloc(efe_ctx* c) {
     const int64_t L = (int64_t)c->E * c->NN;
     EFE_HIP(hipMalloc(&c->mfC, EQ * NC * sizeof(float)));
     EFE_HIP(hipMalloc(&c->mfF, (size_t)L * 3 * sizeof(float)));
-    EFE_HIP(hipMalloc(&c->mfPos, (size_t)L * sizeof(int32_t)));
-    mf_pos_kernel<<<(unsigned)((L + 255) / 256), 256, 0, c->stream>>>(L, c->E, c->NN, c->nlist, c->mfPos);
-    EFE_HIP(hipGetLastError());
     if (!c->Minv32) EFE_HIP(hipMalloc(&c->Minv32, (size_t)c->N * 9 * sizeof(float)));
     // FP32 copy of the natural shape-function derivatives at the evaluation points
     const size_t nd = (size_t)c->Q * c->NN * 3;
@@ -402,7 +394,7 @@ int efe_mf_alloc(efe_ctx* c) {
 // Diagonal blocks, Minv, P, F and (optionally) the right-hand-side correction K[:, dir] dir_values.
 int efe_launch_mf_assembly(efe_ctx* c, double* P, double* F, const double* dir_values, double* dir_correction) {
     const int grid = (c->N + 255) / 256;
-    mf_diag_kernel<<<grid, 256, 0, c->stream>>>(c->N, c->bptr, c->bcol, c->coff, c->clist, c->nptr, c->nlist,
+    mf_diag_kernel<<<grid, 256, 0, c->stream>>>(c->N, c->bptr, c->bcol, c->coff, c->nptr, c->nlist,
                                                 (const float*)c->Kscr, c->Pscr, c->dirflag, c->ndir, c->Minv32, P, F);
     EFE_HIP(hipGetLastError());
     if (dir_values) {
@@ -450,7 +442,5 @@ void efe_mf_free(efe_ctx* c) {
     hipFree(c->mfC);
     hipFree(c->mfF);
     hipFree(c->dnxi32);
-    hipFree(c->mfPos);
     c->mfC = c->mfF = c->dnxi32 = nullptr;
-    c->mfPos = nullptr;
 }

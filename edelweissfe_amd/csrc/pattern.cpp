@@ -32,10 +32,15 @@ bool efe_build_pattern(HostPattern& hp, int N, int E, int NN, const int32_t* con
     for (int64_t i = 0; i < (int64_t)E * NN; ++i) hp.nptr[conn[i] + 1]++;
     for (int n = 0; n < N; ++n) hp.nptr[n + 1] += hp.nptr[n];
     hp.nlist.resize((size_t)E * NN);
+    hp.epos.resize((size_t)E * NN);
     {
         std::vector<int32_t> fill(hp.nptr.begin(), hp.nptr.end() - 1);
         for (int e = 0; e < E; ++e)
-            for (int a = 0; a < NN; ++a) hp.nlist[fill[conn[(int64_t)e * NN + a]]++] = e * NN + a;
+            for (int a = 0; a < NN; ++a) {
+                const int32_t f = fill[conn[(int64_t)e * NN + a]]++;
+                hp.nlist[f] = e * NN + a;
+                hp.epos[(int64_t)a * E + e] = f;  // [a][e]: where (e, a) sits in the node-sorted list
+            }
     }
     // node adjacency
     hp.bptr.assign(N + 1, 0);
@@ -83,10 +88,15 @@ bool efe_build_pattern(HostPattern& hp, int N, int E, int NN, const int32_t* con
         }
     }
     for (int64_t s = 0; s < nb; ++s) hp.coff[s + 1] += hp.coff[s];
-    hp.clist.resize((size_t)E * NN * NN);
+    // Position of element block k = (e, a, b) in the contribution order (block slot, then
+    // element).  The element pass writes each block there, so the contributions of one block
+    // slot, and of one row, are one contiguous range that the assembly reads in order, with
+    // no index lookups: measured on MI355X, the list gather through an index per contribution
+    // was latency bound (1.0 ms for the 64^3 FP64 assembly against 0.7 ms).
+    hp.cpos.resize((size_t)E * NN * NN);
     {
         std::vector<int32_t> fill(hp.coff.begin(), hp.coff.end() - 1);
-        for (size_t k = 0; k < slot.size(); ++k) hp.clist[fill[slot[k]]++] = (int32_t)k;
+        for (size_t k = 0; k < slot.size(); ++k) hp.cpos[k] = fill[slot[k]]++;
     }
     // row slots: inside each window of kSortWindow nodes, rows by decreasing length (stable)
     hp.slot_node.resize(N);
