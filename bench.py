@@ -121,10 +121,14 @@ def spmv_roofline(system, n_launch=50):
 
 
 def profile_spmv(system, n_launch=20, copy_mb=512):
-    """Counter-collection mode: one FP64 assembly of the 64^3 operator, then n_launch SpMV
-    launches and n_launch plain device copies of copy_mb MiB (the calibration kernel whose
-    FETCH_SIZE / WRITE_SIZE per launch are known: copy_mb MiB each)."""
+    """Counter-collection mode: one FP64 element pass and assembly of the operator, then
+    n_launch FP64 SpMV launches, n_launch plain device copies of copy_mb MiB (the calibration
+    kernel whose FETCH_SIZE / WRITE_SIZE per launch are known: copy_mb MiB each), then one
+    matrix-free FP32 element pass and a block-Jacobi PCG of n_launch iterations on the
+    matrix-free operator (the mixed form's inner solve)."""
     import torch
+
+    from edelweissfe_amd import _lib
 
     dU = torch.zeros(system.n_dof, dtype=torch.float64, device="cuda")
     P, F = system.zeros(), system.zeros()
@@ -137,11 +141,13 @@ def profile_spmv(system, n_launch=20, copy_mb=512):
         system.spmv(x, y)
     for _ in range(n_launch):
         b.copy_(a)
+    system.compute_elements(dU, P, F, precision=_lib.EFE_FP32_MATFREE)
+    system.linsolve(x, y, precision=_lib.EFE_FP32_MATFREE, rtol=1e-30, max_iter=n_launch, check_every=n_launch)
     torch.cuda.synchronize()
     nb = system.nnz // 9
     alg = 8 * system.nnz + 4 * nb + 16 * system.n_dof
-    log("[profile] %d spmv launches (algorithmic %d bytes each), %d copies of %d MiB" % (n_launch, alg, n_launch,
-                                                                                      copy_mb))
+    log("[profile] %d spmv launches (algorithmic %d bytes each), %d copies of %d MiB, %d matrix-free PCG "
+        "iterations" % (n_launch, alg, n_launch, copy_mb, n_launch))
 
 
 def load_traffic(name):

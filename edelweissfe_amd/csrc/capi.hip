@@ -225,6 +225,8 @@ int efe_ctx_create(const efe_mesh_desc* d, efe_ctx** out) {
     if ((rc = upload(&c->bcol, hp.bcol.data(), hp.bcol.size(), st))) return fail(rc);
     if ((rc = upload(&c->coff, hp.coff.data(), hp.coff.size(), st))) return fail(rc);
     if ((rc = upload(&c->cpos, hp.cpos.data(), hp.cpos.size(), st))) return fail(rc);
+    if ((rc = upload(&c->cslice, hp.cslice_off.data(), hp.cslice_off.size(), st))) return fail(rc);
+    c->scratchBlocks = hp.scratch_blocks;
     if ((rc = upload(&c->elemPos, hp.epos.data(), hp.epos.size(), st))) return fail(rc);
     if ((rc = upload(&c->nptr, hp.nptr.data(), hp.nptr.size(), st))) return fail(rc);
     if ((rc = upload(&c->nlist, hp.nlist.data(), hp.nlist.size(), st))) return fail(rc);
@@ -267,7 +269,7 @@ void efe_ctx_destroy(efe_ctx* c) {
     efe_mf_free(c);
     void* bufs[] = {c->shapeNe, c->geoElems,
                     c->coords, c->conn,  c->props,  c->dNdxi,  c->shapeN, c->qpw,    c->stateRef, c->stateTmp,
-                    c->Kscr,   c->Pscr,  c->bptr,   c->bcol,   c->coff,   c->cpos,   c->nptr,     c->nlist,
+                    c->Kscr,   c->Pscr,  c->bptr,   c->bcol,   c->coff,   c->cpos,   c->cslice, c->nptr,     c->nlist,
                     c->diagk,  c->slice_off, c->scol, c->K64,  c->K32,    c->Minv64, c->Minv32,   c->dirflag,
                     c->slotNode, c->nodeSlot, c->specStatus, c->elemPos,
                     c->status, c->red,   c->specOfNode, c->specNodeOff, c->specOfElem};
@@ -332,7 +334,7 @@ int efe_compute_elements(efe_ctx* c, const double* U, const double* dU, const do
         efe_set_error("efe_compute_elements: invalid arguments");
         return EFE_ERR_ARG;
     }
-    const size_t nblk = (size_t)c->E * c->NN * c->NN * kBlockPitch;
+    const size_t nblk = (size_t)c->scratchBlocks * kBlockPitch;
     if (!c->Kscr) EFE_HIP(hipMalloc(&c->Kscr, nblk * sizeof(double)));
     const size_t nval = (size_t)c->hp.padded_blocks * 9;
     if (precision == EFE_FP64 && !c->K64) {

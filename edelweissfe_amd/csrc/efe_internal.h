@@ -49,6 +49,8 @@ struct HostPattern {
     std::vector<int32_t> bptr, bcol;      // node adjacency (block CSR), columns ascending
     std::vector<int32_t> diagk;           // position of the diagonal block in each row
     std::vector<int32_t> coff, cpos;      // per block slot: its contribution range; per element block: its position
+    std::vector<int32_t> cslice_off;      // [nslices+1] contribution rows of the element scratch per slice
+    int64_t scratch_blocks = 0;           // element scratch size in 3x3 blocks (slices padded)
     std::vector<int32_t> epos;            // [a][e]: index of (e, a) in nlist
     std::vector<int32_t> nptr, nlist;     // per node: (e*NN + a) incidences, element order
     std::vector<int32_t> slice_off;       // [nslices+1] first padded block of each slice
@@ -115,7 +117,8 @@ struct efe_ctx {
     int32_t* specOfElem = nullptr;   // [E]
 
     // device pattern
-    int32_t *bptr = nullptr, *bcol = nullptr, *coff = nullptr, *cpos = nullptr;
+    int32_t *bptr = nullptr, *bcol = nullptr, *coff = nullptr, *cpos = nullptr, *cslice = nullptr;
+    int64_t scratchBlocks = 0;
     int32_t *nptr = nullptr, *nlist = nullptr, *diagk = nullptr;
     int32_t *slice_off = nullptr, *scol = nullptr;
     int32_t *slotNode = nullptr, *nodeSlot = nullptr;  // operator row slot <-> node

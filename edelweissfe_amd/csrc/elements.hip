@@ -243,10 +243,10 @@ __global__ __launch_bounds__(256) void assemble_kernel(
     const int32_t* __restrict__ slice_off, const KT* __restrict__ Kscr, const double* __restrict__ Pscr,
     const uint8_t* __restrict__ dirflag, int ndir, const double* __restrict__ dirv, double* __restrict__ dircorr,
     KT* __restrict__ Kval, KT* __restrict__ Minv, double* __restrict__ P, double* __restrict__ F,
-    const int32_t* __restrict__ slotNode) {
-    // Threads walk the row slots of the sliced layout (so the value stores of a wavefront
-    // are contiguous), and the workgroups of one XCD own one contiguous slot range (so the
-    // element blocks that neighbouring rows share are read through the same L2).
+    const int32_t* __restrict__ slotNode, const int32_t* __restrict__ cslice) {
+    // Threads walk the row slots of the sliced layout: the value stores of a wavefront are
+    // contiguous, and so are its reads of the element scratch, where contribution j of the
+    // 64 rows of a slice is one run of 64 blocks.
     const int per = gridDim.x >> 3;
     const int blk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     const int ps = blk * blockDim.x + threadIdx.x;
@@ -263,6 +263,8 @@ __global__ __launch_bounds__(256) void assemble_kernel(
     double t0 = 0, t1 = 0, t2 = 0;
     double dg[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     const int b0 = bptr[n], b1 = bptr[n + 1];
+    // block position of this row's contribution c: cbase + (c - coff[b0]) * kSlice
+    const int64_t cbase = (int64_t)cslice[s] * kSlice + l - (int64_t)coff[b0] * kSlice;
     for (int b = b0; b < b1; ++b) {
         const int m = bcol[b];
         KT acc[9];
@@ -271,7 +273,7 @@ __global__ __launch_bounds__(256) void assemble_kernel(
         // the slot's contributions are contiguous and in element order; unrolled so that
         // the loads of several contributions are in flight together
 #pragma unroll 4
-        for (int c = coff[b]; c < coff[b + 1]; ++c) add_block(Kscr + (int64_t)c * kBlockPitch, acc);
+        for (int c = coff[b]; c < coff[b + 1]; ++c) add_block(Kscr + (cbase + (int64_t)c * kSlice) * kBlockPitch, acc);
         if (dirv) {
             const double d0 = dirv[3 * m], d1 = dirv[3 * m + 1], d2 = dirv[3 * m + 2];
             t0 += (double)acc[0] * d0 + (double)acc[1] * d1 + (double)acc[2] * d2;
@@ -469,11 +471,11 @@ int efe_launch_assembly(efe_ctx* c, double* P, double* F, const double* dir_valu
     if (precision == EFE_FP32) {
         assemble_kernel<float><<<grid, bs, 0, c->stream>>>(
             c->N, c->bptr, c->bcol, c->coff, c->nptr, c->nlist, c->slice_off, (const float*)c->Kscr,
-            c->Pscr, c->dirflag, c->ndir, dir_values, dir_correction, c->K32, c->Minv32, P, F, c->slotNode);
+            c->Pscr, c->dirflag, c->ndir, dir_values, dir_correction, c->K32, c->Minv32, P, F, c->slotNode, c->cslice);
     } else {
         assemble_kernel<double><<<grid, bs, 0, c->stream>>>(
             c->N, c->bptr, c->bcol, c->coff, c->nptr, c->nlist, c->slice_off, (const double*)c->Kscr,
-            c->Pscr, c->dirflag, c->ndir, dir_values, dir_correction, c->K64, c->Minv64, P, F, c->slotNode);
+            c->Pscr, c->dirflag, c->ndir, dir_values, dir_correction, c->K64, c->Minv64, P, F, c->slotNode, c->cslice);
     }
     EFE_HIP(hipGetLastError());
     c->Kprec = precision;

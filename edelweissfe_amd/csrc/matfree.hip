@@ -289,14 +289,18 @@ __global__ __launch_bounds__(256) void mf_diag_kernel(
     int N, const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol, const int32_t* __restrict__ coff,
     const int32_t* __restrict__ nptr, const int32_t* __restrict__ nlist,
     const float* __restrict__ Kscr, const double* __restrict__ Pscr, const uint8_t* __restrict__ dirflag, int ndir,
-    float* __restrict__ Minv, double* __restrict__ P, double* __restrict__ F) {
+    float* __restrict__ Minv, double* __restrict__ P, double* __restrict__ F, const int32_t* __restrict__ nodeSlot,
+    const int32_t* __restrict__ cslice) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
+    // element scratch position of this row's contribution c (the assembly's sliced layout)
+    const int ps = nodeSlot[n], sl = ps / kSlice, l = ps - (ps / kSlice) * kSlice;
+    const int64_t cbase = (int64_t)cslice[sl] * kSlice + l - (int64_t)coff[bptr[n]] * kSlice;
     double dg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int b = bptr[n]; b < bptr[n + 1]; ++b) {
         if (bcol[b] != n) continue;
         float acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int c = coff[b]; c < coff[b + 1]; ++c) add_block(Kscr + (int64_t)c * kBlockPitch, acc);
+        for (int c = coff[b]; c < coff[b + 1]; ++c) add_block(Kscr + (cbase + (int64_t)c * kSlice) * kBlockPitch, acc);
 #pragma unroll
         for (int v = 0; v < 9; ++v) dg[v] = acc[v];
     }
@@ -395,7 +399,8 @@ int efe_mf_alloc(efe_ctx* c) {
 int efe_launch_mf_assembly(efe_ctx* c, double* P, double* F, const double* dir_values, double* dir_correction) {
     const int grid = (c->N + 255) / 256;
     mf_diag_kernel<<<grid, 256, 0, c->stream>>>(c->N, c->bptr, c->bcol, c->coff, c->nptr, c->nlist,
-                                                (const float*)c->Kscr, c->Pscr, c->dirflag, c->ndir, c->Minv32, P, F);
+                                                (const float*)c->Kscr, c->Pscr, c->dirflag, c->ndir, c->Minv32, P, F,
+                                                c->nodeSlot, c->cslice);
     EFE_HIP(hipGetLastError());
     if (dir_values) {
         launch_apply<double>(c, c->stream, dir_values, 0, nullptr);

@@ -88,16 +88,6 @@ bool efe_build_pattern(HostPattern& hp, int N, int E, int NN, const int32_t* con
         }
     }
     for (int64_t s = 0; s < nb; ++s) hp.coff[s + 1] += hp.coff[s];
-    // Position of element block k = (e, a, b) in the contribution order (block slot, then
-    // element).  The element pass writes each block there, so the contributions of one block
-    // slot, and of one row, are one contiguous range that the assembly reads in order, with
-    // no index lookups: measured on MI355X, the list gather through an index per contribution
-    // was latency bound (1.0 ms for the 64^3 FP64 assembly against 0.7 ms).
-    hp.cpos.resize((size_t)E * NN * NN);
-    {
-        std::vector<int32_t> fill(hp.coff.begin(), hp.coff.end() - 1);
-        for (size_t k = 0; k < slot.size(); ++k) hp.cpos[k] = fill[slot[k]]++;
-    }
     // row slots: inside each window of kSortWindow nodes, rows by decreasing length (stable)
     hp.slot_node.resize(N);
     hp.node_slot.resize(N);
@@ -109,6 +99,42 @@ bool efe_build_pattern(HostPattern& hp, int N, int E, int NN, const int32_t* con
         });
     }
     for (int p = 0; p < N; ++p) hp.node_slot[hp.slot_node[p]] = p;
+    // Element scratch: contribution j of the row in slot p = 64 s + l (its contributions in
+    // block order, element order inside a block: the order of updateCSR) sits at block
+    // position 64 * cslice_off[s] + 64 j + l.  The element pass writes each block there; the
+    // assembly, one lane per slot, reads contribution j of its 64 rows as one contiguous run
+    // (no index lookups, no per-lane streams: measured on MI355X, an index per contribution
+    // was latency bound and per-lane contiguous runs thrashed L2).
+    {
+        const int nsl = (N + kSlice - 1) / kSlice;
+        hp.cslice_off.assign(nsl + 1, 0);
+        for (int s = 0; s < nsl; ++s) {
+            int mx = 0;
+            for (int l = 0; l < kSlice; ++l) {
+                const int p = s * kSlice + l;
+                if (p < N) {
+                    const int n = hp.slot_node[p];
+                    mx = std::max(mx, hp.coff[hp.bptr[n + 1]] - hp.coff[hp.bptr[n]]);
+                }
+            }
+            if ((int64_t)hp.cslice_off[s] + mx > INT32_MAX / kSlice) {
+                err = "element scratch exceeds the int32 block index range";
+                return false;
+            }
+            hp.cslice_off[s + 1] = hp.cslice_off[s] + mx;
+        }
+        hp.scratch_blocks = (int64_t)hp.cslice_off[nsl] * kSlice;
+        std::vector<int32_t> fill(hp.coff.begin(), hp.coff.end() - 1);
+        hp.cpos.resize((size_t)E * NN * NN);
+        for (size_t k = 0; k < slot.size(); ++k) {
+            const int64_t b = slot[k];
+            const int32_t c = fill[b]++;
+            const int64_t e = (int64_t)k / (NN * NN), a = ((int64_t)k / NN) % NN;
+            const int n = conn[e * NN + a];  // row node of block (e, a, b)
+            const int p = hp.node_slot[n], sl = p / kSlice, l = p % kSlice;
+            hp.cpos[k] = (int32_t)(((int64_t)hp.cslice_off[sl] + (c - hp.coff[hp.bptr[n]])) * kSlice + l);
+        }
+    }
     // sliced-ELL layout of the block rows, in slot order
     int ns = (N + kSlice - 1) / kSlice;
     hp.slice_off.assign(ns + 1, 0);
