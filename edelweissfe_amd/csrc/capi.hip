@@ -267,7 +267,7 @@ void efe_ctx_destroy(efe_ctx* c) {
     efe_gmres_free(c);
     efe_batched_free(c);
     efe_mf_free(c);
-    void* bufs[] = {c->shapeNe, c->geoElems,
+    void* bufs[] = {c->shapeNe, c->geoElems, c->surfElems, c->surfMask,
                     c->coords, c->conn,  c->props,  c->dNdxi,  c->shapeN, c->qpw,    c->stateRef, c->stateTmp,
                     c->Kscr,   c->Pscr,  c->bptr,   c->bcol,   c->coff,   c->cpos,   c->cslice, c->nptr,     c->nlist,
                     c->diagk,  c->slice_off, c->scol, c->K64,  c->K32,    c->Minv64, c->Minv32,   c->dirflag,
@@ -372,6 +372,50 @@ int efe_compute_elements(efe_ctx* c, const double* U, const double* dU, const do
 int efe_body_force(efe_ctx* c, const double* force3, double* PExt) {
     if (int rc = check(c)) return rc;
     return efe_launch_body_force(c, force3, PExt);
+}
+
+int efe_surface_traction(efe_ctx* c, const int32_t* elements, const int32_t* faces, int32_t n,
+                         const double* traction3, double* PExt) {
+    if (int rc = check(c)) return rc;
+    if (n < 0 || (n > 0 && (!elements || !faces)) || !traction3 || !PExt) {
+        efe_set_error("efe_surface_traction: invalid arguments");
+        return EFE_ERR_ARG;
+    }
+    // faces of each loaded element as a bit mask, elements in ascending order
+    std::vector<uint8_t> mask((size_t)c->E, 0);
+    for (int32_t i = 0; i < n; ++i) {
+        if (elements[i] < 0 || elements[i] >= c->E || faces[i] < 1 || faces[i] > 6) {
+            efe_set_error("efe_surface_traction: element out of range or face not in 1..6");
+            return EFE_ERR_ARG;
+        }
+        mask[elements[i]] |= (uint8_t)(1u << (faces[i] - 1));
+    }
+    std::vector<int32_t> el;
+    std::vector<uint8_t> fm;
+    for (int e = 0; e < c->E; ++e)
+        if (mask[e]) {
+            el.push_back(e);
+            fm.push_back(mask[e]);
+        }
+    const size_t m = el.size();
+    if (m > c->surfCap) {
+        hipFree(c->surfElems);
+        hipFree(c->surfMask);
+        c->surfElems = nullptr;
+        c->surfMask = nullptr;
+        c->surfCap = 0;
+        EFE_HIP(hipMalloc(&c->surfElems, m * sizeof(int32_t)));
+        EFE_HIP(hipMalloc(&c->surfMask, m));
+        c->surfCap = m;
+    }
+    if (m) {
+        EFE_HIP(hipMemcpyAsync(c->surfElems, el.data(), m * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+        EFE_HIP(hipMemcpyAsync(c->surfMask, fm.data(), m, hipMemcpyHostToDevice, c->stream));
+    }
+    if (int rc = efe_launch_surface_traction(c, c->surfElems, c->surfMask, (int)m, traction3, PExt)) return rc;
+    // the host vectors above must outlive the copies
+    EFE_HIP(hipStreamSynchronize(c->stream));
+    return EFE_OK;
 }
 
 int efe_set_geostatic(efe_ctx* c, const double def[6], const int32_t* elements, int32_t n) {

@@ -106,6 +106,9 @@ struct efe_ctx {
                                  // element scratch starts at block elemPos * NN)
     bool mfPass = false;             // the current element launch is a matrix-free pass
     int32_t* geoElems = nullptr; // [E] element subset of a geostatic initial state
+    int32_t* surfElems = nullptr;  // loaded elements of the last surface traction call
+    uint8_t* surfMask = nullptr;   // their loaded faces (bit f-1)
+    size_t surfCap = 0;
     double* qpw = nullptr;       // [Q]
     double* stateRef = nullptr;  // [nstate][E][Q]
     double* stateTmp = nullptr;
@@ -156,6 +159,8 @@ int efe_launch_elements(efe_ctx* c, const double* dU, int precision);
 int efe_launch_assembly(efe_ctx* c, double* P, double* F, const double* dir_values, double* dir_correction,
                         int precision);
 int efe_launch_body_force(efe_ctx* c, const double* f3, double* PExt);
+int efe_launch_surface_traction(efe_ctx* c, const int32_t* elems, const uint8_t* fmask, int n, const double* t3,
+                                double* PExt);
 int efe_launch_geostatic(efe_ctx* c, const double* def, const int32_t* elements, int32_t n);
 // matrix-free FP32 operator (matfree.hip)
 int efe_mf_alloc(efe_ctx* c);

@@ -366,6 +366,112 @@ __global__ __launch_bounds__(64) void body_force_kernel(int E, int N, const doub
     }
 }
 
+// Surface traction (the distributed load of nonlinearimplicitstatic.py:460-508 with type
+// "surface traction"): the consistent nodal force f_a = t * integral N_a dA of a uniform
+// traction t on element faces.  Faces are numbered by local node numbers as in Abaqus
+// (1: 1-2-3-4, 2: 5-8-7-6, 3: 1-5-6-2, 4: 2-6-7-3, 5: 3-7-8-4, 6: 4-8-5-1; hexa20 adds the
+// mid-side nodes between consecutive corners), the numbering of the box generator's
+// surfaces (boxgen.py:438-446).  2x2 Gauss points on 4-node faces, 3x3 on 8-node
+// (serendipity) faces.  One thread per loaded element: its faces (bit f-1 of fmask) are
+// summed in face order into the element's force slots.
+__constant__ int8_t kFace8[6][4] = {{0, 1, 2, 3}, {4, 7, 6, 5}, {0, 4, 5, 1},
+                                    {1, 5, 6, 2}, {2, 6, 7, 3}, {3, 7, 4, 0}};
+__constant__ int8_t kFace20[6][8] = {{0, 1, 2, 3, 8, 9, 10, 11},   {4, 7, 6, 5, 15, 14, 13, 12},
+                                     {0, 4, 5, 1, 16, 12, 17, 8},  {1, 5, 6, 2, 17, 13, 18, 9},
+                                     {2, 6, 7, 3, 18, 14, 19, 10}, {3, 7, 4, 0, 19, 15, 16, 11}};
+
+// N, dN/ds, dN/dt of the NF-node face (4: bilinear; 8: serendipity) at (s, t); node order:
+// corners (-1,-1), (1,-1), (1,1), (-1,1), then mid-sides (0,-1), (1,0), (0,1), (-1,0)
+template <int NF>
+__device__ __forceinline__ void face_shape(double s, double t, double* N, double* Ns, double* Nt) {
+    const double cs[4] = {-1, 1, 1, -1}, ct[4] = {-1, -1, 1, 1};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const double a = 1 + s * cs[i], b = 1 + t * ct[i];
+        if (NF == 4) {
+            N[i] = a * b / 4;
+            Ns[i] = cs[i] * b / 4;
+            Nt[i] = ct[i] * a / 4;
+        } else {
+            N[i] = a * b * (s * cs[i] + t * ct[i] - 1) / 4;
+            Ns[i] = cs[i] * b * (2 * s * cs[i] + t * ct[i]) / 4;
+            Nt[i] = ct[i] * a * (s * cs[i] + 2 * t * ct[i]) / 4;
+        }
+    }
+    if (NF == 8) {
+        N[4] = (1 - s * s) * (1 - t) / 2;
+        Ns[4] = -s * (1 - t);
+        Nt[4] = -(1 - s * s) / 2;
+        N[5] = (1 + s) * (1 - t * t) / 2;
+        Ns[5] = (1 - t * t) / 2;
+        Nt[5] = -t * (1 + s);
+        N[6] = (1 - s * s) * (1 + t) / 2;
+        Ns[6] = -s * (1 + t);
+        Nt[6] = (1 - s * s) / 2;
+        N[7] = (1 - s) * (1 - t * t) / 2;
+        Ns[7] = -(1 - t * t) / 2;
+        Nt[7] = -t * (1 - s);
+    }
+}
+
+template <int NN>
+__global__ __launch_bounds__(256) void surface_traction_kernel(int n, int E, int N, const int32_t* __restrict__ elems,
+                                                               const uint8_t* __restrict__ fmask,
+                                                               const double* __restrict__ coords,
+                                                               const int32_t* __restrict__ conn, double tx, double ty,
+                                                               double tz, double* __restrict__ Pscr) {
+    constexpr int NF = NN == 8 ? 4 : 8, NG = NN == 8 ? 2 : 3;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int e = elems[k], mask = fmask[k];
+    const double gp2[2] = {-0.57735026918962576451, 0.57735026918962576451}, gw2[2] = {1.0, 1.0};
+    const double gp3[3] = {-0.77459666924148337704, 0.0, 0.77459666924148337704};
+    const double gw3[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+    double acc[NN];
+#pragma unroll
+    for (int a = 0; a < NN; ++a) acc[a] = 0.0;
+    for (int f = 0; f < 6; ++f) {
+        if (!((mask >> f) & 1)) continue;
+        int loc[NF];
+        double X[NF][3];
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+            loc[j] = NN == 8 ? kFace8[f][j] : kFace20[f][j];
+            const int nd = conn[(int64_t)loc[j] * E + e];
+            X[j][0] = coords[nd];
+            X[j][1] = coords[N + nd];
+            X[j][2] = coords[2 * N + nd];
+        }
+        for (int i = 0; i < NG; ++i)
+            for (int j2 = 0; j2 < NG; ++j2) {
+                const double s = NG == 2 ? gp2[i] : gp3[i], t = NG == 2 ? gp2[j2] : gp3[j2];
+                const double w = NG == 2 ? gw2[i] * gw2[j2] : gw3[i] * gw3[j2];
+                double Nf[NF], Ns[NF], Nt[NF];
+                face_shape<NF>(s, t, Nf, Ns, Nt);
+                double xs[3] = {0, 0, 0}, xt[3] = {0, 0, 0};
+#pragma unroll
+                for (int j = 0; j < NF; ++j)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        xs[c] += Ns[j] * X[j][c];
+                        xt[c] += Nt[j] * X[j][c];
+                    }
+                const double cx = xs[1] * xt[2] - xs[2] * xt[1], cy = xs[2] * xt[0] - xs[0] * xt[2],
+                             cz = xs[0] * xt[1] - xs[1] * xt[0];
+                const double dA = sqrt(cx * cx + cy * cy + cz * cz) * w;
+#pragma unroll
+                for (int j = 0; j < NF; ++j) acc[loc[j]] += Nf[j] * dA;
+            }
+    }
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+        double* dst = Pscr + ((int64_t)e * NN + a) * 3;
+        dst[0] = acc[a] * tx;
+        dst[1] = acc[a] * ty;
+        dst[2] = acc[a] * tz;
+    }
+}
+
 __global__ void gather_add_kernel(int N, const int32_t* __restrict__ nptr, const int32_t* __restrict__ nlist,
                                   const double* __restrict__ Pscr, double* __restrict__ out) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -501,6 +607,23 @@ int efe_launch_body_force(efe_ctx* c, const double* f3, double* PExt) {
         default:
             efe_set_error("unknown element type");
             return EFE_ERR_ARG;
+    }
+    gather_add_kernel<<<(c->N + 255) / 256, 256, 0, c->stream>>>(c->N, c->nptr, c->nlist, c->Pscr, PExt);
+    EFE_HIP(hipGetLastError());
+    return EFE_OK;
+}
+
+int efe_launch_surface_traction(efe_ctx* c, const int32_t* elems, const uint8_t* fmask, int n, const double* t3,
+                                double* PExt) {
+    EFE_HIP(hipMemsetAsync(c->Pscr, 0, (size_t)c->E * c->NN * 3 * sizeof(double), c->stream));
+    const int grid = (n + 255) / 256;
+    if (n > 0) {
+        if (c->NN == 8)
+            surface_traction_kernel<8><<<grid, 256, 0, c->stream>>>(n, c->E, c->N, elems, fmask, c->coords, c->conn,
+                                                                    t3[0], t3[1], t3[2], c->Pscr);
+        else
+            surface_traction_kernel<20><<<grid, 256, 0, c->stream>>>(n, c->E, c->N, elems, fmask, c->coords, c->conn,
+                                                                     t3[0], t3[1], t3[2], c->Pscr);
     }
     gather_add_kernel<<<(c->N + 255) / 256, 256, 0, c->stream>>>(c->N, c->nptr, c->nlist, c->Pscr, PExt);
     EFE_HIP(hipGetLastError());

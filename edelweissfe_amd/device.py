@@ -105,6 +105,17 @@ class DeviceSystem:
         f = (ctypes.c_double * 3)(*[float(v) for v in force])
         check(self.lib.efe_body_force(self.ctx, f, self._ptr(PExt)), "efe_body_force")
 
+    def surface_traction(self, surface, traction, PExt):
+        """Uniform traction (3,) on a surface {face ID (1..6): element indices}, added into PExt."""
+        el = np.concatenate([np.asarray(v, dtype=np.int32) for v in surface.values()]) if surface else \
+            np.zeros(0, np.int32)
+        fc = np.concatenate([np.full(len(v), int(f), dtype=np.int32) for f, v in surface.items()]) if surface else \
+            np.zeros(0, np.int32)
+        el, fc = np.ascontiguousarray(el), np.ascontiguousarray(fc)
+        t = (ctypes.c_double * 3)(*[float(v) for v in traction])
+        check(self.lib.efe_surface_traction(self.ctx, el.ctypes.data, fc.ctypes.data, int(el.size), t, self._ptr(PExt)),
+              "efe_surface_traction")
+
     def set_geostatic(self, definition, elements=None):
         """Geostatic initial stress [p1, h1, p2, h2, xLateral, zLateral] on `elements` (all if None)."""
         d = (ctypes.c_double * 6)(*[float(v) for v in definition])

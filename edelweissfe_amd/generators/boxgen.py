@@ -72,4 +72,7 @@ def generateBox(nX=1, nY=1, nZ=1, lX=1.0, lY=1.0, lZ=1.0, x0=0.0, y0=0.0, z0=0.0
         name + "_right": grid[-1, :, :].ravel(), name + "_left": grid[0, :, :].ravel(),
         name + "_front": grid[:, :, -1].ravel(), name + "_back": grid[:, :, 0].ravel(),
     }
-    return FEModel(coords, elType, conn, nodeSets, elementSets, material=material)
+    # surfaces as boxgen.py:438-446: face IDs by Abaqus local node numbering
+    surfaces = {name + "_" + side: {fid: elementSets[name + "_" + side]}
+                for side, fid in (("bottom", 1), ("top", 2), ("left", 3), ("front", 4), ("right", 5), ("back", 6))}
+    return FEModel(coords, elType, conn, nodeSets, elementSets, material=material, surfaces=surfaces)

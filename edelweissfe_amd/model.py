@@ -13,12 +13,14 @@ import numpy as np
 
 
 class FEModel:
-    def __init__(self, coordinates, elementType, connectivity, nodeSets, elementSets, material=None):
+    def __init__(self, coordinates, elementType, connectivity, nodeSets, elementSets, material=None, surfaces=None):
         self.coordinates = np.ascontiguousarray(coordinates, dtype=np.float64)
         self.elementType = elementType.upper()
         self.connectivity = np.ascontiguousarray(connectivity, dtype=np.int64)
         self.nodeSets = dict(nodeSets)
         self.elementSets = dict(elementSets)
+        # element-based surfaces {name: {face ID (1..6): element indices}} (femodel.py surfaces)
+        self.surfaces = dict(surfaces or {})
         self.material = material
         self.time = 0.0
         self.nDof = 3 * self.coordinates.shape[0]

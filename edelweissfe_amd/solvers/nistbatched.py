@@ -119,8 +119,8 @@ class NISTBatched:
         owners = []
         hasDir = np.zeros(S, dtype=bool)
         for s, (m, st) in enumerate(zip(models, steps)):
-            if st.actions.get("bodyforce"):
-                raise NotImplementedError("body forces in a batch")
+            if st.actions.get("bodyforce") or st.actions.get("distributedload"):
+                raise NotImplementedError("body forces and distributed loads in a batch")
             for d in st.actions["dirichlet"].values():
                 owners.append((s, d, d.indices(m) + self.dofOffset[s]))
                 hasDir[s] = True

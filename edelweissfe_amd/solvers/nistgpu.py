@@ -191,7 +191,7 @@ class NISTGPU:
         model.dU = dU.cpu().numpy()
         model._qpState = None
         model.deviceSystem = sysm
-        for group in ("dirichlet", "nodeforces", "bodyforce", "geostatic"):
+        for group in ("dirichlet", "nodeforces", "distributedload", "bodyforce", "geostatic"):
             for action in step.actions.get(group, {}).values():
                 action.applyAtStepEnd(model)
         return True, model
@@ -310,7 +310,8 @@ class NISTGPU:
         return info
 
     def _externalLoads(self, step, model, timeStep):
-        """assembleLoads (nonlinearimplicitstatic.py:903-945): node forces and body forces."""
+        """assembleLoads (nonlinearimplicitstatic.py:903-945): node forces, distributed loads
+        (surface tractions, computeDistributedLoads :460-508) and body forces."""
         sysm = self.system
         PExt = torch.zeros(sysm.n_dof, dtype=torch.float64, device=sysm.device)
         for nf in step.actions.get("nodeforces", {}).values():
@@ -319,6 +320,8 @@ class NISTGPU:
             load = torch.as_tensor(np.tile(nf.getCurrentLoad(timeStep), (len(nodes), 1)).ravel(),
                                    dtype=torch.float64, device=sysm.device)
             PExt[idx] += load
+        for dl in step.actions.get("distributedload", {}).values():
+            sysm.surface_traction(model.surfaces[dl.surface], dl.getCurrentLoad(timeStep), PExt)
         for bf in step.actions.get("bodyforce", {}).values():
             if len(model.elementSets[bf.elSet]) != model.nElements:
                 raise NotImplementedError("body forces on element subsets")

@@ -1,6 +1,7 @@
 """AdaptiveStep with the option names and defaults of
 edelweissfe/steps/adaptivestep.py:56-113 and its step-action dictionary
-(actions["dirichlet"], ["nodeforces"], ["bodyforce"], ["geostatic"], ["options"]["NISTSolver"])."""
+(actions["dirichlet"], ["nodeforces"], ["distributedload"], ["bodyforce"], ["geostatic"],
+["options"]["NISTSolver"])."""
 
 from .timestepper import AdaptiveTimeStepper
 
@@ -27,7 +28,8 @@ class AdaptiveStep:
         self.criticalIter = kwargs.get("criticalIter", self.defaultCriticalIter)
         self.maxGrowIter = kwargs.get("maxGrowIter", self.defaultMaxGrowingIter)
         self.cutbackFactor = kwargs.get("cutbackFactor", self.defaultCutbackFactor)
-        self.actions = {"dirichlet": {}, "nodeforces": {}, "bodyforce": {}, "geostatic": {}, "options": {}}
+        self.actions = {"dirichlet": {}, "nodeforces": {}, "distributedload": {}, "bodyforce": {}, "geostatic": {},
+                        "options": {}}
         for k, v in (actions or {}).items():
             self.actions[k] = v
         self.incrementGenerator = AdaptiveTimeStepper(model.time, self.length, self.startIncrementSize,

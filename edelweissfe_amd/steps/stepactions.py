@@ -7,6 +7,9 @@
               start + delta * amp(p).
 * BodyForce   edelweissfe/stepactions/bodyforce.py: force per volume on an element
               set, start + delta * amp(p).
+* DistributedLoad edelweissfe/stepactions/distributedload.py:52-110: a load of a given
+              type ("surface traction": force per area, a vector) on a surface
+              {face ID: element set}; the current load is start + delta * amp(p).
 * Geostatic   edelweissfe/stepactions/geostatic.py:51-101: geostatic initial stress
               [p1, h1, p2, h2, xLateral, zLateral] on an element set, set into the
               reference state at every iteration start; displacements are reset at the
@@ -73,6 +76,30 @@ class BodyForce:
         self.name = name
         self.elSet = elSet
         self.delta = np.asarray(vector, dtype=float)
+        self.start = np.zeros(3)
+        self.amplitude = amplitude or _identity
+        self.idle = False
+
+    def getCurrentLoad(self, timeStep):
+        t = 1.0 if self.idle else timeStep.stepProgress
+        return self.start + self.delta * self.amplitude(t)
+
+    def applyAtStepEnd(self, model):
+        if not self.idle:
+            self.start = self.start + self.delta * self.amplitude(1.0)
+            self.delta = np.zeros(3)
+            self.idle = True
+
+
+class DistributedLoad:
+    def __init__(self, name, surface, magnitude, loadType="surface traction", amplitude=None):
+        """surface: the model's surface name; magnitude: the load vector over the step."""
+        if loadType.replace(" ", "").lower() != "surfacetraction":
+            raise NotImplementedError("distributed load type {:}".format(loadType))
+        self.name = name
+        self.surface = surface
+        self.loadType = loadType
+        self.delta = np.asarray(magnitude, dtype=float)
         self.start = np.zeros(3)
         self.amplitude = amplitude or _identity
         self.idle = False

@@ -137,6 +137,16 @@ int efe_compute_elements(efe_ctx* ctx, const double* U, const double* dU, const 
 /* External body-force vector (element.py:348-371), added into PExt (device [n_dof]). */
 int efe_body_force(efe_ctx* ctx, const double* force3, double* PExt);
 
+/* Uniform surface traction traction3 (host [3], force per area) on the faces faces[i]
+ * (1..6, Abaqus numbering by local node number, as the box generator's surfaces,
+ * boxgen.py:438-446) of the elements elements[i] (host [n]): the consistent nodal forces
+ * integral N_a dA * traction3 are added into PExt (device [n_dof]).  This is the element side of
+ * NIST.computeDistributedLoads (nonlinearimplicitstatic.py:460-508) for load type "surface
+ * traction" (stepactions/distributedload.py); the reference's own DisplacementElement has
+ * no computeDistributedLoad (element.py:288). */
+int efe_surface_traction(efe_ctx* ctx, const int32_t* elements, const int32_t* faces, int32_t n,
+                         const double* traction3, double* PExt);
+
 /* Geostatic initial stress (stepactions/geostatic.py:94-101 -> the element's
  * setInitialCondition("geostatic stress", [p1, h1, p2, h2, xLateral, zLateral])):
  * at every QP of the listed elements (host [n], or NULL for all elements) the reference

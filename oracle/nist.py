@@ -23,6 +23,7 @@ import scipy.sparse.linalg as spla
 
 from . import assembly
 from .element import HexElements
+from .surface import surface_traction
 from .exceptions import (CutbackRequest, DivergingSolution, ReachedMaxIncrements, ReachedMaxIterations,
                          ReachedMinIncrementSize, StepFailed)
 
@@ -117,6 +118,9 @@ class Step:
     dirichlets: list = field(default_factory=list)   # dict(nset=str, values={comp: value})
     nodeforces: list = field(default_factory=list)   # dict(nset=str, vector=(3,))
     bodyforces: list = field(default_factory=list)   # dict(elset=str, vector=(3,))
+    # dict(surface={face ID: elset name}, vector=(3,)): uniform surface traction,
+    # stepactions/distributedload.py:52-110 with type "surface traction"
+    distributedloads: list = field(default_factory=list)
     # dict(p1, h1, p2, h2, xLateral, zLateral, elset="all"): geostatic initial stress,
     # stepactions/geostatic.py:41-101
     geostatic: list = field(default_factory=list)
@@ -203,6 +207,10 @@ class NIST:
             idx = (model.dim * np.asarray(nodes)[:, None] + np.arange(model.dim)[None, :]).ravel()
             vec = np.asarray(nf.get("start", 0.0), dtype=float) + np.asarray(nf["vector"], dtype=float) * ts.stepProgress
             PExt[idx] += np.tile(vec, (len(nodes), 1)).ravel()
+        for dl in step.distributedloads:
+            vec = np.asarray(dl.get("start", 0.0), dtype=float) + np.asarray(dl["vector"], dtype=float) * ts.stepProgress
+            surface = {f: model.elsets[s] for f, s in dl["surface"].items()}
+            PExt += surface_traction(model.coords, model.elements.conn, surface, vec, model.nDof)
         for bf in step.bodyforces:
             els = model.elsets[bf["elset"]]
             force = np.asarray(bf.get("start", 0.0), dtype=float) + np.asarray(bf["vector"], dtype=float) * ts.stepProgress
