@@ -1,4 +1,5 @@
 // C ABI entry points (include/efe_gpu.h).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -267,7 +268,7 @@ void efe_ctx_destroy(efe_ctx* c) {
     efe_gmres_free(c);
     efe_batched_free(c);
     efe_mf_free(c);
-    void* bufs[] = {c->shapeNe, c->geoElems, c->surfElems, c->surfMask,
+    void* bufs[] = {c->shapeNe, c->geoElems, c->surfElems, c->surfMask, c->resIdx,
                     c->coords, c->conn,  c->props,  c->dNdxi,  c->shapeN, c->qpw,    c->stateRef, c->stateTmp,
                     c->Kscr,   c->Pscr,  c->bptr,   c->bcol,   c->coff,   c->cpos,   c->cslice, c->nptr,     c->nlist,
                     c->diagk,  c->slice_off, c->scol, c->K64,  c->K32,    c->Minv64, c->Minv32,   c->dirflag,
@@ -523,6 +524,63 @@ int efe_get_specimen_count(efe_ctx* c) {
 int efe_accept_state(efe_ctx* c) {
     if (int rc = check(c)) return rc;
     std::swap(c->stateRef, c->stateTmp);
+    return EFE_OK;
+}
+
+namespace {
+// out[i][j][k] = state component first + k of element elems[i] (i when elems is null) at QP
+// qps[j]; state layout [component][element][QP]
+__global__ void collect_results_kernel(int64_t total, int nq, int cnt, int64_t EQ, int Q, const int32_t* __restrict__ elems,
+                                       const int32_t* __restrict__ qps, int first, const double* __restrict__ st,
+                                       double* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int k = (int)(t % cnt);
+    const int j = (int)((t / cnt) % nq);
+    const int64_t i = t / ((int64_t)cnt * nq);
+    const int64_t e = elems ? elems[i] : i;
+    out[t] = st[(int64_t)(first + k) * EQ + e * Q + qps[j]];
+}
+}  // namespace
+
+int efe_collect_results(efe_ctx* c, int32_t which, const int32_t* elements, int32_t n, const int32_t* qps, int32_t nq,
+                        int32_t first, int32_t count, double* out) {
+    if (int rc = check(c)) return rc;
+    if (!out || !qps || nq <= 0 || count <= 0 || first < 0 || first + count > c->nstate || (elements && n < 0)) {
+        efe_set_error("efe_collect_results: invalid arguments");
+        return EFE_ERR_ARG;
+    }
+    const int64_t ne = elements ? n : c->E;
+    for (int32_t j = 0; j < nq; ++j)
+        if (qps[j] < 0 || qps[j] >= c->Q) {
+            efe_set_error("efe_collect_results: quadrature point out of range");
+            return EFE_ERR_ARG;
+        }
+    for (int64_t i = 0; elements && i < ne; ++i)
+        if (elements[i] < 0 || elements[i] >= c->E) {
+            efe_set_error("efe_collect_results: element out of range");
+            return EFE_ERR_ARG;
+        }
+    if (ne == 0) return EFE_OK;
+    // index lists on the device (kept between calls, grown on demand)
+    const size_t need = (size_t)(elements ? ne : 0) + (size_t)nq;
+    if (need > c->resCap) {
+        hipFree(c->resIdx);
+        c->resIdx = nullptr;
+        c->resCap = 0;
+        EFE_HIP(hipMalloc(&c->resIdx, need * sizeof(int32_t)));
+        c->resCap = need;
+    }
+    std::vector<int32_t> idx(need);
+    if (elements) std::copy(elements, elements + ne, idx.begin());
+    std::copy(qps, qps + nq, idx.begin() + (elements ? ne : 0));
+    EFE_HIP(hipMemcpyAsync(c->resIdx, idx.data(), need * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    const int64_t total = ne * nq * count;
+    collect_results_kernel<<<(unsigned)((total + 255) / 256), 256, 0, c->stream>>>(
+        total, nq, count, (int64_t)c->E * c->Q, c->Q, elements ? c->resIdx : nullptr, c->resIdx + (elements ? ne : 0),
+        first, which ? c->stateTmp : c->stateRef, out);
+    EFE_HIP(hipGetLastError());
+    EFE_HIP(hipStreamSynchronize(c->stream));  // idx must outlive the copy
     return EFE_OK;
 }
 

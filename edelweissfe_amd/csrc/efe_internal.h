@@ -109,6 +109,8 @@ struct efe_ctx {
     int32_t* surfElems = nullptr;  // loaded elements of the last surface traction call
     uint8_t* surfMask = nullptr;   // their loaded faces (bit f-1)
     size_t surfCap = 0;
+    int32_t* resIdx = nullptr;     // element and QP lists of the last result collection
+    size_t resCap = 0;
     double* qpw = nullptr;       // [Q]
     double* stateRef = nullptr;  // [nstate][E][Q]
     double* stateTmp = nullptr;

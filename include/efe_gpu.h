@@ -206,6 +206,15 @@ int efe_accept_state(efe_ctx* ctx);
  * (element.py:222-236). */
 int efe_get_qp_state(efe_ctx* ctx, int32_t which, double* out);
 
+/* Device gather of QP results (utils/elementresultcollector.pyx:37-132, which collects
+ * element.getResultArray(result, qp) for a list of elements and quadrature points): out
+ * (device [n][nq][count]) = components first .. first+count-1 of the reference (which=0) or
+ * temporary (which=1) state, for elements[i] (host [n]; NULL: all elements in order) at the
+ * quadrature points qps[j] (host [nq]).  Results: stress = components 0..5, strain = 6..11,
+ * material state (J2 / Drucker-Prager kappa) = 12 (element.py:222-236). */
+int efe_collect_results(efe_ctx* ctx, int32_t which, const int32_t* elements, int32_t n, const int32_t* qps,
+                        int32_t nq, int32_t first, int32_t count, double* out);
+
 /* Overwrites the reference QP state from the host (same layout); in == NULL sets it to
  * zero (the initial state of a fresh model, element.py:207-240). */
 int efe_set_qp_state(efe_ctx* ctx, const double* in);
