@@ -1,6 +1,6 @@
 // GMRES(m) with multicolor block ILU(0) or block-Jacobi right preconditioning, for the
-// non-symmetric tangents of non-associative plasticity (the survey's replacement for the
-// unsymmetric direct solve, PARDISO mtype 11, linsolve/pardiso/pardiso.pyx:104-151).
+// non-symmetric tangents of non-associative plasticity, in place of the reference's
+// unsymmetric direct solve (PARDISO mtype 11, linsolve/pardiso/pardiso.pyx:104-151).
 //
 // Block ILU(0): the zero-fill incomplete factorisation on the 3x3-block pattern of K, in
 // the elimination order (color, node) of a greedy coloring of the node graph.  The

@@ -8,8 +8,8 @@
 // the block column index (1 int per 9 values) is shared by the three rows of a
 // node.  Padding slots hold zeros and point at the row itself.
 //
-// PCG with 3x3 block-Jacobi (the survey's replacement for the direct solve of
-// nonlinearimplicitstatic.py:727-751 on symmetric tangents).  One iteration is
+// PCG with 3x3 block-Jacobi, in place of the reference's direct solve
+// (nonlinearimplicitstatic.py:727-751) on symmetric tangents.  One iteration is
 // three kernels:
 //   A: q = K p, per-workgroup partials of p.q
 //   B: alpha = rz/pq; x += alpha p; r -= alpha q; z = Minv r; partials of r.z, r.r

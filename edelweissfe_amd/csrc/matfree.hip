@@ -210,6 +210,179 @@ __global__ __launch_bounds__(256) void mf_apply_kernel(int E, int N, const int32
     }
 }
 
+// Hexa8 with 2x2x2 Gauss points: the same product with the trilinear interpolation written
+// in monomials.  A nodal field v gives v(x, y, z) = c0 + c_x x + c_y y + c_z z + c_xy xy +
+// c_yz yz + c_xz xz + c_xyz xyz with c = (1/8) H v (H: the +-1 Walsh matrix of the node
+// signs, applied as a 3-stage butterfly), so its natural gradient at a point costs 3 FMA per
+// direction instead of a sum over the 8 nodes; the element forces f_a = sum_q M_q grad N_a(q)
+// are accumulated in the same monomial basis and mapped back to the nodes by the transposed
+// butterfly.  J (from the coordinates) and H = grad p use the first form, the forces the
+// second: 90 instead of 216 multiply-adds per QP.  The evaluation point of QP q is the
+// reference's exchanged point (eta_q, xi_q, zeta_q).
+//
+// Node order (xi, eta, zeta) signs: n0 (-,-,-) n1 (-,-,+) n2 (+,-,+) n3 (+,-,-) n4 (-,+,-)
+// n5 (-,+,+) n6 (+,+,+) n7 (+,+,-); canonical butterfly index k = [xi>0] + 2 [eta>0] +
+// 4 [zeta>0]:
+__constant__ int8_t kHex8Canon[8] = {0, 3, 4, 7, 1, 2, 5, 6};  // node of canonical index k
+
+// in place: w[k] (canonical order) -> r[k] = sum_j prod_{i in bits(k)} s_i(j) w[j]
+__device__ __forceinline__ void walsh3(float (&w)[8]) {
+#pragma unroll
+    for (int h = 1; h < 8; h <<= 1)
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (!(k & h)) {
+                const float lo = w[k], hi = w[k | h];
+                w[k] = lo + hi;
+                w[k | h] = hi - lo;
+            }
+}
+
+// in place: A[k] (monomial coefficients, A[0] = 0) -> f[k] = sum_S A_S prod_{i in S} s_i(k)
+__device__ __forceinline__ void walsh3_t(float (&w)[8]) {
+#pragma unroll
+    for (int h = 1; h < 8; h <<= 1)
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (!(k & h)) {
+                const float lo = w[k], hi = w[k | h];
+                w[k] = lo - hi;
+                w[k | h] = lo + hi;
+            }
+}
+
+template <int NC, typename TP>
+__global__ __launch_bounds__(256) void mf_apply_hex8_kernel(int E, int N, const int32_t* __restrict__ conn,
+                                                            const double* __restrict__ coords,
+                                                            const float* __restrict__ dnxi,
+                                                            const float* __restrict__ mfC, const TP* __restrict__ p,
+                                                            const uint8_t* __restrict__ dirflag, int mask,
+                                                            const int* __restrict__ flags,
+                                                            const int32_t* __restrict__ pos, float* __restrict__ Fs,
+                                                            const int* __restrict__ active,
+                                                            const int32_t* __restrict__ specOfElem) {
+    (void)dnxi;
+    if (flags && flags[0]) return;
+    const int e = xcd_block_mf() * 256 + threadIdx.x;
+    if (e >= E) return;
+    if (active && !active[specOfElem[e]]) return;  // frozen specimen of a batched solve
+    // monomial coefficients (1/8 included) of p and of the coordinates relative to node 0
+    // (formed in FP64, so that FP32 keeps the element's own scale)
+    float cp[3][8], cx[3][8];
+    double x0[3];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int a = kHex8Canon[k];
+        const int nd = conn[(int64_t)a * E + e];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            float v = (float)p[3 * nd + i];
+            if (mask && dirflag[3 * nd + i]) v = 0.0f;
+            cp[i][k] = v;
+            const double xc = coords[(int64_t)i * N + nd];
+            if (k == 0) x0[i] = xc;
+            cx[i][k] = (float)(xc - x0[i]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        walsh3(cp[i]);
+        walsh3(cx[i]);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            cp[i][k] *= 0.125f;
+            cx[i][k] *= 0.125f;
+        }
+    }
+    // force coefficients in the same basis, x 8
+    float fa[3][8];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) fa[i][k] = 0.0f;
+    const float g = 0.57735026918962576451f;
+    // not unrolled: the unrolled loop hoists the tangent loads of all QPs and needs 154
+    // VGPRs (3 waves per SIMD instead of 4), measured slower; so is loading the next QP's
+    // tangent ahead
+#pragma unroll 1
+    for (int q = 0; q < 8; ++q) {
+        // evaluation point (eta_q, xi_q, zeta_q) of the QP table xi = (-,-,-,-,+,+,+,+),
+        // eta = (-,-,+,+,-,-,+,+), zeta = (-,+,+,-,-,+,+,-) (times 1/sqrt 3)
+        const int r = q & 3;
+        const float x = (r < 2 ? -g : g), y = (q < 4 ? -g : g), z = ((r == 1 || r == 2) ? g : -g);
+        const float xy = x * y, yz = y * z, xz = x * z;
+        // natural gradients: d/dx = c1 + c3 y + c5 z + c7 yz, d/dy = c2 + c3 x + c6 z + c7 xz,
+        // d/dz = c4 + c5 x + c6 y + c7 xy   (index = canonical monomial bits)
+        float H[9], J[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            H[i * 3 + 0] = cp[i][1] + cp[i][3] * y + cp[i][5] * z + cp[i][7] * yz;
+            H[i * 3 + 1] = cp[i][2] + cp[i][3] * x + cp[i][6] * z + cp[i][7] * xz;
+            H[i * 3 + 2] = cp[i][4] + cp[i][5] * x + cp[i][6] * y + cp[i][7] * xy;
+            J[0 * 3 + i] = cx[i][1] + cx[i][3] * y + cx[i][5] * z + cx[i][7] * yz;
+            J[1 * 3 + i] = cx[i][2] + cx[i][3] * x + cx[i][6] * z + cx[i][7] * xz;
+            J[2 * 3 + i] = cx[i][4] + cx[i][5] * x + cx[i][6] * y + cx[i][7] * xy;
+        }
+        const float det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
+                          J[2] * (J[3] * J[7] - J[4] * J[6]);
+        const float id = 1.0f / det;
+        const float iJ[9] = {(J[4] * J[8] - J[5] * J[7]) * id, (J[2] * J[7] - J[1] * J[8]) * id,
+                             (J[1] * J[5] - J[2] * J[4]) * id, (J[5] * J[6] - J[3] * J[8]) * id,
+                             (J[0] * J[8] - J[2] * J[6]) * id, (J[2] * J[3] - J[0] * J[5]) * id,
+                             (J[3] * J[7] - J[4] * J[6]) * id, (J[1] * J[6] - J[0] * J[7]) * id,
+                             (J[0] * J[4] - J[1] * J[3]) * id};
+        float gr[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                gr[i * 3 + j] = H[i * 3] * iJ[j * 3] + H[i * 3 + 1] * iJ[j * 3 + 1] + H[i * 3 + 2] * iJ[j * 3 + 2];
+        const float ep[6] = {gr[0], gr[4], gr[8], gr[1] + gr[3], gr[2] + gr[6], gr[5] + gr[7]};
+        float cf[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) cf[k] = __builtin_nontemporal_load(mfC + (int64_t)(q * NC + k) * E + e);
+        const float A = cf[0], B = cf[1], cnn = cf[2];
+        const float cn1 = NC == 11 ? cf[3] : 0.0f, c1n = NC == 11 ? cf[4] : 0.0f;
+        const float* n = cf + (NC == 11 ? 5 : 3);
+        const float tr = ep[0] + ep[1] + ep[2];
+        const float ne = n[0] * ep[0] + n[1] * ep[1] + n[2] * ep[2] + n[3] * ep[3] + n[4] * ep[4] + n[5] * ep[5];
+        const float t3 = tr * (1.0f / 3.0f);
+        const float sn = cnn * ne + cn1 * tr;
+        const float s1 = A * tr + c1n * ne;
+        float sg[6];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) sg[i] = s1 + B * (ep[i] - t3) + sn * n[i];
+#pragma unroll
+        for (int i = 3; i < 6; ++i) sg[i] = 0.5f * B * ep[i] + sn * n[i];
+        const float S[9] = {sg[0], sg[3], sg[4], sg[3], sg[1], sg[5], sg[4], sg[5], sg[2]};
+        // f_a,i += sum_k M_ik dN_a/dxi_k with M_ik = sum_j sig_ij iJ_jk, in monomials:
+        // fa[1] += M0, fa[2] += M1, fa[4] += M2, fa[3] += M0 y + M1 x, fa[5] += M0 z + M2 x,
+        // fa[6] += M1 z + M2 y, fa[7] += M0 yz + M1 xz + M2 xy   (M_k = column k, x 1/8)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const float m0 = S[i * 3] * iJ[0] + S[i * 3 + 1] * iJ[3] + S[i * 3 + 2] * iJ[6];
+            const float m1 = S[i * 3] * iJ[1] + S[i * 3 + 1] * iJ[4] + S[i * 3 + 2] * iJ[7];
+            const float m2 = S[i * 3] * iJ[2] + S[i * 3 + 1] * iJ[5] + S[i * 3 + 2] * iJ[8];
+            fa[i][1] += m0;
+            fa[i][2] += m1;
+            fa[i][4] += m2;
+            fa[i][3] += m0 * y + m1 * x;
+            fa[i][5] += m0 * z + m2 * x;
+            fa[i][6] += m1 * z + m2 * y;
+            fa[i][7] += m0 * yz + m1 * xz + m2 * xy;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) walsh3_t(fa[i]);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int a = kHex8Canon[k];
+        float* dst = Fs + (int64_t)pos[(int64_t)a * E + e] * 3;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) dst[i] = 0.125f * fa[i][k];
+    }
+}
+
 // Batched solves: q = K p on the nodes of active specimens and the per-node p.q values
 // that the per-specimen reductions sum; nodes of frozen specimens are left alone (p.q 0).
 __global__ __launch_bounds__(256) void mf_gather_batched_kernel(int N, const int32_t* __restrict__ nptr,
@@ -353,8 +526,8 @@ void launch_apply(efe_ctx* c, hipStream_t st, const TP* p, int mask, const int* 
     const bool dp = c->material == EFE_DRUCKER_PRAGER;
     switch (c->eltype) {
         case EFE_C3D8:
-            if (dp) mf_apply_kernel<8, 8, 11, TP><<<grid, 256, 0, st>>>(MF_ARGS);
-            else mf_apply_kernel<8, 8, 9, TP><<<grid, 256, 0, st>>>(MF_ARGS);
+            if (dp) mf_apply_hex8_kernel<11, TP><<<grid, 256, 0, st>>>(MF_ARGS);
+            else mf_apply_hex8_kernel<9, TP><<<grid, 256, 0, st>>>(MF_ARGS);
             break;
         case EFE_C3D20:
             if (dp) mf_apply_kernel<20, 27, 11, TP><<<grid, 256, 0, st>>>(MF_ARGS);
