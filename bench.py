@@ -120,6 +120,13 @@ def spmv_roofline(system, n_launch=50):
     return ms, alg
 
 
+def csr_model_bytes(system):
+    """Bytes of the same product in scalar CSR (8 B value + 4 B column index per nonzero, the
+    row pointers, x and y once): the per-unit model of a CSR SpMV, which the 3x3 block layout
+    undercuts by storing one column index per block."""
+    return 12 * system.nnz + 4 * (system.n_dof + 1) + 16 * system.n_dof
+
+
 def profile_spmv(system, n_launch=20, copy_mb=512):
     """Counter-collection mode: one FP64 element pass and assembly of the operator, then
     n_launch FP64 SpMV launches, n_launch plain device copies of copy_mb MiB (the calibration
@@ -275,6 +282,8 @@ def bench_batched(args):
         "traffic": None,
         "algorithmic_bytes": spmv_alg,
         "avg_launch_ms": spmv_ms,
+        "csr_model_bytes": csr_model_bytes(sysm),
+        "csr_model_achieved": csr_model_bytes(sysm) / (spmv_ms * 1e-3) / 1e9,
     }
     if not args.no_cpu:
         try:
@@ -427,6 +436,8 @@ def bench_footing(args):
         "traffic": None,
         "algorithmic_bytes": spmv_alg,
         "avg_launch_ms": spmv_ms,
+        "csr_model_bytes": csr_model_bytes(system),
+        "csr_model_achieved": csr_model_bytes(system) / (spmv_ms * 1e-3) / 1e9,
     }
     if not args.no_cpu:
         try:
@@ -575,6 +586,8 @@ def main():
         "traffic": traffic["bytes_per_launch"] if traffic else None,
         "algorithmic_bytes": spmv_alg,
         "avg_launch_ms": spmv_ms,
+        "csr_model_bytes": csr_model_bytes(system),
+        "csr_model_achieved": csr_model_bytes(system) / (spmv_ms * 1e-3) / 1e9,
     }
 
     if not args.no_cpu:
