@@ -80,6 +80,7 @@ SIGNATURES = {
     "efe_body_force": (_I, [_P, _P, _P]),
     "efe_surface_traction": (_I, [_P, _P, _P, _I, _P, _P]),
     "efe_collect_results": (_I, [_P, _I, _P, _I, _P, _I, _I, _I, _P]),
+    "efe_get_element_matrices": (_I, [_P, _P, _I, _P, _P]),
     "efe_set_geostatic": (_I, [_P, ctypes.POINTER(_D), _P, _I]),
     "efe_export_csr_values": (_I, [_P, _P]),
     "efe_spmv": (_I, [_P, _P, _P]),

@@ -446,6 +446,35 @@ int efe_export_csr_values(efe_ctx* c, double* values) {
     return efe_launch_export_csr(c, values);
 }
 
+int efe_get_element_matrices(efe_ctx* c, const int32_t* elements, int32_t n, double* Ke, double* Pe) {
+    if (int rc = check(c)) return rc;
+    if (n < 0 || (n > 0 && (!elements || !Ke))) {
+        efe_set_error("efe_get_element_matrices: invalid arguments");
+        return EFE_ERR_ARG;
+    }
+    if (c->Kprec < 0 || c->Kprec == EFE_FP32_MATFREE) {
+        efe_set_error("efe_get_element_matrices: the last element pass formed no element matrices (matrix-free)");
+        return EFE_ERR_STATE;
+    }
+    for (int32_t i = 0; i < n; ++i)
+        if (elements[i] < 0 || elements[i] >= c->E) {
+            efe_set_error("efe_get_element_matrices: element out of range");
+            return EFE_ERR_ARG;
+        }
+    if (n == 0) return EFE_OK;
+    if ((size_t)n > c->resCap) {
+        hipFree(c->resIdx);
+        c->resIdx = nullptr;
+        c->resCap = 0;
+        EFE_HIP(hipMalloc(&c->resIdx, (size_t)n * sizeof(int32_t)));
+        c->resCap = (size_t)n;
+    }
+    EFE_HIP(hipMemcpyAsync(c->resIdx, elements, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    if (int rc = efe_launch_export_elements(c, c->resIdx, n, Ke, Pe)) return rc;
+    EFE_HIP(hipStreamSynchronize(c->stream));
+    return EFE_OK;
+}
+
 int efe_spmv(efe_ctx* c, const double* x, double* y) {
     if (int rc = check(c)) return rc;
     if (!x || !y) {

@@ -174,6 +174,7 @@ int efe_launch_spmv_mf(efe_ctx* c, const double* x, double* y);
 void efe_mf_launch_operator_batched(efe_ctx* c, hipStream_t st, const float* p, float* q, double* npq,
                                     const int* active);
 int efe_launch_export_csr(efe_ctx* c, double* values);
+int efe_launch_export_elements(efe_ctx* c, const int32_t* elems, int n, double* Ke, double* Pe);
 
 // Krylov (krylov.hip)
 int efe_krylov_solve(efe_ctx* c, const double* b, double* x, const efe_solve_opts* o, efe_solve_info* info);

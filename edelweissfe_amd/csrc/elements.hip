@@ -677,6 +677,46 @@ int efe_launch_geostatic(efe_ctx* c, const double* def, const int32_t* elements,
     return EFE_OK;
 }
 
+// Element matrices in the layout of the native element ABI (MarmotElement::computeYourself,
+// marmotelement/element.pxd:97-103: Ke column-major nDof x nDof, Pe nDof, DOFs node-major):
+// thread per (listed element, block a b); the block is read from its contribution position.
+template <typename KT>
+__global__ void export_element_kernel(int n, int NN, const int32_t* __restrict__ elems,
+                                      const int32_t* __restrict__ cpos, const KT* __restrict__ Kscr,
+                                      const double* __restrict__ Pscr, double* __restrict__ Ke,
+                                      double* __restrict__ Pe) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nb = (int64_t)NN * NN;
+    if (t >= (int64_t)n * nb) return;
+    const int k = (int)(t / nb), ab = (int)(t - (int64_t)k * nb);
+    const int a = ab / NN, b = ab - (ab / NN) * NN;
+    const int e = elems[k];
+    const int nd = 3 * NN;
+    const KT* src = Kscr + (int64_t)cpos[(int64_t)e * nb + ab] * kBlockPitch;
+    double* dst = Ke + (int64_t)k * nd * nd;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) dst[(int64_t)(3 * b + j) * nd + 3 * a + i] = (double)src[i * 3 + j];
+    if (Pe && b == 0) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) Pe[(int64_t)k * nd + 3 * a + i] = Pscr[((int64_t)e * NN + a) * 3 + i];
+    }
+}
+
+int efe_launch_export_elements(efe_ctx* c, const int32_t* elems, int n, double* Ke, double* Pe) {
+    const int64_t total = (int64_t)n * c->NN * c->NN;
+    const unsigned grid = (unsigned)((total + 255) / 256);
+    if (c->Kprec == EFE_FP32)
+        export_element_kernel<float><<<grid, 256, 0, c->stream>>>(n, c->NN, elems, c->cpos, (const float*)c->Kscr,
+                                                                   c->Pscr, Ke, Pe);
+    else
+        export_element_kernel<double><<<grid, 256, 0, c->stream>>>(n, c->NN, elems, c->cpos,
+                                                                    (const double*)c->Kscr, c->Pscr, Ke, Pe);
+    EFE_HIP(hipGetLastError());
+    return EFE_OK;
+}
+
 int efe_launch_export_csr(efe_ctx* c, double* values) {
     const int grid = (c->N + 255) / 256;
     if (c->Kprec == EFE_FP32)

@@ -125,6 +125,17 @@ class DeviceSystem:
             el = np.ascontiguousarray(elements, dtype=np.int32)
             check(self.lib.efe_set_geostatic(self.ctx, d, el.ctypes.data, int(el.size)), "efe_set_geostatic")
 
+    def element_matrices(self, elements):
+        """Ke [n][nDof][nDof] (column-major per element, i.e. Ke[k][col][row]) and Pe [n][nDof] of
+        the last assembled element pass, as device tensors (the native element ABI layout)."""
+        el = np.ascontiguousarray(elements, dtype=np.int32)
+        nd = 3 * (8 if self.el_type == "C3D8" else 20)
+        Ke = torch.empty((el.size, nd, nd), dtype=torch.float64, device=self.device)
+        Pe = torch.empty((el.size, nd), dtype=torch.float64, device=self.device)
+        check(self.lib.efe_get_element_matrices(self.ctx, el.ctypes.data, int(el.size), ctypes.c_void_p(Ke.data_ptr()),
+                                                ctypes.c_void_p(Pe.data_ptr())), "efe_get_element_matrices")
+        return Ke, Pe
+
     def export_csr_values(self):
         out = torch.empty(self.nnz, dtype=torch.float64, device=self.device)
         check(self.lib.efe_export_csr_values(self.ctx, ctypes.c_void_p(out.data_ptr())), "efe_export_csr_values")

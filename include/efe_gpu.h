@@ -206,6 +206,14 @@ int efe_accept_state(efe_ctx* ctx);
  * (element.py:222-236). */
 int efe_get_qp_state(efe_ctx* ctx, int32_t which, double* out);
 
+/* Element stiffness and residual of the last element pass for elements[i] (host [n]), in
+ * the layout of the native element ABI (MarmotElement::computeYourself,
+ * marmotelement/element.pxd:97-103): Ke device [n][nDof][nDof] column-major with DOFs
+ * node-major (3a + i), Pe device [n][nDof] (may be NULL); nDof = 3 x nodes per element.
+ * For a host-side assembly in the manner of NISTParallel (nonlinearimplicitstaticparallel.pyx),
+ * or inspection.  Needs an assembled (FP64 or FP32) pass, not a matrix-free one. */
+int efe_get_element_matrices(efe_ctx* ctx, const int32_t* elements, int32_t n, double* Ke, double* Pe);
+
 /* Device gather of QP results (utils/elementresultcollector.pyx:37-132, which collects
  * element.getResultArray(result, qp) for a list of elements and quadrature points): out
  * (device [n][nq][count]) = components first .. first+count-1 of the reference (which=0) or

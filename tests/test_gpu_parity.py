@@ -493,3 +493,29 @@ def test_matfree_operator_matches_assembled(el_type, material, props, n):
     # Dirichlet rows are identity rows of the FP32 operator
     d = torch.as_tensor(np.unique(dirs), device="cuda")
     assert torch.equal(ym[d], x[d].float().double())
+
+
+@pytest.mark.parametrize("case", [
+    ((4, 3, 5), "C3D8", "vonmises", J2, 3e-3),
+    ((2, 2, 2), "C3D20R", "druckerprager", None, 3e-3),
+])
+def test_element_matrices_in_native_element_layout(case):
+    """efe_get_element_matrices: Ke column-major and Pe per element, as
+    MarmotElement::computeYourself returns them, equal to the oracle element pass."""
+    n, et, mat, props, scale = case
+    if props is None:
+        props = DP
+    om = oracle_model(n, (1.0, 1.0, 1.0), et, mat, props, jitter=0.1, seed=4)
+    pm = device_model(n, (1.0, 1.0, 1.0), et, mat, props, jitter=0.1, seed=4)
+    sysm = _system(pm)
+    dU = scale * np.random.RandomState(4).uniform(-1.0, 1.0, om.nDof)
+    Ke, Pe, _ = om.elements.compute(dU[om.elements.dof_map], om.state, om.material)
+    P, F = sysm.zeros(), sysm.zeros()
+    sysm.compute_elements(torch.as_tensor(dU, device="cuda"), P, F)
+    els = np.array([0, om.elements.conn.shape[0] - 1, 3, 1])
+    Kd, Pd = (t.cpu().numpy() for t in sysm.element_matrices(els))
+    sysm.close()
+    for k, e in enumerate(els):
+        # column-major: Kd[k][col][row]
+        assert rel_err(Kd[k].T, Ke[e]) < 1e-12
+        assert rel_err(Pd[k], Pe[e]) < 1e-12
