@@ -221,14 +221,17 @@ class DruckerPrager:
             D_c = (t1[:, None, None] * Id + t2[:, None, None] * nn
                    - t3[:, None, None] * (eta[:, None, None] * nI + etab[:, None, None] * In)
                    + t4[:, None, None] * II)
-            # apex: solve xi/etab-scaled volumetric return (linear hardening => closed form)
-            alpha = xi / etab
-            beta = xi / eta
-            # r(dev) = beta c(e + alpha dev) - p_tr + K dev = 0
-            dev_v = (ptr - beta * coh) / (alpha * beta * H + K)
-            p_a = ptr - K * dev_v
-            sig_a = p_a[:, None] * one[None, :]
-            D_a = (K * (1.0 - K / (K + alpha * beta * H)))[:, None, None] * II
+            # apex: solve xi/etab-scaled volumetric return (linear hardening => closed form);
+            # evaluated for every point and selected below, so a point on the cone with
+            # etab = 0 (no apex return exists) must not warn here
+            with np.errstate(invalid="ignore", divide="ignore"):
+                alpha = xi / etab
+                beta = xi / eta
+                # r(dev) = beta c(e + alpha dev) - p_tr + K dev = 0
+                dev_v = (ptr - beta * coh) / (alpha * beta * H + K)
+                p_a = ptr - K * dev_v
+                sig_a = p_a[:, None] * one[None, :]
+                D_a = (K * (1.0 - K / (K + alpha * beta * H)))[:, None, None] * II
             new_stress = np.where(cone[:, None], sig_c, new_stress)
             new_stress = np.where(apex[:, None], sig_a, new_stress)
             C = np.where(cone[:, None, None], D_c, C)
