@@ -27,3 +27,7 @@ class ReachedMinIncrementSize(Exception):
 
 class DivergingSolution(Exception):
     """The nonlinear iteration seems to diverge."""
+
+
+class ConditionalStop(Exception):
+    """The simulation stops successfully (a stop condition of an output manager holds)."""

@@ -56,8 +56,9 @@ import torch
 
 from .. import _lib
 from ..device import DeviceSystem
-from ..exceptions import (CutbackRequest, DivergingSolution, ReachedMaxIncrements, ReachedMaxIterations,
-                          ReachedMinIncrementSize, StepFailed)
+from ..exceptions import (ConditionalStop, CutbackRequest, DivergingSolution, ReachedMaxIncrements,
+                          ReachedMaxIterations, ReachedMinIncrementSize, StepFailed)
+from ..outputmanagers.timing import createTimingDict
 
 FIELD_CORRECTION_TOL = 1e-8      # phenomena.py fieldCorrectionTolerance["displacement"]
 FLUX_RESIDUAL_TOL = 1e-8         # phenomena.py fluxResidualTolerance["displacement"]
@@ -90,8 +91,11 @@ class NISTGPU:
         self.options = self.NISTOptions.copy()
         self._updateOptions(kwargs)
         self.system = None
-        self.history = []          # per increment: dict(inc, stepProgress, iters, converged, krylov)
-        self.computationTimes = {k: 0.0 for k in ("elements", "linear solve", "convergence check", "loads")}
+        # per increment: the status of the reference's status file (step, inc, iters, converged,
+        # time inc, time end, notes) plus stepProgress, Krylov iterations per solve and precision
+        self.history = []
+        # running host-side times per category (nonlinearimplicitstatic.py:166, config/timing.py)
+        self.computationTimes = createTimingDict()
         self.krylovIterations = 0
         self.linearSolves = 0
         self.elementPasses = 0
@@ -155,10 +159,14 @@ class NISTGPU:
         prevTimeStep = None
         maxIter, criticalIter = step.maxIter, step.criticalIter
         maxGrowingIter, cutbackFactor = step.maxGrowIter, step.cutbackFactor
+        # results go to the model after every increment only when someone reads them there
+        perIncrement = bool(outputmanagers) or fieldOutputController is not None
         try:
             for timeStep in step.getTimeStep():
-                record = {"inc": timeStep.number, "stepProgress": timeStep.stepProgress, "iters": None,
-                          "converged": False, "krylov": [], "precision": self._precisionCode()}
+                record = {"step": step.number, "inc": timeStep.number, "iters": None, "converged": False,
+                          "time inc": timeStep.timeIncrement, "time end": timeStep.totalTime, "notes": "",
+                          "stepProgress": timeStep.stepProgress, "krylov": [], "precision": self._precisionCode()}
+                self.history.append(record)
                 try:
                     U_new, dU, P, iters = self.solveIncrement(U, dU, P, step, model, timeStep, prevTimeStep,
                                                               self.options["extrapolation"], maxIter,
@@ -168,11 +176,13 @@ class NISTGPU:
                     prevTimeStep = None
                     record["iters"] = np.inf
                     record["notes"] = str(e)
+                    self._finalizeOutputs(outputmanagers, record, failed=True)
                 except (ReachedMaxIterations, DivergingSolution) as e:
                     step.discardAndChangeIncrement(cutbackFactor)
                     prevTimeStep = None
                     record["iters"] = np.inf
                     record["notes"] = str(e)
+                    self._finalizeOutputs(outputmanagers, record, failed=True)
                 else:
                     prevTimeStep = timeStep
                     if iters >= criticalIter:
@@ -180,21 +190,51 @@ class NISTGPU:
                     U = U_new
                     sysm.accept_state()
                     model.time = timeStep.totalTime
+                    if perIncrement:
+                        self._writeResults(model, U, P, dU)
                     record["iters"] = iters
                     record["converged"] = True
-                self.history.append(record)
+                    if fieldOutputController is not None:
+                        fieldOutputController.finalizeIncrement()
+                    self._finalizeOutputs(outputmanagers, record, failed=False)
         except (ReachedMaxIncrements, ReachedMinIncrementSize):
             raise StepFailed()
-        torch.cuda.synchronize(dev)
-        model.U = U.cpu().numpy()
-        model.P = P.cpu().numpy()
-        model.dU = dU.cpu().numpy()
+        except ConditionalStop:
+            pass  # a successful end of the step (nonlinearimplicitstatic.py:323-325)
+        if dev.type == "cuda":
+            torch.cuda.synchronize(dev)
+        self._writeResults(model, U, P, dU)
         model._qpState = None
         model.deviceSystem = sysm
         for group in ("dirichlet", "nodeforces", "distributedload", "bodyforce", "geostatic"):
             for action in step.actions.get(group, {}).values():
                 action.applyAtStepEnd(model)
         return True, model
+
+    @staticmethod
+    def _writeResults(model, U, P, dU):
+        model.U = U.cpu().numpy()
+        model.P = P.cpu().numpy()
+        model.dU = dU.cpu().numpy()
+
+    _STATUS_KEYS = ("step", "inc", "iters", "converged", "time inc", "time end", "notes")
+
+    def _finalizeOutputs(self, outputmanagers, record, failed):
+        """finalize(Failed)Increment of every output manager with the reference's statusInfoDict
+        and the running computationTimes (nonlinearimplicitstatic.py:255-315); the time spent
+        there is charged to "output".  A ConditionalStop raised by a manager propagates."""
+        if not outputmanagers:
+            return
+        status = {k: record[k] for k in self._STATUS_KEYS}
+        tic = _time.perf_counter()
+        try:
+            for man in outputmanagers:
+                if failed:
+                    man.finalizeFailedIncrement(statusInfoDict=status, currentComputingTimes=self.computationTimes)
+                else:
+                    man.finalizeIncrement(statusInfoDict=status, currentComputingTimes=self.computationTimes)
+        finally:
+            self.computationTimes["output"] += _time.perf_counter() - tic
 
     # ---- one increment (nonlinearimplicitstatic.py:334-458) ---------------------------------------------------
     def solveIncrement(self, U_n, dU, P, step, model, timeStep, prevTimeStep, extrapolation, maxIter, maxGrowingIter,
@@ -243,11 +283,13 @@ class NISTGPU:
             self.elementPasses += 1
             if st == _lib.EFE_CUTBACK:
                 raise CutbackRequest("Von Mises Newton failed.", pnew)
+            tic = _time.perf_counter()
             torch.add(P, PExt, out=R)
             if first:
                 R[self._dirAll] = delta
             elif self._dirAll.numel():
                 R[self._dirAll] = 0.0
+            self.computationTimes["dirichlet R"] += _time.perf_counter() - tic
             tic = _time.perf_counter()
             norms = self.system.norms(R, ddU, F)
             self.computationTimes["convergence check"] += _time.perf_counter() - tic
@@ -320,12 +362,16 @@ class NISTGPU:
             load = torch.as_tensor(np.tile(nf.getCurrentLoad(timeStep), (len(nodes), 1)).ravel(),
                                    dtype=torch.float64, device=sysm.device)
             PExt[idx] += load
+        tic = _time.perf_counter()
         for dl in step.actions.get("distributedload", {}).values():
             sysm.surface_traction(model.surfaces[dl.surface], dl.getCurrentLoad(timeStep), PExt)
+        toc = _time.perf_counter()
+        self.computationTimes["distributed loads"] += toc - tic
         for bf in step.actions.get("bodyforce", {}).values():
             if len(model.elementSets[bf.elSet]) != model.nElements:
                 raise NotImplementedError("body forces on element subsets")
             sysm.body_force(bf.getCurrentLoad(timeStep), PExt)
+        self.computationTimes["body forces"] += _time.perf_counter() - toc
         return PExt
 
     def _fluxThreshold(self, norms, iterationCounter):

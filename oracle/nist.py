@@ -160,6 +160,9 @@ class NIST:
         self.element_kernel = element_kernel
         self.convention = convention
         self.history = []   # (increment number, stepProgress, iterations or None on failure)
+        # per increment, the fields of the reference's status-file line (statusInfoDict,
+        # nonlinearimplicitstatic.py:207-215, :250-253, :267-270, :304-305)
+        self.increments = []
 
     # -- setup, as solveStep :133-164
     def _setup(self, model):
@@ -296,6 +299,10 @@ class NIST:
             iterationCounter += 1
         return U_np, dU, P, st, iterationCounter
 
+    def _status(self, ts, iters, converged, notes):
+        self.increments.append({"inc": ts.number, "iters": iters, "converged": converged,
+                                "time inc": ts.timeIncrement, "time end": ts.totalTime, "notes": notes})
+
     def solve_step(self, model, step):
         self._setup(model)
         stepper = AdaptiveTimeStepper(model.time, step.stepLength, step.startInc, step.maxInc, step.minInc,
@@ -312,10 +319,12 @@ class NIST:
                     stepper.discardAndChangeIncrement(max(e.cutbackSize, step.cutbackFactor))
                     prev_ts = None
                     self.history.append((ts.number, ts.stepProgress, None))
-                except (ReachedMaxIterations, DivergingSolution):
+                    self._status(ts, np.inf, False, str(e))
+                except (ReachedMaxIterations, DivergingSolution) as e:
                     stepper.discardAndChangeIncrement(step.cutbackFactor)
                     prev_ts = None
                     self.history.append((ts.number, ts.stepProgress, None))
+                    self._status(ts, np.inf, False, str(e))
                 else:
                     prev_ts = ts
                     if iters >= step.criticalIter:
@@ -324,6 +333,7 @@ class NIST:
                     model.state = st
                     model.time = ts.totalTime
                     self.history.append((ts.number, ts.stepProgress, iters))
+                    self._status(ts, iters, True, "")
         except (ReachedMaxIncrements, ReachedMinIncrementSize):
             raise StepFailed()
         model.U = U

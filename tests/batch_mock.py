@@ -131,3 +131,36 @@ def attach_mock(solver, device_models, oracle_models):
     solver.elementOffset = list(np.r_[0, np.cumsum([m.nElements for m in device_models])])
     solver.dofOffset = [3 * o for o in solver.nodeOffset]
     return solver.system
+
+
+class OracleSystem(OracleBatchSystem):
+    """The same stand-in for one DeviceSystem under NISTGPU (one specimen)."""
+
+    def __init__(self, oracle_model):
+        super().__init__([oracle_model])
+        self.n_nodes = oracle_model.coords.shape[0]
+        self.n_specimens = 1
+
+    def reset_state(self):
+        pass
+
+    def norms(self, R, ddU, F):
+        Rh = R.numpy()
+        dh = ddU.numpy() if ddU is not None else None
+        nan = not np.isfinite(Rh).all() or (dh is not None and not np.isfinite(dh).all())
+        return np.array([np.abs(Rh).max(), float(np.abs(Rh).argmax()), np.abs(dh).max() if dh is not None else 0.0,
+                         np.abs(F.numpy()).sum(), float(nan)])
+
+    def linsolve(self, b, x, **kw):
+        info, its = self.linsolve_batched(b, x, None)
+        info.iterations = int(its[0])
+        return info
+
+    def accept_state(self):
+        self.accept_state_batched([True])
+
+
+def attach_mock_single(solver, oracle_model):
+    """Points a NISTGPU at an OracleSystem built from `oracle_model`."""
+    solver.system = OracleSystem(oracle_model)
+    return solver.system

@@ -38,3 +38,35 @@ def test_collector_matches_qp_state_after_a_j2_history():
     assert (q[..., 12] > 0).any()
     with pytest.raises(Exception):
         ElementResultCollector(sysm, els, [0], "porePressure")
+
+
+def test_status_file_of_a_cutback_run_matches_the_oracle(tmp_path):
+    """The status file (statusfile.py) of a device run with material cutbacks against the
+    oracle run's increments: number, Newton iterations (inf when failed), converged flag,
+    time increment, end time and failure message; the compute-time monitor reports every
+    increment."""
+    from edelweissfe_amd.outputmanagers import ComputeTimeMonitor, StatusFile
+    from edelweissfe_amd.outputmanagers.statusfile import read_status_file
+    from edelweissfe_amd.solvers import NISTGPU
+    from helpers import CUTBACK_SPECIMENS, cutback_oracle_run
+
+    n, props, strain, maxInc, maxNumInc, seed = CUTBACK_SPECIMENS[0]
+    pm = device_model(n, (1.0, 1.0, 1.0), "C3D8", "vonmises", props, jitter=0.1, seed=seed)
+    sf, ct = StatusFile(), ComputeTimeMonitor()
+    sf.updateDefinition(filename=str(tmp_path / "job.sta"))
+    ct.initializeStep(None)
+    s = NISTGPU(precision="fp64", krylovAtolFactor=1e-8)
+    s.solveStep(device_step(pm, uniaxial_bcs(strain), maxInc=maxInc, minInc=1e-6, maxNumInc=maxNumInc, maxIter=15),
+                pm, outputmanagers=[sf, ct])
+    so, om, failed = cutback_oracle_run(*CUTBACK_SPECIMENS[0])
+    rows = read_status_file(tmp_path / "job.sta")
+    want = so.increments
+    print([(r["inc"], r["iters"]) for r in rows], [(w["inc"], w["iters"]) for w in want])
+    assert not failed and any(not w["converged"] for w in want)
+    assert len(rows) == len(want)
+    for r, w in zip(rows, want):
+        assert (r["inc"], r["iters"], r["converged"], r["notes"]) == (w["inc"], float(w["iters"]), w["converged"],
+                                                                      w["notes"])
+        assert r["time inc"] == pytest.approx(w["time inc"], rel=5e-4)
+        assert r["time end"] == pytest.approx(w["time end"], rel=5e-4)
+    assert len(ct.increments) == len(want) and ct.increments[-1][2] > 0
