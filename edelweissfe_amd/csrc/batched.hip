@@ -2,8 +2,11 @@
 // PCG whose scalars, tolerance and convergence are per specimen.
 //
 // Specimens are contiguous node ranges.  Per-specimen reductions are deterministic and
-// size independent: every kernel writes one value per node (the node's 3-DOF share of a
-// dot product) and one workgroup per specimen sums its node range in a fixed order.
+// size independent.  Inside the PCG iteration each specimen's node range is cut into
+// 256-node chunks that start at the specimen's first node; a workgroup sums one chunk in a
+// fixed order and one wavefront sums a specimen's chunk partials in a fixed order, so the
+// order depends only on positions relative to the specimen.  The set-up reductions sum
+// per-node values over the whole range in one workgroup per specimen.
 // A converged specimen is frozen (its rows are skipped), so each specimen follows exactly
 // the iteration of a PCG run on its own system.
 #include <cmath>
@@ -15,11 +18,15 @@ struct BatchWork {
     int N = 0, S = 0, prec = -1;
     void *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
     double *nv0 = nullptr, *nv1 = nullptr;     // per-node values
-    double* sc = nullptr;                      // per specimen: [0] rz, [1] pq/alpha, [2] rr, [3] tol2, [4] beta
+    double* sc = nullptr;                      // per specimen: [0] rz, [1] unused, [2] rr, [3] tol2, [4] beta
     int *active = nullptr, *iters = nullptr;   // per specimen
     int* nactive = nullptr;                    // device count
     int* acc = nullptr;                        // per specimen accept flags
     double* nout = nullptr;                    // per specimen norms [S][4]
+    int C = 0;                                 // 256-node chunks over all specimens
+    int32_t *chunkNode0 = nullptr, *chunkSpec = nullptr;  // [C] first node, specimen
+    int32_t* specChunk = nullptr;              // [S+1] first chunk of each specimen
+    double *pqp = nullptr, *rzp = nullptr, *rrp = nullptr;  // [C] chunk partials of p.q, r.z, r.r
     int* hbuf = nullptr;                       // pinned
     hipStream_t ks = nullptr;
     hipEvent_t evA = nullptr, evB = nullptr;
@@ -80,11 +87,15 @@ __device__ __forceinline__ void range_sums(const double* const (&v)[K], int n0, 
     wsums<K>(out, sh);
 }
 
-__device__ __forceinline__ double range_sum(const double* __restrict__ v, int n0, int n1, double* sh) {
-    const double* const arr[1] = {v};
-    double out[1];
-    range_sums<1>(arr, n0, n1, out, sh);
-    return out[0];
+// Sum of chunk partials v[k0, k1) by one wavefront: lane j adds k0 + j, k0 + j + 64, ...,
+// then an xor butterfly; every lane ends with the same bits.
+__device__ __forceinline__ double chunk_sum(const double* __restrict__ v, int k0, int k1) {
+    const int lane = threadIdx.x & 63;
+    double a = 0.0;
+    for (int k = k0 + lane; k < k1; k += 64) a += v[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    return a;
 }
 
 template <typename T>
@@ -177,62 +188,78 @@ __global__ __launch_bounds__(BS) void b_spmv_kernel(int N, const int32_t* __rest
     npq[n] = (double)p[3 * n] * y0 + (double)p[3 * n + 1] * y1 + (double)p[3 * n + 2] * y2;
 }
 
-__global__ __launch_bounds__(BS) void b_alpha_kernel(const int64_t* __restrict__ noff, const double* __restrict__ npq,
-                                                     double* __restrict__ sc, const int* __restrict__ active) {
-    __shared__ double sh[BS];
-    const int s = blockIdx.x;
+// p.q of one chunk (workgroup per chunk; frozen specimens skip)
+__global__ __launch_bounds__(BS) void b_pq_chunk_kernel(const int32_t* __restrict__ chunkNode0,
+                                                        const int32_t* __restrict__ chunkSpec,
+                                                        const int64_t* __restrict__ noff,
+                                                        const int* __restrict__ active,
+                                                        const double* __restrict__ npq, double* __restrict__ pqp) {
+    __shared__ double sh[BS / 64];
+    const int c = blockIdx.x;
+    const int s = chunkSpec[c];
     if (!active[s]) return;
-    const double pq = range_sum(npq, (int)noff[s], (int)noff[s + 1], sh);
-    if (threadIdx.x == 0) sc[s * NSC + ALPHA] = sc[s * NSC + RZ] / pq;
+    const int n = chunkNode0[c] + threadIdx.x;
+    double v[1] = {n < noff[s + 1] ? npq[n] : 0.0};
+    wsums<1>(v, sh);
+    if (threadIdx.x == 0) pqp[c] = v[0];
 }
 
+// x += alpha p, r -= alpha q, z = Minv r with alpha = rz / (sum of the specimen's p.q chunk
+// partials), and the chunk partials of r.z and r.r (workgroup per chunk)
 template <typename T>
-__global__ __launch_bounds__(BS) void b_update_kernel(int N, const int32_t* __restrict__ spec,
+__global__ __launch_bounds__(BS) void b_update_kernel(const int32_t* __restrict__ chunkNode0,
+                                                      const int32_t* __restrict__ chunkSpec,
+                                                      const int32_t* __restrict__ specChunk,
+                                                      const int64_t* __restrict__ noff,
                                                       const int* __restrict__ active, const double* __restrict__ sc,
-                                                      const T* __restrict__ Minv, T* __restrict__ x,
-                                                      T* __restrict__ r, T* __restrict__ z, const T* __restrict__ p,
-                                                      const T* __restrict__ q, double* __restrict__ nrz,
-                                                      double* __restrict__ nrr) {
-    const int n = blockIdx.x * BS + threadIdx.x;
-    if (n >= N) return;
-    const int s = spec[n];
-    if (!active[s]) {
-        nrz[n] = 0.0;
-        nrr[n] = 0.0;
-        return;
-    }
-    const T alpha = (T)sc[s * NSC + ALPHA];
-    T rv[3];
-    double rr = 0, rz = 0;
+                                                      const double* __restrict__ pqp, const T* __restrict__ Minv,
+                                                      T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+                                                      const T* __restrict__ p, const T* __restrict__ q,
+                                                      double* __restrict__ rzp, double* __restrict__ rrp) {
+    __shared__ double sh[2 * (BS / 64)];
+    const int c = blockIdx.x;
+    const int s = chunkSpec[c];
+    if (!active[s]) return;
+    const double pq = chunk_sum(pqp, specChunk[s], specChunk[s + 1]);
+    const T alpha = (T)(sc[s * NSC + RZ] / pq);
+    const int n = chunkNode0[c] + threadIdx.x;
+    double part[2] = {0.0, 0.0};  // r.z, r.r
+    if (n < noff[s + 1]) {
+        T rv[3];
+        double rr = 0, rz = 0;
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        x[3 * n + i] += alpha * p[3 * n + i];
-        rv[i] = r[3 * n + i] - alpha * q[3 * n + i];
-        r[3 * n + i] = rv[i];
-        rr += (double)rv[i] * rv[i];
-    }
-    const T* M = Minv + (int64_t)n * 9;
+        for (int i = 0; i < 3; ++i) {
+            x[3 * n + i] += alpha * p[3 * n + i];
+            rv[i] = r[3 * n + i] - alpha * q[3 * n + i];
+            r[3 * n + i] = rv[i];
+            rr += (double)rv[i] * rv[i];
+        }
+        const T* M = Minv + (int64_t)n * 9;
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        const T zi = M[3 * i] * rv[0] + M[3 * i + 1] * rv[1] + M[3 * i + 2] * rv[2];
-        z[3 * n + i] = zi;
-        rz += (double)rv[i] * zi;
+        for (int i = 0; i < 3; ++i) {
+            const T zi = M[3 * i] * rv[0] + M[3 * i + 1] * rv[1] + M[3 * i + 2] * rv[2];
+            z[3 * n + i] = zi;
+            rz += (double)rv[i] * zi;
+        }
+        part[0] = rz;
+        part[1] = rr;
     }
-    nrz[n] = rz;
-    nrr[n] = rr;
+    wsums<2>(part, sh);
+    if (threadIdx.x == 0) {
+        rzp[c] = part[0];
+        rrp[c] = part[1];
+    }
 }
 
-// beta = rz_new / rz, convergence per specimen, count of specimens still iterating
-__global__ __launch_bounds__(BS) void b_beta_kernel(const int64_t* __restrict__ noff, const double* __restrict__ nrz,
-                                                    const double* __restrict__ nrr, double* __restrict__ sc,
-                                                    int* __restrict__ active, int* __restrict__ iters) {
-    __shared__ double sh[BS];
+// beta = rz_new / rz, convergence per specimen (one wavefront per specimen)
+__global__ __launch_bounds__(64) void b_beta_kernel(const int32_t* __restrict__ specChunk,
+                                                    const double* __restrict__ rzp, const double* __restrict__ rrp,
+                                                    double* __restrict__ sc, int* __restrict__ active,
+                                                    int* __restrict__ iters) {
     const int s = blockIdx.x;
     if (!active[s]) return;
-    const double* const arr[2] = {nrz, nrr};
-    double sums[2];
-    range_sums<2>(arr, (int)noff[s], (int)noff[s + 1], sums, sh);
-    const double rz = sums[0], rr = sums[1];
+    const double rz = chunk_sum(rzp, specChunk[s], specChunk[s + 1]);
+    const double rr = chunk_sum(rrp, specChunk[s], specChunk[s + 1]);
     if (threadIdx.x == 0) {
         sc[s * NSC + BETA] = rz / sc[s * NSC + RZ];
         sc[s * NSC + RZ] = rz;
@@ -283,10 +310,11 @@ void launch_b_batch(efe_ctx* c, BatchWork* w, hipStream_t st, const T* K, const 
                                                       c->slotNode, K, p, q, w->nv0);
         else  // matrix-free FP32 operator
             efe_mf_launch_operator_batched(c, st, (const float*)p, (float*)q, w->nv0, w->active);
-        b_alpha_kernel<<<S, BS, 0, st>>>(c->specNodeOff, w->nv0, w->sc, w->active);
-        b_update_kernel<T><<<gridn(N), BS, 0, st>>>(N, c->specOfNode, w->active, w->sc, Minv, x, r, z, p, q, w->nv0,
-                                                     w->nv1);
-        b_beta_kernel<<<S, BS, 0, st>>>(c->specNodeOff, w->nv0, w->nv1, w->sc, w->active, w->iters);
+        b_pq_chunk_kernel<<<w->C, BS, 0, st>>>(w->chunkNode0, w->chunkSpec, c->specNodeOff, w->active, w->nv0,
+                                               w->pqp);
+        b_update_kernel<T><<<w->C, BS, 0, st>>>(w->chunkNode0, w->chunkSpec, w->specChunk, c->specNodeOff, w->active,
+                                                 w->sc, w->pqp, Minv, x, r, z, p, q, w->rzp, w->rrp);
+        b_beta_kernel<<<S, 64, 0, st>>>(w->specChunk, w->rzp, w->rrp, w->sc, w->active, w->iters);
         b_direction_kernel<T><<<gridn(N), BS, 0, st>>>(N, c->specOfNode, w->active, w->sc, z, p);
     }
     b_count_kernel<<<1, 1, 0, st>>>(S, w->active, w->nactive);
@@ -309,6 +337,36 @@ int ensure_batch(efe_ctx* c, int prec) {
         EFE_HIP(hipMalloc(&w->nout, (size_t)c->nspec * 4 * sizeof(double)));
         EFE_HIP(hipHostMalloc(&w->hbuf, sizeof(int) * 4, hipHostMallocDefault));
         EFE_HIP(hipStreamCreateWithFlags(&w->ks, hipStreamNonBlocking));
+        // 256-node chunks of every specimen, starting at its first node
+        std::vector<int64_t> off((size_t)c->nspec + 1);
+        EFE_HIP(hipMemcpyAsync(off.data(), c->specNodeOff, off.size() * sizeof(int64_t), hipMemcpyDeviceToHost,
+                               c->stream));
+        EFE_HIP(hipStreamSynchronize(c->stream));
+        std::vector<int32_t> n0, cs, sco(1, 0);
+        for (int sp = 0; sp < c->nspec; ++sp) {
+            for (int64_t n = off[sp]; n < off[sp + 1]; n += BS) {
+                n0.push_back((int32_t)n);
+                cs.push_back(sp);
+            }
+            sco.push_back((int32_t)n0.size());
+        }
+        w->C = (int)n0.size();
+        const size_t nc = n0.size() ? n0.size() : 1;
+        EFE_HIP(hipMalloc(&w->chunkNode0, nc * sizeof(int32_t)));
+        EFE_HIP(hipMalloc(&w->chunkSpec, nc * sizeof(int32_t)));
+        EFE_HIP(hipMalloc(&w->specChunk, sco.size() * sizeof(int32_t)));
+        EFE_HIP(hipMalloc(&w->pqp, nc * sizeof(double)));
+        EFE_HIP(hipMalloc(&w->rzp, nc * sizeof(double)));
+        EFE_HIP(hipMalloc(&w->rrp, nc * sizeof(double)));
+        if (w->C) {
+            EFE_HIP(hipMemcpyAsync(w->chunkNode0, n0.data(), n0.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                                   c->stream));
+            EFE_HIP(hipMemcpyAsync(w->chunkSpec, cs.data(), cs.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                                   c->stream));
+        }
+        EFE_HIP(hipMemcpyAsync(w->specChunk, sco.data(), sco.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                               c->stream));
+        EFE_HIP(hipStreamSynchronize(c->stream));
         EFE_HIP(hipEventCreateWithFlags(&w->evA, hipEventDisableTiming));
         EFE_HIP(hipEventCreateWithFlags(&w->evB, hipEventDisableTiming));
     }
@@ -512,8 +570,10 @@ int efe_launch_accept_batched(efe_ctx* c, const uint8_t* accept) {
 void efe_batched_free(efe_ctx* c) {
     BatchWork* w = c->bw;
     if (!w) return;
-    void* bufs[] = {w->x, w->r, w->z, w->p, w->q, w->nv0, w->nv1, w->sc, w->active, w->iters, w->nactive, w->acc,
-                    w->nout};
+    void* bufs[] = {w->x,      w->r,      w->z,      w->p,          w->q,         w->nv0,
+                    w->nv1,    w->sc,     w->active, w->iters,      w->nactive,   w->acc,
+                    w->nout,   w->pqp,    w->rzp,    w->rrp,        w->chunkNode0, w->chunkSpec,
+                    w->specChunk};
     for (void* p : bufs)
         if (p) hipFree(p);
     if (w->hbuf) hipHostFree(w->hbuf);

@@ -152,3 +152,32 @@ def test_material_cutbacks_and_step_failure_stay_per_specimen():
             assert rel_err(pm.U, om.U) < 1e-9
     assert not ok and solver.failedSpecimens == [2]
     assert np.inf in solver.history[0] and solver.history[0][-1] != np.inf
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_specimen_alone_and_in_a_batch_are_bitwise_equal(precision):
+    """The batched PCG sums each specimen's dot products over 256-node chunks that start at
+    the specimen's first node: a specimen of several chunks (8^3, 729 nodes), placed behind
+    others so that its nodes do not start on a chunk boundary of the batch, takes the same
+    Krylov iterations and ends with the same bits as when it is the only specimen."""
+    from edelweissfe_amd.solvers.nistbatched import NISTBatched
+
+    cases = [((2, 2, 2), 210000.0, 250.0, 0), ((8, 8, 8), 200000.0, 260.0, 7), ((3, 3, 3), 180000.0, 300.0, 2)]
+
+    def run(which):
+        models, steps = [], []
+        for n, E, fy, seed in [cases[i] for i in which]:
+            pm = device_model(n, (1.0, 1.0, 1.0), "C3D8", "vonmises", _props(E, fy), jitter=0.1, seed=seed)
+            models.append(pm)
+            steps.append(device_step(pm, uniaxial_bcs(0.01), maxInc=0.25, minInc=1e-6, maxNumInc=100, maxIter=15))
+        solver = NISTBatched(precision=precision, autotune=False)
+        solver.solveSteps(models, steps)
+        return solver, models
+
+    alone, ma = run([1])
+    batch, mb = run([0, 1, 2])
+    print(alone.history[0], batch.history[1], alone.specimenKrylov[0], batch.specimenKrylov[1])
+    assert alone.history[0] == batch.history[1]
+    assert alone.specimenKrylov[0] == batch.specimenKrylov[1] > 0
+    assert np.array_equal(ma[0].U, mb[1].U)
+    assert np.array_equal(alone.qpStates(ma)[0], batch.qpStates(mb)[1])
