@@ -479,7 +479,7 @@ def main():
     ap.add_argument("--batch-increments", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+        print("bench.py measures one GPU only: the library runs the Newton step on one GPU (run with --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "batched":
