@@ -15,6 +15,7 @@ EFE_OK = 0
 EFE_CUTBACK = 1
 EFE_NAN = 2
 EFE_NOT_CONVERGED = 3
+EFE_ERR_ARG, EFE_ERR_HIP, EFE_ERR_STATE = -1, -2, -3
 
 EFE_C3D8, EFE_C3D20, EFE_C3D20R = 0, 1, 2
 EFE_LINEAR_ELASTIC, EFE_VONMISES, EFE_DRUCKER_PRAGER = 0, 1, 2

@@ -78,6 +78,14 @@ def test_header_declarations_match_binding_table():
     assert _header_symbols() == sorted(_lib.SIGNATURES)
 
 
+def test_header_constants_match_binding_constants():
+    text = open(os.path.join(ROOT, "include", "efe_gpu.h")).read()
+    consts = dict(re.findall(r"^#define\s+(EFE_\w+)\s+(-?\d+)", text, re.M))
+    assert len(consts) >= 20
+    for name, value in consts.items():
+        assert getattr(_lib, name) == int(value), name
+
+
 def test_library_loads_and_exports_every_declared_symbol():
     lib = _lib.load()
     for name in _header_symbols():
