@@ -127,6 +127,19 @@ class NISTBatched:
         allDir = np.unique(np.concatenate([ix for _, _, ix in owners])) if owners else np.zeros(0, np.int64)
         sysm.set_dirichlet(allDir)
         dirT = torch.as_tensor(allDir, dtype=torch.int64, device=dev)
+        # the prescribed increments as a device table: DOF, value over the step and owning
+        # action of every prescribed DOF, the last action naming a DOF winning; per round only
+        # the actions' amplitude factors travel to the device
+        if owners:
+            ix = np.concatenate([o[2] for o in owners])
+            vals = np.concatenate([np.tile(d.values, len(i) // len(d.values)) for _, d, i in owners])
+            own = np.concatenate([np.full(len(o[2]), k, dtype=np.int64) for k, o in enumerate(owners)])
+            _, firstInRev = np.unique(ix[::-1], return_index=True)
+            pick = np.sort(ix.size - 1 - firstInRev)
+            self._dirTable = (torch.as_tensor(ix[pick], device=dev), torch.as_tensor(vals[pick], device=dev),
+                              torch.as_tensor(own[pick], device=dev))
+        else:
+            self._dirTable = None
         isDir = torch.zeros(D, dtype=torch.bool, device=dev)
         isDir[dirT] = True
         specOfDof = torch.as_tensor(np.repeat(np.arange(S), np.diff(self.dofOffset)), device=dev)
@@ -216,11 +229,11 @@ class NISTBatched:
             if self.options["extrapolation"] == "linear" and p is not None and p.timeIncrement:
                 scale[s] = ts[s].stepProgressIncrement / p.stepProgressIncrement
                 extrap[s] = True
-        delta = np.zeros(D)
-        for s, d, ix in owners:
-            if live[s]:
-                delta[ix] = d.getDelta(models[s], ts[s])
-        deltaT = torch.as_tensor(delta, device=dev)
+        deltaT = torch.zeros(D, dtype=torch.float64, device=dev)
+        if self._dirTable is not None:
+            factor = np.array([d.incrementFactor(ts[s]) if live[s] else 0.0 for s, d, _ in owners])
+            ix, vals, own = self._dirTable
+            deltaT[ix] = vals * torch.as_tensor(factor, device=dev)[own]
         dU = dU * perDof(scale)
         dU = torch.where(perDof(extrap) & isDir, deltaT, dU)
         PExt = None

@@ -38,13 +38,17 @@ class Dirichlet:
         nodes = np.asarray(model.nodeSets[self.nSet])
         return (3 * nodes[:, None] + np.asarray(self.components)[None, :]).ravel()
 
+    def incrementFactor(self, timeStep):
+        """amp(p) - amp(p - dp) of this increment, 0 once the action is inactive: the
+        increment of every prescribed value is value * incrementFactor."""
+        if not self.active:
+            return 0.0
+        a = self.amplitude
+        return a(timeStep.stepProgress) - a(timeStep.stepProgress - timeStep.stepProgressIncrement)
+
     def getDelta(self, model, timeStep):
         n = len(model.nodeSets[self.nSet])
-        delta = np.tile(self.values, (n, 1))
-        if not self.active:
-            return (delta * 0.0).ravel()
-        a = self.amplitude
-        return (delta * (a(timeStep.stepProgress) - a(timeStep.stepProgress - timeStep.stepProgressIncrement))).ravel()
+        return (np.tile(self.values, (n, 1)) * self.incrementFactor(timeStep)).ravel()
 
     def applyAtStepEnd(self, model):
         self.active = False
