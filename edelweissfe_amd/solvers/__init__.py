@@ -1,1 +1,2 @@
 from .nistgpu import NISTGPU  # noqa: F401
+from .nistbatched import NISTBatched, NISTBatchedGroups  # noqa: F401

@@ -326,3 +326,59 @@ class NISTBatched:
         [n_elements][n_qp][n_state] arrays."""
         st = self.system.qp_state(which)
         return [st[self.elementOffset[s]:self.elementOffset[s + 1]] for s in range(len(models))]
+
+
+class NISTBatchedGroups:
+    """Specimens of several element and material types, grouped by (element type, material):
+    every group is one NISTBatched, i.e. one block-diagonal context with one element launch
+    per Newton iteration and one batched PCG, and the groups run one after the other on the
+    GPU.  Each specimen is solved exactly as in a batch of its own group; per-specimen results
+    (history, failures, work counts, failedSpecimens) come back in the order of the models."""
+
+    identification = "NISTBatchedGroupsSolver"
+
+    def __init__(self, **kwargs):
+        NISTBatched(**kwargs)  # rejects unknown options now rather than at the first group
+        self.kwargs = kwargs
+        self.groups = []  # (key, specimen indices, NISTBatched)
+
+    @staticmethod
+    def groupKey(model):
+        return model.elementType.upper(), model.material["name"].lower()
+
+    def solveSteps(self, models, steps):
+        """As NISTBatched.solveSteps, for specimens of any element types and materials."""
+        S = len(models)
+        if len(steps) != S:
+            raise ValueError("one step per model")
+        keys = [self.groupKey(m) for m in models]
+        self.groups = []
+        self.history, self.failures = [None] * S, [None] * S
+        self.specimenPasses = np.zeros(S, dtype=np.int64)
+        self.specimenSolves = np.zeros(S, dtype=np.int64)
+        self.specimenKrylov = np.zeros(S, dtype=np.int64)
+        self.failedSpecimens = []
+        allOk = True
+        for key in dict.fromkeys(keys):  # groups in order of first appearance
+            idx = [i for i, k in enumerate(keys) if k == key]
+            solver = NISTBatched(**self.kwargs)
+            ok, _ = solver.solveSteps([models[i] for i in idx], [steps[i] for i in idx])
+            allOk = allOk and ok
+            for j, i in enumerate(idx):
+                self.history[i] = solver.history[j]
+                self.failures[i] = solver.failures[j]
+                self.specimenPasses[i] = solver.specimenPasses[j]
+                self.specimenSolves[i] = solver.specimenSolves[j]
+                self.specimenKrylov[i] = solver.specimenKrylov[j]
+            self.failedSpecimens += [idx[j] for j in solver.failedSpecimens]
+            self.groups.append((key, idx, solver))
+        self.failedSpecimens.sort()
+        return allOk, models
+
+    def qpStates(self, models, which=0):
+        """QP state of every specimen, in the order of the models."""
+        out = [None] * len(models)
+        for _, idx, solver in self.groups:
+            for j, st in enumerate(solver.qpStates([models[i] for i in idx], which)):
+                out[idx[j]] = st
+        return out

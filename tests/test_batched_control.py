@@ -7,8 +7,8 @@ import numpy as np
 import pytest
 
 from batch_mock import attach_mock
-from helpers import (CUTBACK_SPECIMENS, cutback_oracle_run, device_model, device_step, oracle_model, rel_err,
-                     uniaxial_bcs)
+from helpers import (CUTBACK_SPECIMENS, cutback_oracle_run, device_model, device_step, oracle_model, oracle_step,
+                     rel_err, uniaxial_bcs)
 
 
 def _run(specimens):
@@ -48,3 +48,45 @@ def test_specimen_alone_matches_specimen_in_batch():
     batch, _, m3, _ = _run(CUTBACK_SPECIMENS)
     assert alone.history[0] == batch.history[0]
     np.testing.assert_array_equal(m1[0].U, m3[0].U)
+
+
+GROUPED = [  # element type, material, props, mesh, applied strain, seed
+    ("C3D8", "vonmises", [210000.0, 0.3, 250.0, 1000.0, 0.0, 0.0], (2, 2, 2), 0.01, 0),
+    ("C3D8", "linearelastic", [210000.0, 0.3], (2, 3, 2), 0.01, 1),
+    ("C3D20R", "vonmises", [180000.0, 0.3, 300.0, 1000.0, 0.0, 0.0], (1, 2, 1), 0.015, 2),
+    ("C3D8", "vonmises", [150000.0, 0.3, 200.0, 1000.0, 0.0, 0.0], (3, 2, 2), 0.01, 3),
+    ("C3D8", "linearelastic", [100000.0, 0.25], (2, 2, 2), 0.005, 4),
+]
+
+
+def test_groups_by_element_and_material_follow_each_oracle_run(monkeypatch):
+    """NISTBatchedGroups on five specimens of three (element type, material) groups,
+    interleaved: one batch per group, and every specimen follows its own oracle NIST run."""
+    from edelweissfe_amd.solvers.nistbatched import NISTBatched, NISTBatchedGroups
+    from oracle import nist as onist
+
+    models, steps, oms = [], [], {}
+    for et, mat, props, n, strain, seed in GROUPED:
+        pm = device_model(n, (1.0, 1.0, 1.0), et, mat, props, jitter=0.1, seed=seed)
+        models.append(pm)
+        steps.append(device_step(pm, uniaxial_bcs(strain), maxInc=0.25, minInc=1e-6, maxNumInc=100, maxIter=15))
+        oms[id(pm)] = oracle_model(n, (1.0, 1.0, 1.0), et, mat, props, jitter=0.1, seed=seed)
+
+    def build(self, group_models):  # the oracle stand-in of the group's device system
+        return attach_mock(self, group_models, [oms[id(m)] for m in group_models])
+
+    monkeypatch.setattr(NISTBatched, "build", build)
+    solver = NISTBatchedGroups(precision="fp64", autotune=False)
+    ok, _ = solver.solveSteps(models, steps)
+    assert ok and [g[1] for g in solver.groups] == [[0, 3], [1, 4], [2]]
+    states = solver.qpStates(models)
+    for s, (et, mat, props, n, strain, seed) in enumerate(GROUPED):
+        om = oracle_model(n, (1.0, 1.0, 1.0), et, mat, props, jitter=0.1, seed=seed)
+        so = onist.NIST()
+        so.solve_step(om, oracle_step(uniaxial_bcs(strain), maxInc=0.25, minInc=1e-6, maxNumInc=100, maxIter=15))
+        want = [np.inf if h[2] is None else h[2] for h in so.history]
+        assert solver.history[s] == want, (s, solver.history[s], want)
+        assert rel_err(models[s].U, om.U) < 1e-12
+        assert rel_err(states[s][..., 0:6], om.state[..., 0:6]) < 1e-12
+    with pytest.raises(AttributeError):
+        NISTBatchedGroups(notAnOption=1)
