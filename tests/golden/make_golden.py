@@ -25,6 +25,7 @@ UREF_TESTS = [
     "WallShearHexa20",
     "CantileverBeamHexa8",
     "CantileverBeamVonMisesPE",  # plane strain CPE8, von Mises with nonlinear hardening
+    "WallShearHexa8GMRES",  # C3D8N (an alias of C3D8, elements/library.py:541) solved by GMRES
 ]
 
 

@@ -213,6 +213,23 @@ def test_nist_reference_golden_vectors(golden, name, n, l, et, E, nu, bcs, nfs, 
     assert [r["iters"] for r in s.history] == [1, 2]
 
 
+@pytest.mark.parametrize("precond", ["ilu0", "bjacobi"])
+def test_nist_reference_golden_vector_gmres(golden, precond):
+    """testfiles/WallShearHexa8GMRES: the 10 x 10 x 1 wall of C3D8N (the C3D8 element,
+    elements/library.py:541) that the reference solves with GMRES (linsolver=gmres), here with
+    the device GMRES(50) and block ILU(0) or block Jacobi, FP64."""
+    pm = device_model((10, 10, 1), (1000, 1000, 5), "C3D8", "linearelastic", [2.1e4, 0.22])
+    s = _run_device(pm, dict(maxInc=1.0, minInc=1e-9, maxNumInc=1000, maxIter=25),
+                    [("bottom", "gen_bottom", {0: 0.0, 1: 0.0, 2: 0.0})], nodeforces=[("gen_top", (1.0, 0, 0))],
+                    bodyforces=[("all", (0.0, -0.000077, 0.0))], extrapolation="off", linsolver="gmres",
+                    preconditioner=precond)
+    ref = golden["uref_WallShearHexa8GMRES"]
+    print(precond, s.history)
+    assert np.abs(pm.U - ref).max() < 1e-6
+    assert rel_err(pm.U, ref) < 1e-8  # U.ref holds the reference's GMRES error (2e-9 relative)
+    assert [r["iters"] for r in s.history] == [1, 2]
+
+
 def test_nist_j2_load_history_matches_oracle():
     n, l = (5, 5, 5), (1.0, 1.0, 1.0)
     om = oracle_model(n, l, "C3D8", "vonmises", J2, jitter=0.1, seed=0)

@@ -32,6 +32,12 @@ CASES = {
                            dirichlets=[dict(nset="gen_bottom", values={0: 0.0, 1: 0.0, 2: 0.0})],
                            nodeforces=[dict(nset="gen_top", vector=(1.0, 0, 0))],
                            bodyforces=[dict(elset="all", vector=(0.0, -0.000077, 0))]),
+    # testfiles/WallShearHexa8GMRES/test.inp: 10 x 10 x 1 C3D8N (elements/library.py:541: the
+    # C3D8 element) solved with GMRES (linsolver=gmres, opt.json)
+    "WallShearHexa8GMRES": dict(n=(10, 10, 1), l=(1000, 1000, 5), npe=8, el_type="C3D8", E=2.1e4, nu=0.22,
+                                dirichlets=[dict(nset="gen_bottom", values={0: 0.0, 1: 0.0, 2: 0.0})],
+                                nodeforces=[dict(nset="gen_top", vector=(1.0, 0, 0))],
+                                bodyforces=[dict(elset="all", vector=(0.0, -0.000077, 0))]),
     # testfiles/WallShearHexa20/test.inp (C3D20, 27 QP)
     "WallShearHexa20": dict(n=(10, 10, 1), l=(1000, 1000, 5), npe=20, el_type="C3D20", E=2.1e4, nu=0.22,
                             dirichlets=[dict(nset="gen_bottom", values={0: 0.0, 1: 0.0, 2: 0.0})],
@@ -40,13 +46,19 @@ CASES = {
 }
 
 
+# relative agreement beyond the reference's own 1e-6 absolute criterion; the GMRES case's
+# U.ref carries the error of the reference's iterative solve (pyamg GMRES at its default
+# tolerance): 2e-9 relative, 5e-12 absolute, against the direct solves here
+RTOL = {"WallShearHexa8GMRES": 1e-8}
+
+
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_oracle_reproduces_reference_u_ref(golden, name):
     U, s = _box(**CASES[name])
     ref = golden["uref_" + name]
     assert U.shape == ref.shape
     assert np.abs(U - ref).max() < 1e-6
-    assert np.abs(U - ref).max() <= 1e-9 * np.abs(ref).max()
+    assert np.abs(U - ref).max() <= RTOL.get(name, 1e-9) * np.abs(ref).max()
     # zero increment (1 solve) then the load increment (solve + converged check)
     assert [h[2] for h in s.history] == [1, 2]
 
