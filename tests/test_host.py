@@ -104,3 +104,35 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "absent.so"))
     with pytest.raises(_lib.LibraryMissing):
         _lib.load()
+
+
+def test_mesh_descriptor_checks_need_no_gpu():
+    """efe_ctx_create rejects a bad descriptor before any device call: empty mesh, unknown
+    element type, property count, sizes beyond 32-bit indices, connectivity outside the nodes."""
+    lib = _lib.load()
+    X = np.zeros((8, 3))
+    conn = np.arange(8, dtype=np.int32).reshape(1, 8)
+    props = np.array([1.0, 0.3])
+    keep = (X, conn, props)
+
+    def create(**kw):
+        d = dict(n_nodes=8, n_elements=1, element_type=_lib.EFE_C3D8, material=_lib.EFE_LINEAR_ELASTIC,
+                 coords=X.ctypes.data, conn=conn.ctypes.data, n_props=2, props_per_element=0, props=props.ctypes.data,
+                 specimen_of_node=None, n_specimens=1, stream=None)
+        d.update(kw)
+        ctx = ctypes.c_void_p()
+        rc = lib.efe_ctx_create(ctypes.byref(_lib.MeshDesc(**d)), ctypes.byref(ctx))
+        return rc, lib.efe_last_error().decode()
+
+    assert create(n_nodes=0) == (_lib.EFE_ERR_ARG, "invalid mesh descriptor")
+    assert create(coords=None)[0] == _lib.EFE_ERR_ARG
+    assert create(element_type=7) == (_lib.EFE_ERR_ARG, "unknown element type")
+    rc, msg = create(material=_lib.EFE_VONMISES)
+    assert rc == _lib.EFE_ERR_ARG and "n_props" in msg
+    rc, msg = create(n_nodes=1 << 30)  # checked before the arrays are read
+    assert rc == _lib.EFE_ERR_ARG and "32-bit" in msg
+    bad = conn.copy()
+    bad[0, 3] = 8
+    rc, msg = create(conn=bad.ctypes.data)
+    assert rc == _lib.EFE_ERR_ARG and "outside [0, n_nodes)" in msg
+    del keep
