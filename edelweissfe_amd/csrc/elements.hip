@@ -35,7 +35,8 @@ struct MatState {
 
 // MF (matrix-free form, KT = float): phase 1 also stores the compact tangent
 // w detJ [A, B, cnn, (cn1, c1n,) n] per QP for the matrix-free operator, and phase 2 forms
-// only the diagonal blocks K_aa (for block Jacobi).
+// only the diagonal blocks K_aa (for block Jacobi), stored element-major [e][a][9] at the
+// start of the element scratch.
 template <int NN, int Q, int MAT, typename KT, int EPW, bool MF>
 __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double* __restrict__ coords,
                                                      const int32_t* __restrict__ conn,
@@ -193,6 +194,10 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
                 acc[3 + j] += ay * CB[1][j] + ax * CB[3][j] + az * CB[5][j];
                 acc[6 + j] += az * CB[2][j] + ax * CB[4][j] + ay * CB[5][j];
             }
+        }
+        if (MF) {  // K_aa element-major at the start of the scratch: [e][a][9]
+            store_block(Kscr + ((int64_t)e * NN + a) * kBlockPitch, acc);
+            continue;
         }
         const int32_t* cp = cpos + (int64_t)e * NN * NN;
         store_block(Kscr + (int64_t)cp[a * NN + b] * kBlockPitch, acc);

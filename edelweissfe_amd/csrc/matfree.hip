@@ -459,24 +459,20 @@ __global__ __launch_bounds__(BS) void mf_gather_kernel(int N, const int32_t* __r
 // Diagonal 3x3 blocks (element order), symmetric Dirichlet elimination inside the block,
 // block-Jacobi inverse, and the P / F gather of the element residuals.
 __global__ __launch_bounds__(256) void mf_diag_kernel(
-    int N, const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol, const int32_t* __restrict__ coff,
-    const int32_t* __restrict__ nptr, const int32_t* __restrict__ nlist,
-    const float* __restrict__ Kscr, const double* __restrict__ Pscr, const uint8_t* __restrict__ dirflag, int ndir,
-    float* __restrict__ Minv, double* __restrict__ P, double* __restrict__ F, const int32_t* __restrict__ nodeSlot,
-    const int32_t* __restrict__ cslice) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    int N, const int32_t* __restrict__ nptr, const int32_t* __restrict__ nlist, const float* __restrict__ Kscr,
+    const double* __restrict__ Pscr, const uint8_t* __restrict__ dirflag, int ndir, float* __restrict__ Minv,
+    double* __restrict__ P, double* __restrict__ F) {
+    // XCD-aware numbering: each XCD sums a contiguous node range, so the element blocks that
+    // neighbouring node rows share are fetched into one L2, not into several
+    const int n = xcd_block_mf() * blockDim.x + threadIdx.x;
     if (n >= N) return;
-    // element scratch position of this row's contribution c (the assembly's sliced layout)
-    const int ps = nodeSlot[n], sl = ps / kSlice, l = ps - (ps / kSlice) * kSlice;
-    const int64_t cbase = (int64_t)cslice[sl] * kSlice + l - (int64_t)coff[bptr[n]] * kSlice;
-    double dg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int b = bptr[n]; b < bptr[n + 1]; ++b) {
-        if (bcol[b] != n) continue;
-        float acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int c = coff[b]; c < coff[b + 1]; ++c) add_block(Kscr + (cbase + (int64_t)c * kSlice) * kBlockPitch, acc);
+    // K_nn = sum of the K_aa blocks [e][a][9] of the (element, local node) pairs of node n,
+    // in element order (the order of the diagonal block's contributions in the assembly)
+    float acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = nptr[n]; c < nptr[n + 1]; ++c) add_block(Kscr + (int64_t)nlist[c] * kBlockPitch, acc);
+    double dg[9];
 #pragma unroll
-        for (int v = 0; v < 9; ++v) dg[v] = acc[v];
-    }
+    for (int v = 0; v < 9; ++v) dg[v] = acc[v];
     if (ndir) {
         const bool fr[3] = {(bool)dirflag[3 * n], (bool)dirflag[3 * n + 1], (bool)dirflag[3 * n + 2]};
 #pragma unroll
@@ -571,9 +567,8 @@ int efe_mf_alloc(efe_ctx* c) {
 // Diagonal blocks, Minv, P, F and (optionally) the right-hand-side correction K[:, dir] dir_values.
 int efe_launch_mf_assembly(efe_ctx* c, double* P, double* F, const double* dir_values, double* dir_correction) {
     const int grid = (c->N + 255) / 256;
-    mf_diag_kernel<<<grid, 256, 0, c->stream>>>(c->N, c->bptr, c->bcol, c->coff, c->nptr, c->nlist,
-                                                (const float*)c->Kscr, c->Pscr, c->dirflag, c->ndir, c->Minv32, P, F,
-                                                c->nodeSlot, c->cslice);
+    mf_diag_kernel<<<round8_mf(grid), 256, 0, c->stream>>>(c->N, c->nptr, c->nlist, (const float*)c->Kscr, c->Pscr,
+                                                           c->dirflag, c->ndir, c->Minv32, P, F);
     EFE_HIP(hipGetLastError());
     if (dir_values) {
         launch_apply<double>(c, c->stream, dir_values, 0, nullptr);
