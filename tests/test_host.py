@@ -136,3 +136,13 @@ def test_mesh_descriptor_checks_need_no_gpu():
     rc, msg = create(conn=bad.ctypes.data)
     assert rc == _lib.EFE_ERR_ARG and "outside [0, n_nodes)" in msg
     del keep
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr
