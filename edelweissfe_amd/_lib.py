@@ -95,6 +95,7 @@ SIGNATURES = {
     "efe_accept_state": (_I, [_P]),
     "efe_get_qp_state": (_I, [_P, _I, _P]),
     "efe_set_qp_state": (_I, [_P, _P]),
+    "efe_set_active_specimens": (_I, [_P, _P]),
     "efe_set_launch": (_I, [_P, _I, _I]),
     "efe_get_phase_times": (_I, [_P, ctypes.POINTER(_D)]),
 }

@@ -220,6 +220,7 @@ int efe_ctx_create(const efe_mesh_desc* d, efe_ctx** out) {
             if ((rc = upload(&c->specOfElem, se.data(), se.size(), st))) return fail(rc);
             EFE_HIP(hipMalloc(&c->specStatus, (size_t)c->nspec * sizeof(int)));
             EFE_HIP(hipMemsetAsync(c->specStatus, 0, (size_t)c->nspec * sizeof(int), st));
+            EFE_HIP(hipMalloc(&c->specActive, (size_t)c->nspec * sizeof(int)));
         }
     }
     if ((rc = upload(&c->bptr, hp.bptr.data(), hp.bptr.size(), st))) return fail(rc);
@@ -272,7 +273,7 @@ void efe_ctx_destroy(efe_ctx* c) {
                     c->coords, c->conn,  c->props,  c->dNdxi,  c->shapeN, c->qpw,    c->stateRef, c->stateTmp,
                     c->Kscr,   c->Pscr,  c->bptr,   c->bcol,   c->coff,   c->cpos,   c->cslice, c->nptr,     c->nlist,
                     c->diagk,  c->slice_off, c->scol, c->K64,  c->K32,    c->Minv64, c->Minv32,   c->dirflag,
-                    c->slotNode, c->nodeSlot, c->specStatus, c->elemPos,
+                    c->slotNode, c->nodeSlot, c->specStatus, c->specActive, c->elemPos,
                     c->status, c->red,   c->specOfNode, c->specNodeOff, c->specOfElem};
     for (void* p : bufs)
         if (p) hipFree(p);
@@ -542,6 +543,24 @@ int efe_get_specimen_status(efe_ctx* c, uint8_t* out) {
     EFE_HIP(hipMemcpyAsync(h.data(), c->specStatus, h.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     EFE_HIP(hipStreamSynchronize(c->stream));
     for (int s = 0; s < c->nspec; ++s) out[s] = h[s] ? 1 : 0;
+    return EFE_OK;
+}
+
+int efe_set_active_specimens(efe_ctx* c, const uint8_t* active) {
+    if (int rc = check(c)) return rc;
+    if (!c->specActive) {
+        efe_set_error("efe_set_active_specimens needs a context created with specimens");
+        return EFE_ERR_STATE;
+    }
+    if (!active) {
+        c->useActive = false;
+        return EFE_OK;
+    }
+    std::vector<int> a(c->nspec);
+    for (int s = 0; s < c->nspec; ++s) a[s] = active[s] ? 1 : 0;
+    EFE_HIP(hipMemcpyAsync(c->specActive, a.data(), a.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    EFE_HIP(hipStreamSynchronize(c->stream));  // a must outlive the copy
+    c->useActive = true;
     return EFE_OK;
 }
 

@@ -139,6 +139,8 @@ struct efe_ctx {
     int ndir = 0;
     int* status = nullptr;       // device status word (cutback / NaN)
     int* specStatus = nullptr;   // [nspec] cutback request per specimen (batched contexts)
+    int* specActive = nullptr;   // [nspec] element passes skip specimens with 0 (batched contexts)
+    bool useActive = false;      // whether the element pass reads specActive
     double* red = nullptr;       // reduction scratch
     int redn = 0;
 

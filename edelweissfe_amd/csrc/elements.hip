@@ -49,8 +49,11 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
                                                      int* __restrict__ status, float* __restrict__ mfC,
                                                      int* __restrict__ specStatus,
                                                      const int32_t* __restrict__ specOfElem,
-                                                     const int32_t* __restrict__ cpos) {
+                                                     const int32_t* __restrict__ cpos,
+                                                     const int* __restrict__ specActive) {
     constexpr int NS = 12 + MatState<MAT>::n;
+    // elements past the end and, in batched contexts, of specimens the pass skips
+    auto skip = [&](int e) { return e >= E || (specActive && !specActive[specOfElem[e]]); };
     constexpr int NP = (MAT == EFE_LINEAR_ELASTIC) ? 2 : 6;
     constexpr int NC = (MAT == EFE_DRUCKER_PRAGER) ? 11 : 9;
     __shared__ double g_s[EPW][Q][NN][3];
@@ -63,7 +66,7 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
     for (int t = lane; t < EPW * Q; t += 64) {
         const int el = t / Q, q = t - (t / Q) * Q;
         const int e = e0 + el;
-        if (e >= E) continue;
+        if (skip(e)) continue;
         const double* dn = dNdxi + (size_t)q * NN * 3;
         double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll 4
@@ -173,7 +176,7 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
             b = r0 - a * NN;
         }
         const int e = e0 + el;
-        if (e >= E) continue;
+        if (skip(e)) continue;
         KT acc[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) acc[k] = (KT)0;
@@ -210,7 +213,7 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
     for (int t = lane; t < EPW * NN; t += 64) {
         const int el = t / NN, a = t - (t / NN) * NN;
         const int e = e0 + el;
-        if (e >= E) continue;
+        if (skip(e)) continue;
         double p0 = 0, p1 = 0, p2 = 0;
         for (int q = 0; q < Q; ++q) {
             const double ax = g_s[el][q][a][0], ay = g_s[el][q][a][1], az = g_s[el][q][a][2];
@@ -248,7 +251,8 @@ __global__ __launch_bounds__(256) void assemble_kernel(
     const int32_t* __restrict__ slice_off, const KT* __restrict__ Kscr, const double* __restrict__ Pscr,
     const uint8_t* __restrict__ dirflag, int ndir, const double* __restrict__ dirv, double* __restrict__ dircorr,
     KT* __restrict__ Kval, KT* __restrict__ Minv, double* __restrict__ P, double* __restrict__ F,
-    const int32_t* __restrict__ slotNode, const int32_t* __restrict__ cslice) {
+    const int32_t* __restrict__ slotNode, const int32_t* __restrict__ cslice, const int* __restrict__ specActive,
+    const int32_t* __restrict__ specOfNode) {
     // Threads walk the row slots of the sliced layout: the value stores of a wavefront are
     // contiguous, and so are its reads of the element scratch, where contribution j of the
     // 64 rows of a slice is one run of 64 blocks.
@@ -257,6 +261,12 @@ __global__ __launch_bounds__(256) void assemble_kernel(
     const int ps = blk * blockDim.x + threadIdx.x;
     if (ps >= N) return;
     const int n = slotNode[ps];
+    if (specActive && !specActive[specOfNode[n]]) {
+        // a skipped specimen keeps its rows; its increment is not solved, so its right-hand
+        // side correction is zero
+        if (dirv) dircorr[3 * n] = dircorr[3 * n + 1] = dircorr[3 * n + 2] = 0.0;
+        return;
+    }
     const int s = ps / kSlice, l = ps - (ps / kSlice) * kSlice;
     const int64_t vbase = (int64_t)slice_off[s] * 9 + l;
     bool fr[3] = {false, false, false};
@@ -514,14 +524,15 @@ template <int NN, int Q, int MAT, typename KT, int EPW>
 void launch_elem_t(efe_ctx* c, const double* dU) {
     const int grid = (c->E + EPW - 1) / EPW;
     const int propStride = c->props_per_element ? c->E : 0;
+    const int* act = c->useActive ? c->specActive : nullptr;
     if (c->mfPass)
         element_kernel<NN, Q, MAT, float, EPW, true><<<grid, 64, 0, c->stream>>>(
             c->E, c->N, c->coords, c->conn, c->props, propStride, c->dNdxi, c->qpw, dU, c->stateRef, c->stateTmp,
-            (float*)c->Kscr, c->Pscr, c->status, c->mfC, c->specStatus, c->specOfElem, c->cpos);
+            (float*)c->Kscr, c->Pscr, c->status, c->mfC, c->specStatus, c->specOfElem, c->cpos, act);
     else
         element_kernel<NN, Q, MAT, KT, EPW, false><<<grid, 64, 0, c->stream>>>(
             c->E, c->N, c->coords, c->conn, c->props, propStride, c->dNdxi, c->qpw, dU, c->stateRef, c->stateTmp,
-            (KT*)c->Kscr, c->Pscr, c->status, nullptr, c->specStatus, c->specOfElem, c->cpos);
+            (KT*)c->Kscr, c->Pscr, c->status, nullptr, c->specStatus, c->specOfElem, c->cpos, act);
 }
 
 template <int NN, int Q, int MAT, typename KT>
@@ -584,11 +595,13 @@ int efe_launch_assembly(efe_ctx* c, double* P, double* F, const double* dir_valu
     if (precision == EFE_FP32) {
         assemble_kernel<float><<<grid, bs, 0, c->stream>>>(
             c->N, c->bptr, c->bcol, c->coff, c->nptr, c->nlist, c->slice_off, (const float*)c->Kscr,
-            c->Pscr, c->dirflag, c->ndir, dir_values, dir_correction, c->K32, c->Minv32, P, F, c->slotNode, c->cslice);
+            c->Pscr, c->dirflag, c->ndir, dir_values, dir_correction, c->K32, c->Minv32, P, F, c->slotNode, c->cslice,
+            c->useActive ? c->specActive : nullptr, c->specOfNode);
     } else {
         assemble_kernel<double><<<grid, bs, 0, c->stream>>>(
             c->N, c->bptr, c->bcol, c->coff, c->nptr, c->nlist, c->slice_off, (const double*)c->Kscr,
-            c->Pscr, c->dirflag, c->ndir, dir_values, dir_correction, c->K64, c->Minv64, P, F, c->slotNode, c->cslice);
+            c->Pscr, c->dirflag, c->ndir, dir_values, dir_correction, c->K64, c->Minv64, P, F, c->slotNode, c->cslice,
+            c->useActive ? c->specActive : nullptr, c->specOfNode);
     }
     EFE_HIP(hipGetLastError());
     c->Kprec = precision;

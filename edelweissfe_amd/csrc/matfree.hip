@@ -461,11 +461,13 @@ __global__ __launch_bounds__(BS) void mf_gather_kernel(int N, const int32_t* __r
 __global__ __launch_bounds__(256) void mf_diag_kernel(
     int N, const int32_t* __restrict__ nptr, const int32_t* __restrict__ nlist, const float* __restrict__ Kscr,
     const double* __restrict__ Pscr, const uint8_t* __restrict__ dirflag, int ndir, float* __restrict__ Minv,
-    double* __restrict__ P, double* __restrict__ F) {
+    double* __restrict__ P, double* __restrict__ F, const int* __restrict__ specActive,
+    const int32_t* __restrict__ specOfNode) {
     // XCD-aware numbering: each XCD sums a contiguous node range, so the element blocks that
     // neighbouring node rows share are fetched into one L2, not into several
     const int n = xcd_block_mf() * blockDim.x + threadIdx.x;
     if (n >= N) return;
+    if (specActive && !specActive[specOfNode[n]]) return;  // a skipped specimen keeps its rows
     // K_nn = sum of the K_aa blocks [e][a][9] of the (element, local node) pairs of node n,
     // in element order (the order of the diagonal block's contributions in the assembly)
     float acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -568,7 +570,8 @@ int efe_mf_alloc(efe_ctx* c) {
 int efe_launch_mf_assembly(efe_ctx* c, double* P, double* F, const double* dir_values, double* dir_correction) {
     const int grid = (c->N + 255) / 256;
     mf_diag_kernel<<<round8_mf(grid), 256, 0, c->stream>>>(c->N, c->nptr, c->nlist, (const float*)c->Kscr, c->Pscr,
-                                                           c->dirflag, c->ndir, c->Minv32, P, F);
+                                                           c->dirflag, c->ndir, c->Minv32, P, F,
+                                                           c->useActive ? c->specActive : nullptr, c->specOfNode);
     EFE_HIP(hipGetLastError());
     if (dir_values) {
         launch_apply<double>(c, c->stream, dir_values, 0, nullptr);

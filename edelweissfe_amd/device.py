@@ -186,6 +186,14 @@ class DeviceSystem:
         check(self.lib.efe_get_specimen_status(self.ctx, out.ctypes.data), "efe_get_specimen_status")
         return out.astype(bool)
 
+    def set_active_specimens(self, active):
+        """Specimens the following element passes cover (None: all); see efe_set_active_specimens."""
+        if active is None:
+            check(self.lib.efe_set_active_specimens(self.ctx, None), "efe_set_active_specimens")
+            return
+        a = np.ascontiguousarray(active, dtype=np.uint8)
+        check(self.lib.efe_set_active_specimens(self.ctx, a.ctypes.data), "efe_set_active_specimens")
+
     def accept_state_batched(self, accept):
         a = np.ascontiguousarray(accept, dtype=np.uint8)
         check(self.lib.efe_accept_state_batched(self.ctx, a.ctypes.data), "efe_accept_state_batched")

@@ -11,7 +11,7 @@ still iterating are one batched PCG with per-specimen scalars and tolerances.
 
 Increments advance in lockstep rounds: in every round each unfinished specimen takes the
 next increment of its own time stepper.  A specimen that has converged idles (its
-correction is zero, its temporary state is recomputed unchanged) until the slowest
+correction is zero; the element passes skip it and keep its last state) until the slowest
 specimen of the round has converged or failed; a failed specimen cuts back by its step's
 cutback factor and retries in the next round.
 
@@ -164,46 +164,49 @@ class NISTBatched:
         self.specimenPasses = np.zeros(S, dtype=np.int64)
         self.specimenSolves = np.zeros(S, dtype=np.int64)
         self.specimenKrylov = np.zeros(S, dtype=np.int64)
-        while True:
-            ts = [None] * S
-            for s in range(S):
-                if not finished[s]:
-                    try:
-                        ts[s] = next(gens[s], None)
-                    except (ReachedMaxIncrements, ReachedMinIncrementSize):
-                        ts[s] = None
-                        self.failedSpecimens.append(s)
-                    finished[s] = ts[s] is None
-            live = ~finished
-            if not live.any():
-                break
-            self.rounds += 1
-            dU, PExt, deltaT, extrap = self._startIncrements(models, steps, ts, prevTS, live, owners, dU, isDir,
-                                                             perDof)
-            self._cutbackSize = np.zeros(S)
-            state, it = self._newtonRound(sysm, steps, live, extrap, hasDir, dU, P, F, PExt, deltaT, dirT,
-                                          perDof, nAcc, prec, rtol)
-            dU = self._lastDU
-            ok = state == CONVERGED
-            sysm.accept_state_batched(ok)
-            U = torch.where(perDof(ok), U + dU, U)
-            for s in np.flatnonzero(live):
-                if ok[s]:
-                    prevTS[s] = ts[s]
-                    if it[s] >= steps[s].criticalIter:
-                        steps[s].preventIncrementIncrease()
-                    models[s].time = ts[s].totalTime
-                    self.history[s].append(int(it[s]))
-                else:
-                    prevTS[s] = None
-                    try:
-                        steps[s].discardAndChangeIncrement(max(self._cutbackSize[s], steps[s].cutbackFactor))
-                    except (ReachedMaxIncrements, ReachedMinIncrementSize):
-                        finished[s] = True
-                        self.failedSpecimens.append(s)
-                        continue
-                    self.history[s].append(np.inf)
-            dU = torch.where(perDof(~ok), torch.zeros_like(dU), dU)
+        try:
+            while True:
+                ts = [None] * S
+                for s in range(S):
+                    if not finished[s]:
+                        try:
+                            ts[s] = next(gens[s], None)
+                        except (ReachedMaxIncrements, ReachedMinIncrementSize):
+                            ts[s] = None
+                            self.failedSpecimens.append(s)
+                        finished[s] = ts[s] is None
+                live = ~finished
+                if not live.any():
+                    break
+                self.rounds += 1
+                dU, PExt, deltaT, extrap = self._startIncrements(models, steps, ts, prevTS, live, owners, dU, isDir,
+                                                                 perDof)
+                self._cutbackSize = np.zeros(S)
+                state, it = self._newtonRound(sysm, steps, live, extrap, hasDir, dU, P, F, PExt, deltaT, dirT,
+                                              perDof, nAcc, prec, rtol)
+                dU = self._lastDU
+                ok = state == CONVERGED
+                sysm.accept_state_batched(ok)
+                U = torch.where(perDof(ok), U + dU, U)
+                for s in np.flatnonzero(live):
+                    if ok[s]:
+                        prevTS[s] = ts[s]
+                        if it[s] >= steps[s].criticalIter:
+                            steps[s].preventIncrementIncrease()
+                        models[s].time = ts[s].totalTime
+                        self.history[s].append(int(it[s]))
+                    else:
+                        prevTS[s] = None
+                        try:
+                            steps[s].discardAndChangeIncrement(max(self._cutbackSize[s], steps[s].cutbackFactor))
+                        except (ReachedMaxIncrements, ReachedMinIncrementSize):
+                            finished[s] = True
+                            self.failedSpecimens.append(s)
+                            continue
+                        self.history[s].append(np.inf)
+                dU = torch.where(perDof(~ok), torch.zeros_like(dU), dU)
+        finally:
+            sysm.set_active_specimens(None)  # later passes (autotuner, probes) see every specimen
         for s, st in enumerate(steps):
             for group in ("dirichlet", "nodeforces"):
                 for action in st.actions.get(group, {}).values():
@@ -266,6 +269,8 @@ class NISTBatched:
                 corr = torch.empty(D, dtype=torch.float64, device=dev)
             else:
                 dirVals = corr = None
+            # only the specimens still iterating are computed; the others keep their last pass
+            sysm.set_active_specimens(state == ITERATING)
             st, _ = sysm.compute_elements(dU, P, F, dir_values=dirVals, dir_correction=corr, precision=prec)
             self.elementPasses += 1
             self.specimenPasses += state == ITERATING

@@ -195,6 +195,14 @@ int efe_accept_state_batched(efe_ctx* ctx, const uint8_t* accept);
  * the CutbackRequest of vonmises.py:230-231 raised per specimen model. */
 int efe_get_specimen_status(efe_ctx* ctx, uint8_t* out);
 
+/* Batched contexts: the specimens the following element passes cover (host [n_specimens],
+ * 1 = computed; NULL = all).  A skipped specimen keeps the temporary QP state, element
+ * residuals, P, F, operator and block-Jacobi rows of its last computed pass, which a pass with
+ * an unchanged increment would reproduce bit for bit: the lockstep driver skips the specimens
+ * that have converged, failed or idle in a round.  No counterpart in the reference, which
+ * solves each model on its own. */
+int efe_set_active_specimens(efe_ctx* ctx, const uint8_t* active);
+
 /* Number of specimens of the context (1 for an ordinary context). */
 int efe_get_specimen_count(efe_ctx* ctx);
 

@@ -86,6 +86,11 @@ class OracleBatchSystem:
     def specimen_status(self):
         return self.status.copy()
 
+    def set_active_specimens(self, active):
+        # the oracle stand-in recomputes every specimen; the driver's mask is recorded only
+        self.activeCalls = getattr(self, "activeCalls", 0) + 1
+        self.lastActive = None if active is None else np.asarray(active, dtype=bool).copy()
+
     def norms_batched(self, R, ddU, F):
         out = np.zeros((self.n_specimens, 4))
         Rh, Fh = R.numpy(), F.numpy()
