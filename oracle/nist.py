@@ -334,6 +334,10 @@ class NIST:
                     model.time = ts.totalTime
                     self.history.append((ts.number, ts.stepProgress, iters))
                     self._status(ts, iters, True, "")
+                    # per converged increment: the displacements and QP stresses, for parity
+                    # checks increment by increment
+                    self.increments[-1]["U"] = U.copy()
+                    self.increments[-1]["stress"] = st[..., 0:6].copy()
         except (ReachedMaxIncrements, ReachedMinIncrementSize):
             raise StepFailed()
         model.U = U
