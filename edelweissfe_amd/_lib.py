@@ -78,6 +78,7 @@ SIGNATURES = {
     "efe_get_csr_pattern": (_I, [_P, _P, _P]),
     "efe_set_dirichlet": (_I, [_P, _P, _I]),
     "efe_compute_elements": (_I, [_P, _P, _P, _P, _D, _P, _P, _P, _P, _I, ctypes.POINTER(_D)]),
+    "efe_compute_residual": (_I, [_P, _P, _P, _P, _D, _P, _P, _I, ctypes.POINTER(_D)]),
     "efe_body_force": (_I, [_P, _P, _P]),
     "efe_surface_traction": (_I, [_P, _P, _P, _I, _P, _P]),
     "efe_collect_results": (_I, [_P, _I, _P, _I, _P, _I, _I, _I, _P]),

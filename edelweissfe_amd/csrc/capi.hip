@@ -371,6 +371,27 @@ int efe_compute_elements(efe_ctx* c, const double* U, const double* dU, const do
     return hs ? EFE_CUTBACK : EFE_OK;
 }
 
+int efe_compute_residual(efe_ctx* c, const double* U, const double* dU, const double time[2], double dT, double* P,
+                         double* F, int precision, double* pNewDT) {
+    (void)U;
+    (void)time;
+    (void)dT;
+    if (int rc = check(c)) return rc;
+    if (!dU || !P || !F || (precision != EFE_FP64 && precision != EFE_FP32 && precision != EFE_FP32_MATFREE)) {
+        efe_set_error("efe_compute_residual: invalid arguments");
+        return EFE_ERR_ARG;
+    }
+    if (!c->Kscr) EFE_HIP(hipMalloc(&c->Kscr, (size_t)c->scratchBlocks * kBlockPitch * sizeof(double)));
+    EFE_HIP(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
+    if (c->specStatus) EFE_HIP(hipMemsetAsync(c->specStatus, 0, (size_t)c->nspec * sizeof(int), c->stream));
+    if (int rc = efe_launch_residual(c, dU, P, F, precision)) return rc;
+    int hs = 0;
+    EFE_HIP(hipMemcpyAsync(&hs, c->status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    EFE_HIP(hipStreamSynchronize(c->stream));
+    if (pNewDT) *pNewDT = hs ? 0.5 : 1.0;
+    return hs ? EFE_CUTBACK : EFE_OK;
+}
+
 int efe_body_force(efe_ctx* c, const double* force3, double* PExt) {
     if (int rc = check(c)) return rc;
     return efe_launch_body_force(c, force3, PExt);

@@ -141,6 +141,7 @@ struct efe_ctx {
     int* specStatus = nullptr;   // [nspec] cutback request per specimen (batched contexts)
     int* specActive = nullptr;   // [nspec] element passes skip specimens with 0 (batched contexts)
     bool useActive = false;      // whether the element pass reads specActive
+    int formK = 1;               // 0: the element pass forms residuals only (efe_compute_residual)
     double* red = nullptr;       // reduction scratch
     int redn = 0;
 
@@ -160,6 +161,7 @@ struct efe_ctx {
 
 // element kernels (elements.hip)
 int efe_launch_elements(efe_ctx* c, const double* dU, int precision);
+int efe_launch_residual(efe_ctx* c, const double* dU, double* P, double* F, int precision);
 int efe_launch_assembly(efe_ctx* c, double* P, double* F, const double* dir_values, double* dir_correction,
                         int precision);
 int efe_launch_body_force(efe_ctx* c, const double* f3, double* PExt);

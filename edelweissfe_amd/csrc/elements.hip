@@ -50,7 +50,7 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
                                                      int* __restrict__ specStatus,
                                                      const int32_t* __restrict__ specOfElem,
                                                      const int32_t* __restrict__ cpos,
-                                                     const int* __restrict__ specActive) {
+                                                     const int* __restrict__ specActive, int formK) {
     constexpr int NS = 12 + MatState<MAT>::n;
     // elements past the end and, in batched contexts, of specimens the pass skips
     auto skip = [&](int e) { return e >= E || (specActive && !specActive[specOfElem[e]]); };
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
             *status = 1;
             if (specStatus) specStatus[specOfElem[e]] = 1;  // which specimen asked (batched contexts)
         }
-        if (MF) {
+        if (MF && formK) {
             const double fq = det * qpw[q];
             float* dc = mfC + (int64_t)q * NC * E + e;
 #pragma unroll
@@ -147,8 +147,9 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
         if (MatState<MAT>::n) sTmp[12 * EQ + sidx] = ms;
         (void)NS;
         const double f = det * qpw[q];
+        if (formK)
 #pragma unroll
-        for (int k = 0; k < 36; ++k) fC_s[el][q][k] = (KT)(f * C[k]);
+            for (int k = 0; k < 36; ++k) fC_s[el][q][k] = (KT)(f * C[k]);
 #pragma unroll
         for (int k = 0; k < 6; ++k) fs_s[el][q][k] = f * s[k];
     }
@@ -158,7 +159,8 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
     // stored as K_ab^T.  Drucker-Prager (non-associative in general) forms all NN^2 blocks.
     constexpr bool SYM = !MF && MAT != EFE_DRUCKER_PRAGER;
     constexpr int NB = MF ? NN : (SYM ? NN * (NN + 1) / 2 : NN * NN);  // blocks per element formed here
-    for (int t = lane; t < EPW * NB; t += 64) {
+    // a residual-only pass (formK = 0) leaves the tangent blocks alone
+    for (int t = formK ? lane : EPW * NB; t < EPW * NB; t += 64) {
         const int el = t / NB, r0 = t - (t / NB) * NB;
         int a, b;
         if (MF) {
@@ -487,6 +489,30 @@ __global__ __launch_bounds__(256) void surface_traction_kernel(int n, int E, int
     }
 }
 
+// P and F of a residual-only pass: the element residuals of every node summed in element
+// order, as the assembly's gather sums them.
+__global__ void pf_gather_kernel(int N, const int32_t* __restrict__ nptr, const int32_t* __restrict__ nlist,
+                                 const double* __restrict__ Pscr, double* __restrict__ P, double* __restrict__ F) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double p0 = 0, p1 = 0, p2 = 0, f0 = 0, f1 = 0, f2 = 0;
+    for (int c = nptr[n]; c < nptr[n + 1]; ++c) {
+        const double* src = Pscr + (int64_t)nlist[c] * 3;
+        p0 += src[0];
+        p1 += src[1];
+        p2 += src[2];
+        f0 += fabs(src[0]);
+        f1 += fabs(src[1]);
+        f2 += fabs(src[2]);
+    }
+    P[3 * n] = p0;
+    P[3 * n + 1] = p1;
+    P[3 * n + 2] = p2;
+    F[3 * n] = f0;
+    F[3 * n + 1] = f1;
+    F[3 * n + 2] = f2;
+}
+
 __global__ void gather_add_kernel(int N, const int32_t* __restrict__ nptr, const int32_t* __restrict__ nlist,
                                   const double* __restrict__ Pscr, double* __restrict__ out) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -528,11 +554,11 @@ void launch_elem_t(efe_ctx* c, const double* dU) {
     if (c->mfPass)
         element_kernel<NN, Q, MAT, float, EPW, true><<<grid, 64, 0, c->stream>>>(
             c->E, c->N, c->coords, c->conn, c->props, propStride, c->dNdxi, c->qpw, dU, c->stateRef, c->stateTmp,
-            (float*)c->Kscr, c->Pscr, c->status, c->mfC, c->specStatus, c->specOfElem, c->cpos, act);
+            (float*)c->Kscr, c->Pscr, c->status, c->mfC, c->specStatus, c->specOfElem, c->cpos, act, c->formK);
     else
         element_kernel<NN, Q, MAT, KT, EPW, false><<<grid, 64, 0, c->stream>>>(
             c->E, c->N, c->coords, c->conn, c->props, propStride, c->dNdxi, c->qpw, dU, c->stateRef, c->stateTmp,
-            (KT*)c->Kscr, c->Pscr, c->status, nullptr, c->specStatus, c->specOfElem, c->cpos, act);
+            (KT*)c->Kscr, c->Pscr, c->status, nullptr, c->specStatus, c->specOfElem, c->cpos, act, c->formK);
 }
 
 template <int NN, int Q, int MAT, typename KT>
@@ -585,6 +611,16 @@ int efe_launch_elements(efe_ctx* c, const double* dU, int precision) {
     c->mfPass = precision == EFE_FP32_MATFREE;
     int rc = (precision == EFE_FP64) ? launch_elem_type<double>(c, dU) : launch_elem_type<float>(c, dU);
     if (rc != EFE_OK) return rc;
+    EFE_HIP(hipGetLastError());
+    return EFE_OK;
+}
+
+int efe_launch_residual(efe_ctx* c, const double* dU, double* P, double* F, int precision) {
+    c->formK = 0;
+    const int rc = efe_launch_elements(c, dU, precision);  // the precision's launch shape; same residual
+    c->formK = 1;
+    if (rc != EFE_OK) return rc;
+    pf_gather_kernel<<<(c->N + 255) / 256, 256, 0, c->stream>>>(c->N, c->nptr, c->nlist, c->Pscr, P, F);
     EFE_HIP(hipGetLastError());
     return EFE_OK;
 }

@@ -101,6 +101,15 @@ class DeviceSystem:
         check(rc, "efe_compute_elements")
         return rc, pnew.value
 
+    def compute_residual(self, dU, P, F, precision=_lib.EFE_FP64, time=(0.0, 0.0), dT=0.0, U=None):
+        """Residual-only element pass (efe_compute_residual): the status and pNewDT."""
+        t = (ctypes.c_double * 2)(*time)
+        pnew = ctypes.c_double(1.0)
+        rc = self.lib.efe_compute_residual(self.ctx, self._ptr(U) if U is not None else None, self._ptr(dU), t,
+                                           float(dT), self._ptr(P), self._ptr(F), int(precision), ctypes.byref(pnew))
+        check(rc, "efe_compute_residual")
+        return rc, pnew.value
+
     def body_force(self, force, PExt):
         f = (ctypes.c_double * 3)(*[float(v) for v in force])
         check(self.lib.efe_body_force(self.ctx, f, self._ptr(PExt)), "efe_body_force")

@@ -134,6 +134,18 @@ int efe_compute_elements(efe_ctx* ctx, const double* U, const double* dU, const 
                          double* P, double* F, const double* dir_values, double* dir_correction,
                          int32_t precision, double* pNewDT);
 
+/* The residual half of efe_compute_elements (nonlinearimplicitstatic.py:794-849 without the
+ * tangent): the same material update of the reference state into the temporary state and
+ * the same P and F, bit for bit, but no element tangents, operator or block-Jacobi inverse.
+ * The operator stays that of the last efe_compute_elements.  For a convergence check that
+ * needs no solve when it passes: the Newton iteration 0 of an increment that starts from an
+ * extrapolated (or zero) increment, where the reference computes the tangent and drops it.
+ * precision picks the element pass's launch shape (that of the following efe_compute_elements);
+ * the residual is FP64 and the same in every precision.
+ * Returns EFE_OK or EFE_CUTBACK (then *pNewDT = 0.5). */
+int efe_compute_residual(efe_ctx* ctx, const double* U, const double* dU, const double time[2], double dT,
+                         double* P, double* F, int32_t precision, double* pNewDT);
+
 /* External body-force vector (element.py:348-371), added into PExt (device [n_dof]). */
 int efe_body_force(efe_ctx* ctx, const double* force3, double* PExt);
 

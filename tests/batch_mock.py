@@ -83,6 +83,11 @@ class OracleBatchSystem:
             dir_correction.copy_(torch.from_numpy(corr))
         return (_lib.EFE_CUTBACK if self.status.any() else _lib.EFE_OK), (0.5 if self.status.any() else 1.0)
 
+    def compute_residual(self, dU, P, F, **kw):
+        # the oracle's element routine forms the tangent as well; a following full pass
+        # recomputes the same state
+        return self.compute_elements(dU, P, F)
+
     def specimen_status(self):
         return self.status.copy()
 

@@ -59,3 +59,26 @@ def test_every_increment_matches_the_oracle(el_type, material, props, n, strain,
         assert inc == w["inc"]
         assert rel_err(U, w["U"]) < 1e-9, inc
         assert rel_err(sig, w["stress"]) < 1e-9, inc
+
+
+@pytest.mark.parametrize("material,props,precision,linsolver", [
+    ("vonmises", J2, "fp64", "pcg"), ("vonmises", J2, "mixed", "pcg"), ("druckerprager", DP, "fp64", "gmres")])
+def test_residual_first_changes_nothing_but_the_work(material, props, precision, linsolver):
+    """residualFirst (the default) checks Newton iteration 0 on a residual-only pass and forms
+    the tangent only when a solve follows: the same Newton counts, U and QP state bit for bit as
+    with a full element pass at every check, and fewer tangent passes."""
+    from edelweissfe_amd.solvers import NISTGPU
+
+    runs = []
+    for residualFirst in (True, False):
+        pm = device_model((4, 3, 3), (1.0, 1.0, 1.0), "C3D8", material, props, jitter=0.1, seed=6)
+        s = NISTGPU(precision=precision, linsolver=linsolver, residualFirst=residualFirst)
+        s.solveStep(device_step(pm, uniaxial_bcs(0.006 if material == "vonmises" else 0.003), maxInc=0.1,
+                                minInc=1e-6, maxNumInc=100, maxIter=15), pm)
+        runs.append((s, pm, pm.qpState.copy()))
+    (s1, m1, q1), (s0, m0, q0) = runs
+    print([r["iters"] for r in s1.history], s1.elementPasses, s0.elementPasses)
+    assert [r["iters"] for r in s1.history] == [r["iters"] for r in s0.history]
+    assert 0 in [r["iters"] for r in s1.history]  # increments that converge at their first check
+    assert np.array_equal(m1.U, m0.U) and np.array_equal(q1, q0)
+    assert s1.elementPasses < s0.elementPasses
