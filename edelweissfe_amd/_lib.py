@@ -89,7 +89,7 @@ SIGNATURES = {
     "efe_linsolve": (_I, [_P, _P, _P, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]),
     "efe_convergence_norms": (_I, [_P, _P, _P, _P, ctypes.POINTER(_D)]),
     "efe_convergence_norms_batched": (_I, [_P, _P, _P, _P, _P]),
-    "efe_linsolve_batched": (_I, [_P, _P, _P, ctypes.POINTER(SolveOpts), _P, _P, _P, ctypes.POINTER(SolveInfo)]),
+    "efe_linsolve_batched": (_I, [_P, _P, _P, ctypes.POINTER(SolveOpts), _P, _P, _P, _P, ctypes.POINTER(SolveInfo)]),
     "efe_accept_state_batched": (_I, [_P, _P]),
     "efe_get_specimen_count": (_I, [_P]),
     "efe_get_specimen_status": (_I, [_P, _P]),

@@ -18,7 +18,7 @@ struct BatchWork {
     int N = 0, S = 0, prec = -1;
     void *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
     double *nv0 = nullptr, *nv1 = nullptr;     // per-node values
-    double* sc = nullptr;                      // per specimen: [0] rz, [1] unused, [2] rr, [3] tol2, [4] beta
+    double* sc = nullptr;                      // per specimen: [0] rz, [1] rtol2, [2] rr, [3] tol2, [4] beta
     int *active = nullptr, *iters = nullptr;   // per specimen
     int* nactive = nullptr;                    // device count
     int* acc = nullptr;                        // per specimen accept flags
@@ -38,7 +38,7 @@ struct BatchWork {
 namespace {
 
 constexpr int BS = 256;
-enum { RZ = 0, ALPHA = 1, RR = 2, TOL2 = 3, BETA = 4, NSC = 5 };
+enum { RZ = 0, RTOL2 = 1, RR = 2, TOL2 = 3, BETA = 4, NSC = 5 };
 
 // Fixed-order workgroup sums: an xor butterfly inside each wavefront (all lanes end with
 // bitwise identical sums), then the per-wave sums in wave order.
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(BS) void b_init_kernel(int N, const int32_t* __rest
 // per specimen: rz, rr; converged specimens (or r = 0) are frozen at iteration 0
 __global__ __launch_bounds__(BS) void b_init_reduce_kernel(const int64_t* __restrict__ noff,
                                                            const double* __restrict__ nrz,
-                                                           const double* __restrict__ nrr, double rtol2,
+                                                           const double* __restrict__ nrr,
                                                            double* __restrict__ sc, int* __restrict__ active,
                                                            int* __restrict__ iters) {
     __shared__ double sh[BS];
@@ -141,8 +141,8 @@ __global__ __launch_bounds__(BS) void b_init_reduce_kernel(const int64_t* __rest
     range_sums<2>(arr, (int)noff[s], (int)noff[s + 1], sums, sh);
     const double rz = sums[0], rr = sums[1];
     if (threadIdx.x == 0) {
-        // stop rule ||r_s|| <= max(atol_s, rtol ||b_s||)
-        const double t2 = fmax(sc[s * NSC + TOL2], rtol2 * rr);
+        // stop rule ||r_s|| <= max(atol_s, rtol_s ||b_s||)
+        const double t2 = fmax(sc[s * NSC + TOL2], sc[s * NSC + RTOL2] * rr);
         sc[s * NSC + TOL2] = t2;
         sc[s * NSC + RZ] = rz;
         sc[s * NSC + RR] = rr;
@@ -387,7 +387,8 @@ int ensure_batch(efe_ctx* c, int prec) {
 
 template <typename T>
 int batched_pcg(efe_ctx* c, const double* b, double* xout, const efe_solve_opts* o, const double* spec_tol,
-                const uint8_t* spec_active, int32_t* spec_iters, efe_solve_info* info, const T* K, const T* Minv) {
+                const double* spec_rtol, const uint8_t* spec_active, int32_t* spec_iters, efe_solve_info* info,
+                const T* K, const T* Minv) {
     BatchWork* w = c->bw;
     hipStream_t st = w->ks;
     const int N = c->N, S = c->nspec, D = c->D;
@@ -395,6 +396,8 @@ int batched_pcg(efe_ctx* c, const double* b, double* xout, const efe_solve_opts*
     std::vector<int> act(S);
     for (int s = 0; s < S; ++s) {
         sc[(size_t)s * NSC + TOL2] = spec_tol[s] * spec_tol[s];
+        const double rt = spec_rtol ? spec_rtol[s] : o->rtol;
+        sc[(size_t)s * NSC + RTOL2] = rt * rt;
         act[s] = spec_active ? (spec_active[s] ? 1 : 0) : 1;
     }
     EFE_HIP(hipEventRecord(w->evA, c->stream));
@@ -403,8 +406,7 @@ int batched_pcg(efe_ctx* c, const double* b, double* xout, const efe_solve_opts*
     EFE_HIP(hipMemcpyAsync(w->active, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, st));
     b_init_kernel<T><<<gridn(N), BS, 0, st>>>(N, c->specOfNode, w->active, b, Minv, (T*)w->x, (T*)w->r, (T*)w->z,
                                               (T*)w->p, w->nv0, w->nv1);
-    b_init_reduce_kernel<<<S, BS, 0, st>>>(c->specNodeOff, w->nv0, w->nv1, o->rtol * o->rtol, w->sc, w->active,
-                                           w->iters);
+    b_init_reduce_kernel<<<S, BS, 0, st>>>(c->specNodeOff, w->nv0, w->nv1, w->sc, w->active, w->iters);
     EFE_HIP(hipGetLastError());
     int batch = o->check_every > 0 ? o->check_every : 16;
     const int maxit = o->max_iter > 0 ? o->max_iter : 10000;
@@ -459,7 +461,8 @@ int batched_pcg(efe_ctx* c, const double* b, double* xout, const efe_solve_opts*
 }  // namespace
 
 int efe_batched_linsolve(efe_ctx* c, const double* b, double* x, const efe_solve_opts* o, const double* spec_tol,
-                         const uint8_t* spec_active, int32_t* spec_iters, efe_solve_info* info) {
+                         const double* spec_rtol, const uint8_t* spec_active, int32_t* spec_iters,
+                         efe_solve_info* info) {
     if (!c->specOfNode || c->nspec < 1) {
         efe_set_error("batched solve needs a context created with specimens");
         return EFE_ERR_STATE;
@@ -476,10 +479,12 @@ int efe_batched_linsolve(efe_ctx* c, const double* b, double* x, const efe_solve
     int rc = ensure_batch(c, vp);
     if (rc) return rc;
     if (o->precision == EFE_FP32_MATFREE)
-        return batched_pcg<float>(c, b, x, o, spec_tol, spec_active, spec_iters, info, nullptr, c->Minv32);
+        return batched_pcg<float>(c, b, x, o, spec_tol, spec_rtol, spec_active, spec_iters, info, nullptr,
+                                  c->Minv32);
     if (o->precision == EFE_FP32)
-        return batched_pcg<float>(c, b, x, o, spec_tol, spec_active, spec_iters, info, c->K32, c->Minv32);
-    return batched_pcg<double>(c, b, x, o, spec_tol, spec_active, spec_iters, info, c->K64, c->Minv64);
+        return batched_pcg<float>(c, b, x, o, spec_tol, spec_rtol, spec_active, spec_iters, info, c->K32,
+                                  c->Minv32);
+    return batched_pcg<double>(c, b, x, o, spec_tol, spec_rtol, spec_active, spec_iters, info, c->K64, c->Minv64);
 }
 
 // per-specimen convergence quantities: [max|R|, ||ddU||_inf, ||F||_1, NaN]

@@ -532,13 +532,14 @@ int efe_convergence_norms_batched(efe_ctx* c, const double* R, const double* ddU
 }
 
 int efe_linsolve_batched(efe_ctx* c, const double* b, double* x, const efe_solve_opts* o, const double* spec_atol,
-                         const uint8_t* spec_active, int32_t* spec_iterations, efe_solve_info* info) {
+                         const double* spec_rtol, const uint8_t* spec_active, int32_t* spec_iterations,
+                         efe_solve_info* info) {
     if (int rc = check(c)) return rc;
     if (!b || !x || !o || !info || !spec_atol) {
         efe_set_error("efe_linsolve_batched: invalid arguments");
         return EFE_ERR_ARG;
     }
-    return efe_batched_linsolve(c, b, x, o, spec_atol, spec_active, spec_iterations, info);
+    return efe_batched_linsolve(c, b, x, o, spec_atol, spec_rtol, spec_active, spec_iterations, info);
 }
 
 int efe_accept_state_batched(efe_ctx* c, const uint8_t* accept) {

@@ -189,7 +189,8 @@ void efe_krylov_free(efe_ctx* c);
 
 // batched specimens (batched.hip)
 int efe_batched_linsolve(efe_ctx* c, const double* b, double* x, const efe_solve_opts* o, const double* spec_tol,
-                         const uint8_t* spec_active, int32_t* spec_iters, efe_solve_info* info);
+                         const double* spec_rtol, const uint8_t* spec_active, int32_t* spec_iters,
+                         efe_solve_info* info);
 int efe_launch_accept_batched(efe_ctx* c, const uint8_t* accept);
 void efe_batched_free(efe_ctx* c);
 

@@ -169,15 +169,18 @@ class DeviceSystem:
         return np.array(out[:])
 
     def linsolve_batched(self, b, x, spec_atol, spec_active=None, precision=_lib.EFE_FP64, rtol=0.0, max_iter=20000,
-                         check_every=16):
-        """One block-Jacobi PCG per specimen, all at once; returns (info, iterations per specimen)."""
+                         check_every=16, spec_rtol=None):
+        """One block-Jacobi PCG per specimen, all at once; returns (info, iterations per specimen).
+        spec_rtol: per-specimen relative tolerances (default: rtol for all)."""
         o = _lib.SolveOpts(solver=_lib.EFE_SOLVER_CG, precond=_lib.EFE_PRECOND_BLOCK_JACOBI, precision=precision,
                            max_iter=max_iter, restart=0, check_every=check_every, rtol=rtol, atol=0.0)
         info = _lib.SolveInfo()
         tol = np.ascontiguousarray(spec_atol, dtype=np.float64)
         act = None if spec_active is None else np.ascontiguousarray(spec_active, dtype=np.uint8)
         its = np.zeros(self.n_specimens, dtype=np.int32)
+        rt = None if spec_rtol is None else np.ascontiguousarray(spec_rtol, dtype=np.float64)
         check(self.lib.efe_linsolve_batched(self.ctx, self._ptr(b), self._ptr(x), ctypes.byref(o), tol.ctypes.data,
+                                            rt.ctypes.data if rt is not None else None,
                                             act.ctypes.data if act is not None else None, its.ctypes.data,
                                             ctypes.byref(info)), "efe_linsolve_batched")
         return info, its

@@ -40,7 +40,8 @@ class NISTBatched:
     identification = "NISTBatchedSolver"
 
     options_default = {"extrapolation": "linear", "precision": "fp64", "krylovAtolFactor": 1e-2,
-                       "krylovRtol": 1e-13, "krylovRtolMixed": 1e-4, "krylovMaxIter": 50000,
+                       "krylovRtol": 1e-13, "krylovRtolMixed": 1e-4, "krylovRtolMixedFirst": 1e-7,
+                       "krylovMaxIter": 50000,
                        "krylovCheckEvery": 16, "autotune": True}
 
     def __init__(self, **kwargs):
@@ -306,8 +307,12 @@ class NISTBatched:
                 break
             b = R - corr if corr is not None else R
             ddU = torch.empty(D, dtype=torch.float64, device=dev)
+            # mixed form: a specimen's FP32 solve of its prescribed increment runs to
+            # krylovRtolMixedFirst, as in NISTGPU (its next correction then passes 1e-8)
+            specRtol = np.where(first, o["krylovRtolMixedFirst"], rtol) if self._mixed else None
             info, its = sysm.linsolve_batched(b, ddU, o["krylovAtolFactor"] * fluxThreshold, go.astype(np.uint8),
-                                              precision=prec, rtol=rtol, max_iter=o["krylovMaxIter"],
+                                              spec_rtol=specRtol, precision=prec, rtol=rtol,
+                                              max_iter=o["krylovMaxIter"],
                                               check_every=o["krylovCheckEvery"])
             self.linearSolves += int(go.sum())
             self.krylovIterations += int(its[go].sum())

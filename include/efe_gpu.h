@@ -190,13 +190,14 @@ int efe_convergence_norms_batched(efe_ctx* ctx, const double* R, const double* d
 
 /* Batched contexts (specimen_of_node given): one block-Jacobi PCG per specimen, run
  * together.  Each specimen has its own alpha/beta, its own stop rule
- * ||r_s||_2 <= max(spec_atol[s], opts->rtol ||b_s||_2) and freezes once converged;
+ * ||r_s||_2 <= max(spec_atol[s], rtol_s ||b_s||_2), rtol_s = spec_rtol[s] (host [n_specimens])
+ * or opts->rtol when spec_rtol is NULL, and freezes once converged;
  * spec_active (host [n_specimens] or NULL = all) excludes specimens (x = 0 there);
  * spec_iterations (host [n_specimens] or NULL) receives the iteration counts.
  * Replaces one linearSolve per specimen model (nonlinearimplicitstatic.py:727-751). */
 int efe_linsolve_batched(efe_ctx* ctx, const double* b, double* x, const efe_solve_opts* opts,
-                         const double* spec_atol, const uint8_t* spec_active, int32_t* spec_iterations,
-                         efe_solve_info* info);
+                         const double* spec_atol, const double* spec_rtol, const uint8_t* spec_active,
+                         int32_t* spec_iterations, efe_solve_info* info);
 
 /* Accepts the temporary state of the specimens with accept[s] != 0 (host [n_specimens]);
  * femodel.py:245-258 per specimen model. */

@@ -109,7 +109,7 @@ class OracleBatchSystem:
         return out
 
     def linsolve_batched(self, b, x, spec_atol, spec_active=None, precision=_lib.EFE_FP64, rtol=0.0, max_iter=0,
-                         check_every=0):
+                         check_every=0, spec_rtol=None):
         bh = b.numpy()
         xh = np.zeros(self.n_dof)
         its = np.zeros(self.n_specimens, dtype=np.int32)

@@ -126,7 +126,7 @@ def test_entry_points_reject_bad_arguments_and_keep_the_context_usable():
     assert lib.efe_accept_state_batched(ctx, u8) != _lib.EFE_OK
     assert lib.efe_set_active_specimens(ctx, u8) == _lib.EFE_ERR_STATE and "specimens" in _err(lib)
     tol = (ctypes.c_double * 1)(1e-8)
-    assert lib.efe_linsolve_batched(ctx, ptr(b), ptr(x), ctypes.byref(o), tol, None, None,
+    assert lib.efe_linsolve_batched(ctx, ptr(b), ptr(x), ctypes.byref(o), tol, None, None, None,
                                     ctypes.byref(info)) != _lib.EFE_OK
 
     # the context still works: an element pass, a solve and the element matrices
