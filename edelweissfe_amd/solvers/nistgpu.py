@@ -113,7 +113,9 @@ class NISTGPU:
         self.computationTimes = createTimingDict()
         self.krylovIterations = 0
         self.linearSolves = 0
-        self.elementPasses = 0
+        self.elementPasses = 0     # element passes that formed tangents
+        self.residualPasses = 0    # residual-only passes (residualFirst)
+        self.newtonChecks = 0      # element evaluations of the reference: one per Newton iteration
 
     def _updateOptions(self, updated):
         for k, v in updated.items():
@@ -293,10 +295,12 @@ class NISTGPU:
             # Newton iteration 0 without a prescribed-value solve checks convergence first:
             # a residual-only pass, and the tangent pass only if a solve follows
             residualOnly = self.options["residualFirst"] and iterationCounter == 0 and not first
+            self.newtonChecks += 1
             tic = _time.perf_counter()
             if residualOnly:
                 st, pnew = sysm.compute_residual(dU, P, F, precision=prec, time=(timeStep.stepTime, timeStep.totalTime),
                                                  dT=timeStep.timeIncrement)
+                self.residualPasses += 1
             else:
                 st, pnew = sysm.compute_elements(dU, P, F, dir_values=dirVals if first else None,
                                                  dir_correction=corr, precision=prec,
