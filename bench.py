@@ -333,6 +333,25 @@ def footing_case(n, increments, settlement):
     return model, steps_for
 
 
+def phase_avg(before, after):
+    """History averages of the device phase times (ms per call) between two phase_totals
+    snapshots: element kernel and assembly per tangent pass, residual-only pass, Krylov
+    iteration (CG or GMRES), ILU(0) factorisation per solve, norms per call; and the counts."""
+    d = {k: after[k] - before[k] for k in after if k != "avg_ms"}
+
+    def avg(a, n):
+        return d[a] / d[n] if d[n] else 0.0
+
+    return {"elements_per_tangent_pass": avg("elements_ms", "element_passes"),
+            "assembly_per_tangent_pass": avg("assembly_ms", "element_passes"),
+            "residual_only_pass": avg("residual_ms", "residual_passes"),
+            "krylov_iteration": avg("krylov_ms", "krylov_iterations"),
+            "ilu_factor_per_solve": avg("ilu_factor_ms", "solves"),
+            "norms": avg("norms_ms", "norm_calls"),
+            "tangent_passes": int(d["element_passes"]), "residual_only_passes": int(d["residual_passes"]),
+            "solves": int(d["solves"]), "krylov_iterations": int(d["krylov_iterations"])}
+
+
 def run_footing(system, template, steps_for, precision, steps, warmup):
     import torch
 
@@ -372,8 +391,10 @@ def bench_footing(args):
     system = DeviceSystem(template.coordinates, template.connectivity, "C3D20R", "druckerprager", DP)
     torch.cuda.synchronize()
     log("[bench] device system ready: %d DOFs, %d nnz (%.1f s)" % (system.n_dof, system.nnz, time.perf_counter() - t0))
+    pt0 = system.phase_totals()
     t64, s64, m64 = run_footing(system, template, steps_for, "fp64", args.steps, args.warmup)
-    ph64 = system.phase_times()
+    pt1 = system.phase_totals()
+    ph64 = phase_avg(pt0, pt1)
     q64 = m64[-1].qpState
     spmv_ms, spmv_alg = spmv_roofline(system)
     D = system.n_dof
@@ -422,7 +443,7 @@ def bench_footing(args):
                 "kappa_rel_inf": float(np.abs(q32[..., 12] - q64[..., 12]).max()
                                        / max(np.abs(q64[..., 12]).max(), 1e-300)),
             },
-            "phase_ms": system.phase_times(),
+            "phase_ms": phase_avg(pt1, system.phase_totals()),
         }
         log("[bench] footing mixed: %.3f s (%.2fx), U rel err %.2e" % (
             wall32, wall64 / wall32, result["mixed"]["error_vs_fp64"]["U_rel_inf"]))
@@ -512,8 +533,10 @@ def main():
         profile_spmv(system)
         return
 
+    pt0 = system.phase_totals()
     t64, s64, m64 = run_history(system, template, step_for, "fp64", args.steps, args.warmup)
-    ph64 = system.phase_times()
+    pt1 = system.phase_totals()
+    ph64 = phase_avg(pt0, pt1)
     q64 = m64[-1].qpState  # download now: the mixed run reuses the device state buffers
     spmv_ms, spmv_alg = spmv_roofline(system)
     log("[bench] fp64 SpMV %.4f ms, %.1f GB/s algorithmic" % (spmv_ms, spmv_alg / spmv_ms / 1e6))
@@ -549,7 +572,7 @@ def main():
 
     if not args.no_mixed:
         t32, s32, m32 = run_history(system, template, step_for, "mixed", args.steps, args.warmup)
-        ph32 = system.phase_times()
+        ph32 = phase_avg(pt1, system.phase_totals())
         wall32 = sum(t32)
         nis32 = sum(s.linearSolves for s in s32) / wall32
         u64, u32 = m64[-1].U, m32[-1].U

@@ -20,7 +20,7 @@ EFE_ERR_ARG, EFE_ERR_HIP, EFE_ERR_STATE = -1, -2, -3
 EFE_C3D8, EFE_C3D20, EFE_C3D20R = 0, 1, 2
 EFE_LINEAR_ELASTIC, EFE_VONMISES, EFE_DRUCKER_PRAGER = 0, 1, 2
 EFE_FP64, EFE_FP32, EFE_FP32_MATFREE = 0, 1, 2
-EFE_SOLVER_CG, EFE_SOLVER_GMRES = 0, 1
+EFE_SOLVER_CG, EFE_SOLVER_GMRES, EFE_SOLVER_BICGSTAB = 0, 1, 2
 EFE_PRECOND_BLOCK_JACOBI, EFE_PRECOND_ILU0 = 0, 1
 
 ELEMENT_TYPES = {"C3D8": EFE_C3D8, "C3D20": EFE_C3D20, "C3D20R": EFE_C3D20R}
@@ -41,6 +41,7 @@ class MeshDesc(ctypes.Structure):
         ("specimen_of_node", ctypes.c_void_p),
         ("n_specimens", ctypes.c_int32),
         ("stream", ctypes.c_void_p),
+        ("material_of_element", ctypes.c_void_p),
     ]
 
 
@@ -79,7 +80,7 @@ SIGNATURES = {
     "efe_set_dirichlet": (_I, [_P, _P, _I]),
     "efe_compute_elements": (_I, [_P, _P, _P, _P, _D, _P, _P, _P, _P, _I, ctypes.POINTER(_D)]),
     "efe_compute_residual": (_I, [_P, _P, _P, _P, _D, _P, _P, _I, ctypes.POINTER(_D)]),
-    "efe_body_force": (_I, [_P, _P, _P]),
+    "efe_body_force": (_I, [_P, _P, _I, _P, _P]),
     "efe_surface_traction": (_I, [_P, _P, _P, _I, _P, _P]),
     "efe_collect_results": (_I, [_P, _I, _P, _I, _P, _I, _I, _I, _P]),
     "efe_get_element_matrices": (_I, [_P, _P, _I, _P, _P]),
@@ -99,6 +100,9 @@ SIGNATURES = {
     "efe_set_active_specimens": (_I, [_P, _P]),
     "efe_set_launch": (_I, [_P, _I, _I]),
     "efe_get_phase_times": (_I, [_P, ctypes.POINTER(_D)]),
+    "efe_csr_linsolve": (_I, [_I, _P, _P, _P, _P, _P, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]),
+    "efe_csr_linsolve_device": (_I, [_I, _P, _P, _P, _P, _P, _P, ctypes.POINTER(SolveOpts),
+                                     ctypes.POINTER(SolveInfo)]),
 }
 
 _lib = None

@@ -1,8 +1,10 @@
 """Launch-shape autotuner: elements per workgroup of the element kernel and threads per
 workgroup of the Krylov kernels, chosen by timing on the device the system lives on.
 
-Results are cached per (element type, material, element count, node count, device name)
-in a JSON file (EFE_AUTOTUNE_CACHE, default edelweissfe_amd/lib/autotune.json).
+Results are cached per (element type, material, element count, node count, device name,
+library build) in a JSON file (EFE_AUTOTUNE_CACHE, default edelweissfe_amd/lib/autotune.json):
+a rebuilt library (another size or modification time of libefe_gpu.so) is tuned afresh, so a
+changed kernel never runs with the launch shapes or operator costs measured for the old one.
 """
 
 import json
@@ -22,9 +24,19 @@ def _cache_path():
                           os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "autotune.json"))
 
 
+def _build_id():
+    """Identity of the loaded library build: size and modification time of libefe_gpu.so."""
+    try:
+        st = os.stat(_lib.LIB_PATH)
+        return "%d-%d" % (st.st_size, st.st_mtime_ns)
+    except OSError:
+        return "unknown"
+
+
 def _key(system):
     dev = torch.cuda.get_device_name(system.device)
-    return "%s|%s|%d|%d|%s" % (system.el_type, system.material, system.n_elements, system.n_nodes, dev)
+    return "%s|%s|%d|%d|%s|%s" % (system.el_type, system.material, system.n_elements, system.n_nodes, dev,
+                                  _build_id())
 
 
 def _load():

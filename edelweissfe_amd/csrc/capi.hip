@@ -106,6 +106,39 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 
 }  // namespace
 
+namespace {
+// device buffers of one call, freed on every return path
+struct DeviceBuffers {
+    std::vector<void*> p;
+    ~DeviceBuffers() {
+        for (void* q : p) hipFree(q);
+    }
+    template <typename T>
+    hipError_t alloc(T** out, size_t n) {
+        void* q = nullptr;
+        const hipError_t e = hipMalloc(&q, (n ? n : 1) * sizeof(T));
+        if (e == hipSuccess) {
+            p.push_back(q);
+            *out = (T*)q;
+        }
+        return e;
+    }
+};
+
+int check_solve_args(int32_t n, const void* ip, const void* ix, const void* v, const void* b, const void* x,
+                     const efe_solve_opts* o, const efe_solve_info* info, const char* what) {
+    if (n <= 0 || !ip || !ix || !v || !b || !x || !o || !info) {
+        efe_set_error(std::string(what) + ": invalid arguments");
+        return EFE_ERR_ARG;
+    }
+    if (o->solver != EFE_SOLVER_CG && o->solver != EFE_SOLVER_BICGSTAB) {
+        efe_set_error(std::string(what) + ": solver must be EFE_SOLVER_CG or EFE_SOLVER_BICGSTAB");
+        return EFE_ERR_ARG;
+    }
+    return EFE_OK;
+}
+}  // namespace
+
 void efe_set_error(const std::string& msg) { g_err = msg; }
 
 extern "C" {
@@ -125,7 +158,22 @@ int efe_ctx_create(const efe_mesh_desc* d, efe_ctx** out) {
         default: efe_set_error("unknown element type"); return EFE_ERR_ARG;
     }
     int np = (d->material == EFE_LINEAR_ELASTIC) ? 2 : 6;
-    if (d->material < 0 || d->material > 2 || d->n_props != np) {
+    const int32_t* mat_of = d->material_of_element;
+    int mat_count[3] = {0, 0, 0};
+    if (mat_of) {
+        if (d->n_props != 6 || !d->props_per_element) {
+            efe_set_error("several sections need per-element properties with n_props = 6");
+            return EFE_ERR_ARG;
+        }
+        for (int e = 0; e < d->n_elements; ++e) {
+            if (mat_of[e] < 0 || mat_of[e] > 2) {
+                efe_set_error("material_of_element: unknown material");
+                return EFE_ERR_ARG;
+            }
+            ++mat_count[mat_of[e]];
+        }
+        np = 6;
+    } else if (d->material < 0 || d->material > 2 || d->n_props != np) {
         efe_set_error("material / n_props mismatch (linear elastic: 2, von Mises and Drucker-Prager: 6)");
         return EFE_ERR_ARG;
     }
@@ -144,6 +192,21 @@ int efe_ctx_create(const efe_mesh_desc* d, efe_ctx** out) {
     c->nprops = np;
     c->props_per_element = d->props_per_element ? 1 : 0;
     c->nstate = 12 + (d->material == EFE_LINEAR_ELASTIC ? 0 : 1);
+    if (mat_of) {
+        c->ngroups = 0;
+        for (int m = 0; m < 3; ++m)
+            if (mat_count[m]) {
+                c->groupMat[c->ngroups] = m;
+                c->groupCount[c->ngroups] = mat_count[m];
+                ++c->ngroups;
+            }
+        c->material = c->groupMat[c->ngroups - 1];  // the material with the most state (LE < J2 < DP)
+        c->nstate = 12 + (mat_count[EFE_VONMISES] || mat_count[EFE_DRUCKER_PRAGER] ? 1 : 0);
+    } else {
+        c->groupMat[0] = c->material;
+        c->groupCount[0] = c->E;
+    }
+    c->mfNC = (mat_of ? mat_count[EFE_DRUCKER_PRAGER] > 0 : d->material == EFE_DRUCKER_PRAGER) ? 11 : 9;
     c->stream = (hipStream_t)d->stream;
     c->nspec = d->specimen_of_node ? d->n_specimens : 1;
     std::string err;
@@ -180,6 +243,14 @@ int efe_ctx_create(const efe_mesh_desc* d, efe_ctx** out) {
             pr.assign(d->props, d->props + np);
         }
         if ((rc = upload(&c->props, pr.data(), pr.size(), st))) return fail(rc);
+        if (mat_of && c->ngroups > 1)
+            for (int g = 0; g < c->ngroups; ++g) {
+                std::vector<int32_t> el;
+                el.reserve(c->groupCount[g]);
+                for (int e = 0; e < c->E; ++e)
+                    if (mat_of[e] == c->groupMat[g]) el.push_back(e);
+                if ((rc = upload(&c->groupElems[g], el.data(), el.size(), st))) return fail(rc);
+            }
         std::vector<double> xi, eta, zeta, w;
         qp_table(c->eltype, xi, eta, zeta, w);
         std::vector<double> dn((size_t)Q * NN * 3), sn((size_t)Q * NN), tmp((size_t)NN * 3), tn((size_t)Q * NN);
@@ -191,7 +262,19 @@ int efe_ctx_create(const efe_mesh_desc* d, efe_ctx** out) {
             const double pt[3] = {xi[q], eta[q], zeta[q]};
             shape(NN, pt, sn.data() + (size_t)q * NN, tmp.data());
         }
+        // The table J is formed with.  hexa20: the reference's third Jacobian row takes the
+        // coefficient a5 of the monomial (eta zeta) of its polynomial fit times zeta_q where the
+        // derivative has xi_q (_elementcomputationmatrices.py:502,514,526); a5 = sum_a h_a x_a with
+        // h_a = d2N_a / (d eta d zeta) at the centre (+-1/4 on the mid-side nodes (0, +-1, +-1)),
+        // so J's third row gains (zeta_q - xi_q) h_a per node.  Zero on undistorted elements.
+        std::vector<double> dj(dn);
+        if (NN == 20)
+            for (int q = 0; q < Q; ++q)
+                for (int a = 0; a < NN; ++a)
+                    if (kNat20[a][0] == 0)
+                        dj[((size_t)q * NN + a) * 3 + 2] += (zeta[q] - xi[q]) * kNat20[a][1] * kNat20[a][2] / 4.0;
         if ((rc = upload(&c->dNdxi, dn.data(), dn.size(), st))) return fail(rc);
+        if ((rc = upload(&c->dJdxi, dj.data(), dj.size(), st))) return fail(rc);
         if ((rc = upload(&c->shapeN, sn.data(), sn.size(), st))) return fail(rc);
         if ((rc = upload(&c->shapeNe, tn.data(), tn.size(), st))) return fail(rc);
         if ((rc = upload(&c->qpw, w.data(), w.size(), st))) return fail(rc);
@@ -269,11 +352,12 @@ void efe_ctx_destroy(efe_ctx* c) {
     efe_gmres_free(c);
     efe_batched_free(c);
     efe_mf_free(c);
-    void* bufs[] = {c->shapeNe, c->geoElems, c->surfElems, c->surfMask, c->resIdx,
+    void* bufs[] = {c->shapeNe, c->dJdxi, c->geoElems, c->surfElems, c->surfMask, c->resIdx,
                     c->coords, c->conn,  c->props,  c->dNdxi,  c->shapeN, c->qpw,    c->stateRef, c->stateTmp,
                     c->Kscr,   c->Pscr,  c->bptr,   c->bcol,   c->coff,   c->cpos,   c->cslice, c->nptr,     c->nlist,
                     c->diagk,  c->slice_off, c->scol, c->K64,  c->K32,    c->Minv64, c->Minv32,   c->dirflag,
                     c->slotNode, c->nodeSlot, c->specStatus, c->specActive, c->elemPos,
+                    c->groupElems[0], c->groupElems[1], c->groupElems[2],
                     c->status, c->red,   c->specOfNode, c->specNodeOff, c->specOfElem};
     for (void* p : bufs)
         if (p) hipFree(p);
@@ -362,11 +446,16 @@ int efe_compute_elements(efe_ctx* c, const double* U, const double* dU, const do
         return rc;
     }
     EFE_HIP(hipEventRecord(c->ev[2], c->stream));
+    c->kscrValid = precision != EFE_FP32_MATFREE;
+    c->pscrValid = true;
     int hs = 0;
     EFE_HIP(hipMemcpyAsync(&hs, c->status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     EFE_HIP(hipStreamSynchronize(c->stream));
     c->t_elem = elapsed(c->ev[0], c->ev[1]);
     c->t_asm = elapsed(c->ev[1], c->ev[2]);
+    c->tot[EFE_T_ELEMENTS] += c->t_elem;
+    c->tot[EFE_T_ELEMENT_PASSES] += 1;
+    c->tot[EFE_T_ASSEMBLY] += c->t_asm;
     if (pNewDT) *pNewDT = hs ? 0.5 : 1.0;
     return hs ? EFE_CUTBACK : EFE_OK;
 }
@@ -384,17 +473,51 @@ int efe_compute_residual(efe_ctx* c, const double* U, const double* dU, const do
     if (!c->Kscr) EFE_HIP(hipMalloc(&c->Kscr, (size_t)c->scratchBlocks * kBlockPitch * sizeof(double)));
     EFE_HIP(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
     if (c->specStatus) EFE_HIP(hipMemsetAsync(c->specStatus, 0, (size_t)c->nspec * sizeof(int), c->stream));
+    EFE_HIP(hipEventRecord(c->ev[5], c->stream));
     if (int rc = efe_launch_residual(c, dU, P, F, precision)) return rc;
+    EFE_HIP(hipEventRecord(c->ev[6], c->stream));
+    c->kscrValid = false;  // the blocks are those of an earlier state now
+    c->pscrValid = true;
     int hs = 0;
     EFE_HIP(hipMemcpyAsync(&hs, c->status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     EFE_HIP(hipStreamSynchronize(c->stream));
+    c->tot[EFE_T_RESIDUAL] += elapsed(c->ev[5], c->ev[6]);
+    c->tot[EFE_T_RESIDUAL_PASSES] += 1;
     if (pNewDT) *pNewDT = hs ? 0.5 : 1.0;
     return hs ? EFE_CUTBACK : EFE_OK;
 }
 
-int efe_body_force(efe_ctx* c, const double* force3, double* PExt) {
+int efe_body_force(efe_ctx* c, const int32_t* elements, int32_t n, const double* force3, double* PExt) {
     if (int rc = check(c)) return rc;
-    return efe_launch_body_force(c, force3, PExt);
+    if (!force3 || !PExt || (elements && n < 0)) {
+        efe_set_error("efe_body_force: invalid arguments");
+        return EFE_ERR_ARG;
+    }
+    if (!elements) return efe_launch_body_force(c, nullptr, c->E, force3, PExt);
+    // the element set, each element once, ascending (an element set holds an element once)
+    std::vector<uint8_t> in((size_t)c->E, 0);
+    for (int32_t i = 0; i < n; ++i) {
+        if (elements[i] < 0 || elements[i] >= c->E) {
+            efe_set_error("efe_body_force: element out of range");
+            return EFE_ERR_ARG;
+        }
+        in[elements[i]] = 1;
+    }
+    std::vector<int32_t> el;
+    for (int e = 0; e < c->E; ++e)
+        if (in[e]) el.push_back(e);
+    if (el.size() > c->resCap) {
+        hipFree(c->resIdx);
+        c->resIdx = nullptr;
+        c->resCap = 0;
+        EFE_HIP(hipMalloc(&c->resIdx, el.size() * sizeof(int32_t)));
+        c->resCap = el.size();
+    }
+    if (!el.empty())
+        EFE_HIP(hipMemcpyAsync(c->resIdx, el.data(), el.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    if (int rc = efe_launch_body_force(c, c->resIdx, (int)el.size(), force3, PExt)) return rc;
+    EFE_HIP(hipStreamSynchronize(c->stream));  // el must outlive the copy
+    return EFE_OK;
 }
 
 int efe_surface_traction(efe_ctx* c, const int32_t* elements, const int32_t* faces, int32_t n,
@@ -474,8 +597,14 @@ int efe_get_element_matrices(efe_ctx* c, const int32_t* elements, int32_t n, dou
         efe_set_error("efe_get_element_matrices: invalid arguments");
         return EFE_ERR_ARG;
     }
-    if (c->Kprec < 0 || c->Kprec == EFE_FP32_MATFREE) {
-        efe_set_error("efe_get_element_matrices: the last element pass formed no element matrices (matrix-free)");
+    if (c->Kprec < 0 || c->Kprec == EFE_FP32_MATFREE || !c->kscrValid) {
+        efe_set_error("efe_get_element_matrices: the last element pass formed no element matrices "
+                      "(matrix-free or residual-only)");
+        return EFE_ERR_STATE;
+    }
+    if (Pe && !c->pscrValid) {
+        efe_set_error("efe_get_element_matrices: a load computation has overwritten the element residuals "
+                      "since the last element pass");
         return EFE_ERR_STATE;
     }
     for (int32_t i = 0; i < n; ++i)
@@ -523,6 +652,8 @@ int efe_convergence_norms(efe_ctx* c, const double* R, const double* ddU, const 
     EFE_HIP(hipEventRecord(c->ev[4], c->stream));
     EFE_HIP(hipEventSynchronize(c->ev[4]));
     c->t_norm = elapsed(c->ev[3], c->ev[4]);
+    c->tot[EFE_T_NORMS] += c->t_norm;
+    c->tot[EFE_T_NORM_CALLS] += 1;
     return rc;
 }
 
@@ -654,6 +785,63 @@ int efe_collect_results(efe_ctx* c, int32_t which, const int32_t* elements, int3
     return EFE_OK;
 }
 
+
+int efe_csr_linsolve(int32_t n, const int32_t* indptr, const int32_t* indices, const double* data, const double* b,
+                     double* x, const efe_solve_opts* o, efe_solve_info* info) {
+    if (int rc = check_solve_args(n, indptr, indices, data, b, x, o, info, "efe_csr_linsolve")) return rc;
+    if (indptr[0] != 0) {
+        efe_set_error("efe_csr_linsolve: indptr[0] must be 0");
+        return EFE_ERR_ARG;
+    }
+    for (int32_t i = 0; i < n; ++i)
+        if (indptr[i + 1] < indptr[i]) {
+            efe_set_error("efe_csr_linsolve: indptr must be non-decreasing");
+            return EFE_ERR_ARG;
+        }
+    const int64_t nnz = indptr[n];
+    for (int64_t k = 0; k < nnz; ++k)
+        if (indices[k] < 0 || indices[k] >= n) {
+            efe_set_error("efe_csr_linsolve: column index out of range");
+            return EFE_ERR_ARG;
+        }
+    DeviceBuffers db;
+    int32_t *dip, *dix;
+    double *dv, *dbv, *dx, *ws;
+    EFE_HIP(db.alloc(&dip, (size_t)n + 1));
+    EFE_HIP(db.alloc(&dix, (size_t)nnz));
+    EFE_HIP(db.alloc(&dv, (size_t)nnz));
+    EFE_HIP(db.alloc(&dbv, (size_t)n));
+    EFE_HIP(db.alloc(&dx, (size_t)n));
+    EFE_HIP(db.alloc(&ws, efe_csr_workspace_doubles(n, o->solver)));
+    hipStream_t st = nullptr;
+    EFE_HIP(hipMemcpyAsync(dip, indptr, ((size_t)n + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    EFE_HIP(hipMemcpyAsync(dix, indices, (size_t)nnz * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    EFE_HIP(hipMemcpyAsync(dv, data, (size_t)nnz * sizeof(double), hipMemcpyHostToDevice, st));
+    EFE_HIP(hipMemcpyAsync(dbv, b, (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+    if (int rc = efe_csr_solve_ws(n, dip, dix, dv, dbv, dx, o, info, ws, st)) return rc;
+    EFE_HIP(hipMemcpyAsync(x, dx, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+    EFE_HIP(hipStreamSynchronize(st));
+    return EFE_OK;
+}
+
+int efe_csr_linsolve_device(int32_t n, const int32_t* indptr, const int32_t* indices, const double* data,
+                            const double* b, double* x, void* stream, const efe_solve_opts* o, efe_solve_info* info) {
+    if (int rc = check_solve_args(n, indptr, indices, data, b, x, o, info, "efe_csr_linsolve_device")) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    DeviceBuffers db;
+    double* ws;
+    int* flag;
+    EFE_HIP(db.alloc(&ws, efe_csr_workspace_doubles(n, o->solver)));
+    EFE_HIP(db.alloc(&flag, 1));
+    int bad = 0;
+    if (int rc = efe_csr_check_pattern(n, indptr, indices, flag, st, &bad)) return rc;
+    if (bad) {
+        efe_set_error("efe_csr_linsolve_device: invalid CSR pattern (indptr or column indices)");
+        return EFE_ERR_ARG;
+    }
+    return efe_csr_solve_ws(n, indptr, indices, data, b, x, o, info, ws, st);
+}
+
 int efe_get_qp_state(efe_ctx* c, int32_t which, double* out) {
     if (int rc = check(c)) return rc;
     const size_t EQ = (size_t)c->E * c->Q, ns = c->nstate;
@@ -693,12 +881,13 @@ int efe_set_launch(efe_ctx* c, int32_t epw, int32_t kblock) {
     return EFE_OK;
 }
 
-int efe_get_phase_times(efe_ctx* c, double out[4]) {
+int efe_get_phase_times(efe_ctx* c, double out[16]) {
     if (int rc = check(c)) return rc;
     out[0] = c->t_elem;
     out[1] = c->t_asm;
     out[2] = c->t_kry;
     out[3] = c->t_norm;
+    for (int k = 0; k < 12; ++k) out[4 + k] = k < EFE_T_COUNT ? c->tot[k] : 0.0;
     return EFE_OK;
 }
 

@@ -78,6 +78,22 @@ bool efe_build_ilu_pattern(const HostPattern& hp, IluPattern& ip, std::string& e
 // (host, conn [E][NN] node positions).  Returns false on inconsistent input.
 bool efe_build_pattern(HostPattern& hp, int N, int E, int NN, const int32_t* conn, std::string& err);
 
+// running totals of efe_get_phase_times (ms, or counts)
+enum {
+    EFE_T_ELEMENTS = 0,   // element kernel of tangent passes
+    EFE_T_ELEMENT_PASSES, // count
+    EFE_T_ASSEMBLY,       // assembly (or matrix-free diagonal gather)
+    EFE_T_RESIDUAL,       // residual-only passes (element kernel + P/F gather)
+    EFE_T_RESIDUAL_PASSES,
+    EFE_T_KRYLOV,         // Krylov iterations (without preconditioner set-up)
+    EFE_T_KRYLOV_ITERS,
+    EFE_T_SOLVES,
+    EFE_T_PRECOND,        // ILU(0) factorisation
+    EFE_T_NORMS,
+    EFE_T_NORM_CALLS,
+    EFE_T_COUNT
+};
+
 struct KrylovWork;
 struct GmresWork;
 struct BatchWork;
@@ -97,14 +113,26 @@ struct efe_ctx {
     int32_t* conn = nullptr;     // [NN][E]
     double* props = nullptr;     // [nprops][E] or [nprops]
     double* dNdxi = nullptr;     // [Q][NN][3] at the evaluation points
+    double* dJdxi = nullptr;     // [Q][NN][3] the table J is formed with (hexa20: the reference's third row)
     double* shapeN = nullptr;    // [Q][NN] at the table points (body force)
     double* shapeNe = nullptr;   // [Q][NN] at the evaluation points (QP coordinates)
-    // matrix-free FP32 operator: [Q][NC][E] compact tangent, [E*NN][3] node-sorted force slots
+    // matrix-free FP32 operator: [Q][NC][E] compact tangent, [E*NN][3] node-sorted force slots,
+    // dnxi32 = FP32 dNdxi then dJdxi ([2][Q][NN][3])
     float *mfC = nullptr, *mfF = nullptr, *dnxi32 = nullptr;
     int32_t* elemPos = nullptr;  // [NN][E] index of (e, a) in the node-sorted incidence list nlist
                                  // (force slot of the matrix-free operator; block-row (e, a) of the
                                  // element scratch starts at block elemPos * NN)
     bool mfPass = false;             // the current element launch is a matrix-free pass
+    int mfNC = 9;                    // compact tangent values per QP of the matrix-free operator
+                                     // (11 when a Drucker-Prager section is present)
+    // material groups (sections): the element kernel runs once per group over its element list
+    int ngroups = 1;
+    int groupMat[3] = {0, 0, 0};
+    int groupCount[3] = {0, 0, 0};
+    int32_t* groupElems[3] = {nullptr, nullptr, nullptr};  // device, ascending; nullptr: all elements
+    int curMat = 0;                  // the group of the current element launch
+    const int32_t* curList = nullptr;
+    int curCount = 0;
     int32_t* geoElems = nullptr; // [E] element subset of a geostatic initial state
     int32_t* surfElems = nullptr;  // loaded elements of the last surface traction call
     uint8_t* surfMask = nullptr;   // their loaded faces (bit f-1)
@@ -142,6 +170,9 @@ struct efe_ctx {
     int* specActive = nullptr;   // [nspec] element passes skip specimens with 0 (batched contexts)
     bool useActive = false;      // whether the element pass reads specActive
     int formK = 1;               // 0: the element pass forms residuals only (efe_compute_residual)
+    bool kscrValid = false;      // Kscr holds the element blocks of the last element pass (assembled)
+    bool pscrValid = false;      // Pscr holds the element residuals of the last element pass (no load
+                                 // computation has used it as scratch since)
     double* red = nullptr;       // reduction scratch
     int redn = 0;
 
@@ -154,9 +185,10 @@ struct efe_ctx {
     int epw = 0;       // elements per workgroup of the element kernel (0 = default)
     int kblock = 256;  // threads per workgroup of Krylov kernels
 
-    // phase timing
+    // phase timing: the last call of each phase, and totals over the context's life
     hipEvent_t ev[8] = {};
     double t_elem = 0, t_asm = 0, t_kry = 0, t_norm = 0;
+    double tot[EFE_T_COUNT] = {};
 };
 
 // element kernels (elements.hip)
@@ -164,7 +196,7 @@ int efe_launch_elements(efe_ctx* c, const double* dU, int precision);
 int efe_launch_residual(efe_ctx* c, const double* dU, double* P, double* F, int precision);
 int efe_launch_assembly(efe_ctx* c, double* P, double* F, const double* dir_values, double* dir_correction,
                         int precision);
-int efe_launch_body_force(efe_ctx* c, const double* f3, double* PExt);
+int efe_launch_body_force(efe_ctx* c, const int32_t* elems, int n, const double* f3, double* PExt);
 int efe_launch_surface_traction(efe_ctx* c, const int32_t* elems, const uint8_t* fmask, int n, const double* t3,
                                 double* PExt);
 int efe_launch_geostatic(efe_ctx* c, const double* def, const int32_t* elements, int32_t n);
@@ -193,6 +225,14 @@ int efe_batched_linsolve(efe_ctx* c, const double* b, double* x, const efe_solve
                          efe_solve_info* info);
 int efe_launch_accept_batched(efe_ctx* c, const uint8_t* accept);
 void efe_batched_free(efe_ctx* c);
+
+// scalar CSR Krylov solves (csrsolve.hip)
+constexpr int kCsrPartials = 1024;  // workgroup partials per dot product
+size_t efe_csr_workspace_doubles(int32_t n, int32_t solver);
+int efe_csr_check_pattern(int32_t n, const int32_t* indptr, const int32_t* indices, int* flag, hipStream_t st,
+                          int* bad);
+int efe_csr_solve_ws(int32_t n, const int32_t* ip, const int32_t* ix, const double* v, const double* b, double* xout,
+                     const efe_solve_opts* o, efe_solve_info* info, double* ws, hipStream_t st);
 
 // GMRES (gmres.hip)
 int efe_gmres_solve(efe_ctx* c, const double* b, double* x, const efe_solve_opts* o, efe_solve_info* info);

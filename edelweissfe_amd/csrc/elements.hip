@@ -42,6 +42,7 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
                                                      const int32_t* __restrict__ conn,
                                                      const double* __restrict__ props, int propStride,
                                                      const double* __restrict__ dNdxi,
+                                                     const double* __restrict__ dJdxi,
                                                      const double* __restrict__ qpw,
                                                      const double* __restrict__ dU,
                                                      const double* __restrict__ sRef, double* __restrict__ sTmp,
@@ -50,24 +51,32 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
                                                      int* __restrict__ specStatus,
                                                      const int32_t* __restrict__ specOfElem,
                                                      const int32_t* __restrict__ cpos,
-                                                     const int* __restrict__ specActive, int formK) {
+                                                     const int* __restrict__ specActive, int formK,
+                                                     const int32_t* __restrict__ elist, int ecount, int ncw) {
     constexpr int NS = 12 + MatState<MAT>::n;
-    // elements past the end and, in batched contexts, of specimens the pass skips
-    auto skip = [&](int e) { return e >= E || (specActive && !specActive[specOfElem[e]]); };
+    const int e0 = blockIdx.x * EPW;
+    // Element el of this workgroup: entry e0 + el of the group's element list (elist, or all
+    // elements in order when it is null); -1 past the end and, in batched contexts, for
+    // specimens the pass skips.
+    auto elem = [&](int el) {
+        const int i = e0 + el;
+        if (i >= ecount) return -1;
+        const int e = elist ? elist[i] : i;
+        return (specActive && !specActive[specOfElem[e]]) ? -1 : e;
+    };
     constexpr int NP = (MAT == EFE_LINEAR_ELASTIC) ? 2 : 6;
-    constexpr int NC = (MAT == EFE_DRUCKER_PRAGER) ? 11 : 9;
     __shared__ double g_s[EPW][Q][NN][3];
     __shared__ KT fC_s[EPW][Q][36];
     __shared__ double fs_s[EPW][Q][6];
     const int lane = threadIdx.x;
-    const int e0 = blockIdx.x * EPW;
     const int64_t EQ = (int64_t)E * Q;
 
     for (int t = lane; t < EPW * Q; t += 64) {
         const int el = t / Q, q = t - (t / Q) * Q;
-        const int e = e0 + el;
-        if (skip(e)) continue;
+        const int e = elem(el);
+        if (e < 0) continue;
         const double* dn = dNdxi + (size_t)q * NN * 3;
+        const double* dj = dJdxi + (size_t)q * NN * 3;
         double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll 4
         for (int a = 0; a < NN; ++a) {
@@ -75,7 +84,7 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
             const double x = coords[nd], y = coords[N + nd], z = coords[2 * N + nd];
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
-                const double d = dn[a * 3 + r];
+                const double d = dj[a * 3 + r];
                 J[r * 3 + 0] += d * x;
                 J[r * 3 + 1] += d * y;
                 J[r * 3 + 2] += d * z;
@@ -131,13 +140,12 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
             *status = 1;
             if (specStatus) specStatus[specOfElem[e]] = 1;  // which specimen asked (batched contexts)
         }
-        if (MF && formK) {
+        if (MF && formK) {  // ncw = 9 [A, B, cnn, n] or 11 [A, B, cnn, cn1, c1n, n] (cn1 = c1n = 0 but DP)
             const double fq = det * qpw[q];
-            float* dc = mfC + (int64_t)q * NC * E + e;
+            float* dc = mfC + (int64_t)q * ncw * E + e;
+            for (int k = 0; k < ncw - 6; ++k) dc[(int64_t)k * E] = (float)(fq * cc[k]);
 #pragma unroll
-            for (int k = 0; k < NC - 6; ++k) dc[(int64_t)k * E] = (float)(fq * cc[k]);
-#pragma unroll
-            for (int k = 0; k < 6; ++k) dc[(int64_t)(NC - 6 + k) * E] = (float)cc[5 + k];
+            for (int k = 0; k < 6; ++k) dc[(int64_t)(ncw - 6 + k) * E] = (float)cc[5 + k];
         }
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
@@ -177,8 +185,8 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
             a = r0 / NN;
             b = r0 - a * NN;
         }
-        const int e = e0 + el;
-        if (skip(e)) continue;
+        const int e = elem(el);
+        if (e < 0) continue;
         KT acc[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) acc[k] = (KT)0;
@@ -214,8 +222,8 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
 
     for (int t = lane; t < EPW * NN; t += 64) {
         const int el = t / NN, a = t - (t / NN) * NN;
-        const int e = e0 + el;
-        if (skip(e)) continue;
+        const int e = elem(el);
+        if (e < 0) continue;
         double p0 = 0, p1 = 0, p2 = 0;
         for (int q = 0; q < Q; ++q) {
             const double ax = g_s[el][q][a][0], ay = g_s[el][q][a][1], az = g_s[el][q][a][2];
@@ -343,16 +351,20 @@ __global__ __launch_bounds__(256) void assemble_kernel(
 }
 
 // Body force (element.py:348-371): Pe = sum_q N_q (x) f detJ_q w_q with N at the table
-// point and detJ at the evaluation point; written to the element scratch, then gathered.
+// point and detJ of the element's J at the evaluation point (dJdxi); written to the
+// element scratch, then gathered.
+// One thread per listed element (elems[k], or element k when elems is NULL).
 template <int NN, int Q>
-__global__ __launch_bounds__(64) void body_force_kernel(int E, int N, const double* __restrict__ coords,
+__global__ __launch_bounds__(64) void body_force_kernel(int n, int E, int N, const int32_t* __restrict__ elems,
+                                                        const double* __restrict__ coords,
                                                         const int32_t* __restrict__ conn,
                                                         const double* __restrict__ dNdxi,
                                                         const double* __restrict__ shapeN,
                                                         const double* __restrict__ qpw, double fx, double fy,
                                                         double fz, double* __restrict__ Pscr) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= E) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int e = elems ? elems[k] : k;
     double acc[NN];
 #pragma unroll
     for (int a = 0; a < NN; ++a) acc[a] = 0.0;
@@ -490,11 +502,14 @@ __global__ __launch_bounds__(256) void surface_traction_kernel(int n, int E, int
 }
 
 // P and F of a residual-only pass: the element residuals of every node summed in element
-// order, as the assembly's gather sums them.
+// order, as the assembly's gather sums them.  Nodes of specimens the pass skips (batched
+// contexts) keep P and F, as in the assembly.
 __global__ void pf_gather_kernel(int N, const int32_t* __restrict__ nptr, const int32_t* __restrict__ nlist,
-                                 const double* __restrict__ Pscr, double* __restrict__ P, double* __restrict__ F) {
+                                 const double* __restrict__ Pscr, double* __restrict__ P, double* __restrict__ F,
+                                 const int* __restrict__ specActive, const int32_t* __restrict__ specOfNode) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
+    if (specActive && !specActive[specOfNode[n]]) return;
     double p0 = 0, p1 = 0, p2 = 0, f0 = 0, f1 = 0, f2 = 0;
     for (int c = nptr[n]; c < nptr[n + 1]; ++c) {
         const double* src = Pscr + (int64_t)nlist[c] * 3;
@@ -548,17 +563,20 @@ __global__ void export_csr_kernel(int N, const int32_t* __restrict__ bptr, const
 
 template <int NN, int Q, int MAT, typename KT, int EPW>
 void launch_elem_t(efe_ctx* c, const double* dU) {
-    const int grid = (c->E + EPW - 1) / EPW;
+    const int grid = (c->curCount + EPW - 1) / EPW;
+    if (grid == 0) return;
     const int propStride = c->props_per_element ? c->E : 0;
     const int* act = c->useActive ? c->specActive : nullptr;
     if (c->mfPass)
         element_kernel<NN, Q, MAT, float, EPW, true><<<grid, 64, 0, c->stream>>>(
-            c->E, c->N, c->coords, c->conn, c->props, propStride, c->dNdxi, c->qpw, dU, c->stateRef, c->stateTmp,
-            (float*)c->Kscr, c->Pscr, c->status, c->mfC, c->specStatus, c->specOfElem, c->cpos, act, c->formK);
+            c->E, c->N, c->coords, c->conn, c->props, propStride, c->dNdxi, c->dJdxi, c->qpw, dU, c->stateRef, c->stateTmp,
+            (float*)c->Kscr, c->Pscr, c->status, c->mfC, c->specStatus, c->specOfElem, c->cpos, act, c->formK,
+            c->curList, c->curCount, c->mfNC);
     else
         element_kernel<NN, Q, MAT, KT, EPW, false><<<grid, 64, 0, c->stream>>>(
-            c->E, c->N, c->coords, c->conn, c->props, propStride, c->dNdxi, c->qpw, dU, c->stateRef, c->stateTmp,
-            (KT*)c->Kscr, c->Pscr, c->status, nullptr, c->specStatus, c->specOfElem, c->cpos, act, c->formK);
+            c->E, c->N, c->coords, c->conn, c->props, propStride, c->dNdxi, c->dJdxi, c->qpw, dU, c->stateRef, c->stateTmp,
+            (KT*)c->Kscr, c->Pscr, c->status, nullptr, c->specStatus, c->specOfElem, c->cpos, act, c->formK,
+            c->curList, c->curCount, c->mfNC);
 }
 
 template <int NN, int Q, int MAT, typename KT>
@@ -585,7 +603,7 @@ int launch_elem_epw(efe_ctx* c, const double* dU) {
 
 template <int NN, int Q, typename KT>
 int launch_elem_mat(efe_ctx* c, const double* dU) {
-    switch (c->material) {
+    switch (c->curMat) {
         case EFE_LINEAR_ELASTIC: return launch_elem_epw<NN, Q, EFE_LINEAR_ELASTIC, KT>(c, dU);
         case EFE_VONMISES: return launch_elem_epw<NN, Q, EFE_VONMISES, KT>(c, dU);
         case EFE_DRUCKER_PRAGER: return launch_elem_epw<NN, Q, EFE_DRUCKER_PRAGER, KT>(c, dU);
@@ -609,9 +627,14 @@ int launch_elem_type(efe_ctx* c, const double* dU) {
 
 int efe_launch_elements(efe_ctx* c, const double* dU, int precision) {
     c->mfPass = precision == EFE_FP32_MATFREE;
-    int rc = (precision == EFE_FP64) ? launch_elem_type<double>(c, dU) : launch_elem_type<float>(c, dU);
-    if (rc != EFE_OK) return rc;
-    EFE_HIP(hipGetLastError());
+    for (int g = 0; g < c->ngroups; ++g) {  // one launch per material group (section)
+        c->curMat = c->groupMat[g];
+        c->curList = c->groupElems[g];
+        c->curCount = c->groupCount[g];
+        int rc = (precision == EFE_FP64) ? launch_elem_type<double>(c, dU) : launch_elem_type<float>(c, dU);
+        if (rc != EFE_OK) return rc;
+        EFE_HIP(hipGetLastError());
+    }
     return EFE_OK;
 }
 
@@ -620,7 +643,9 @@ int efe_launch_residual(efe_ctx* c, const double* dU, double* P, double* F, int 
     const int rc = efe_launch_elements(c, dU, precision);  // the precision's launch shape; same residual
     c->formK = 1;
     if (rc != EFE_OK) return rc;
-    pf_gather_kernel<<<(c->N + 255) / 256, 256, 0, c->stream>>>(c->N, c->nptr, c->nlist, c->Pscr, P, F);
+    pf_gather_kernel<<<(c->N + 255) / 256, 256, 0, c->stream>>>(c->N, c->nptr, c->nlist, c->Pscr, P, F,
+                                                                 c->useActive ? c->specActive : nullptr,
+                                                                 c->specOfNode);
     EFE_HIP(hipGetLastError());
     return EFE_OK;
 }
@@ -645,19 +670,22 @@ int efe_launch_assembly(efe_ctx* c, double* P, double* F, const double* dir_valu
     return EFE_OK;
 }
 
-int efe_launch_body_force(efe_ctx* c, const double* f3, double* PExt) {
-    const int grid = (c->E + 63) / 64;
-    switch (c->eltype) {
+int efe_launch_body_force(efe_ctx* c, const int32_t* elems, int n, const double* f3, double* PExt) {
+    // a subset leaves the other elements' slots at zero
+    if (elems) EFE_HIP(hipMemsetAsync(c->Pscr, 0, (size_t)c->E * c->NN * 3 * sizeof(double), c->stream));
+    c->pscrValid = false;
+    const int grid = (n + 63) / 64;
+    if (n > 0) switch (c->eltype) {
         case EFE_C3D8:
-            body_force_kernel<8, 8><<<grid, 64, 0, c->stream>>>(c->E, c->N, c->coords, c->conn, c->dNdxi,
+            body_force_kernel<8, 8><<<grid, 64, 0, c->stream>>>(n, c->E, c->N, elems, c->coords, c->conn, c->dJdxi,
                                                                c->shapeN, c->qpw, f3[0], f3[1], f3[2], c->Pscr);
             break;
         case EFE_C3D20:
-            body_force_kernel<20, 27><<<grid, 64, 0, c->stream>>>(c->E, c->N, c->coords, c->conn, c->dNdxi,
+            body_force_kernel<20, 27><<<grid, 64, 0, c->stream>>>(n, c->E, c->N, elems, c->coords, c->conn, c->dJdxi,
                                                                  c->shapeN, c->qpw, f3[0], f3[1], f3[2], c->Pscr);
             break;
         case EFE_C3D20R:
-            body_force_kernel<20, 8><<<grid, 64, 0, c->stream>>>(c->E, c->N, c->coords, c->conn, c->dNdxi,
+            body_force_kernel<20, 8><<<grid, 64, 0, c->stream>>>(n, c->E, c->N, elems, c->coords, c->conn, c->dJdxi,
                                                                 c->shapeN, c->qpw, f3[0], f3[1], f3[2], c->Pscr);
             break;
         default:
@@ -672,6 +700,7 @@ int efe_launch_body_force(efe_ctx* c, const double* f3, double* PExt) {
 int efe_launch_surface_traction(efe_ctx* c, const int32_t* elems, const uint8_t* fmask, int n, const double* t3,
                                 double* PExt) {
     EFE_HIP(hipMemsetAsync(c->Pscr, 0, (size_t)c->E * c->NN * 3 * sizeof(double), c->stream));
+    c->pscrValid = false;
     const int grid = (n + 255) / 256;
     if (n > 0) {
         if (c->NN == 8)

@@ -40,6 +40,7 @@ struct GmresWork {
     double* hhost = nullptr; // pinned
     hipStream_t ks = nullptr;
     hipEvent_t evA = nullptr, evB = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr, t2 = nullptr;  // timing: solve start, factor done, end
 };
 
 namespace {
@@ -476,6 +477,7 @@ int gmres_solve(efe_ctx* c, GmresWork* g, const double* b, double* xout, const e
     const int D = c->D, N = c->N, m = g->m;
     EFE_HIP(hipEventRecord(g->evA, c->stream));
     EFE_HIP(hipStreamWaitEvent(st, g->evA, 0));
+    EFE_HIP(hipEventRecord(g->t0, st));
     T *F = nullptr, *Dinv = nullptr;
     if (o->precond == EFE_PRECOND_ILU0) {
         if (!g->ilu_ready) {
@@ -489,6 +491,7 @@ int gmres_solve(efe_ctx* c, GmresWork* g, const double* b, double* xout, const e
         F = Fr;
         Dinv = Dr;
     }
+    EFE_HIP(hipEventRecord(g->t1, st));
     T* V = (T*)g->V;
     T* w = (T*)g->w;
     T* z = (T*)g->z;
@@ -587,12 +590,21 @@ int gmres_solve(efe_ctx* c, GmresWork* g, const double* b, double* xout, const e
         if (!(beta == beta)) bad = true;
         if (beta <= tol) done = true;
     }
+    EFE_HIP(hipEventRecord(g->t2, st));
     finish_kernel<T><<<gd, BS, 0, st>>>(D, x, b, c->dirflag, c->ndir, xout);
     EFE_HIP(hipGetLastError());
     EFE_HIP(hipEventRecord(g->evB, st));
     EFE_HIP(hipStreamWaitEvent(c->stream, g->evB, 0));
     EFE_HIP(hipStreamSynchronize(st));
     info->iterations = it;
+    float fms = 0, ims = 0;
+    hipEventElapsedTime(&fms, g->t0, g->t1);
+    hipEventElapsedTime(&ims, g->t1, g->t2);
+    c->t_kry = it > 0 ? ims / it : 0.0;
+    c->tot[EFE_T_KRYLOV] += ims;
+    c->tot[EFE_T_KRYLOV_ITERS] += it;
+    c->tot[EFE_T_SOLVES] += 1;
+    c->tot[EFE_T_PRECOND] += fms;
     if (bad) info->status = EFE_NAN;
     else if (!done) info->status = EFE_NOT_CONVERGED;
     return EFE_OK;
@@ -608,6 +620,9 @@ int ensure_gmres(efe_ctx* c, int m, int prec) {
         EFE_HIP(hipStreamCreateWithFlags(&g->ks, hipStreamNonBlocking));
         EFE_HIP(hipEventCreateWithFlags(&g->evA, hipEventDisableTiming));
         EFE_HIP(hipEventCreateWithFlags(&g->evB, hipEventDisableTiming));
+        EFE_HIP(hipEventCreate(&g->t0));
+        EFE_HIP(hipEventCreate(&g->t1));
+        EFE_HIP(hipEventCreate(&g->t2));
         g->nwg = 512;
         EFE_HIP(hipHostMalloc(&g->hhost, 3 * 258 * sizeof(double), hipHostMallocDefault));
     }
@@ -654,6 +669,8 @@ void efe_gmres_free(efe_ctx* c) {
     if (g->ks) hipStreamDestroy(g->ks);
     if (g->evA) hipEventDestroy(g->evA);
     if (g->evB) hipEventDestroy(g->evB);
+    for (hipEvent_t e : {g->t0, g->t1, g->t2})
+        if (e) hipEventDestroy(e);
     delete g;
     c->gw = nullptr;
 }

@@ -622,6 +622,9 @@ int pcg_solve(efe_ctx* c, const double* b, double* xout, const efe_solve_opts* o
     float ms = 0;
     hipEventElapsedTime(&ms, w->t0, w->t1);
     c->t_kry = it > 0 ? ms / it : 0.0;
+    c->tot[EFE_T_KRYLOV] += ms;
+    c->tot[EFE_T_KRYLOV_ITERS] += it;
+    c->tot[EFE_T_SOLVES] += 1;
     return EFE_OK;
 }
 

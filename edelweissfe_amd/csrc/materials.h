@@ -200,6 +200,13 @@ __device__ __forceinline__ int dp_update(const double* p, double s[6], double& e
             for (int i = 0; i < 6; ++i) cc[5 + i] = n[i];
         }
     } else {
+        // beyond the apex: with zero dilatancy (etab = 0) the flow is purely deviatoric and no
+        // return to the apex exists; report it like a failed local Newton (a cutback request)
+        if (!(etab > 0.0)) {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) s[i] = t[i];
+            return 2;
+        }
         const double alpha = xi / etab, beta = xi / eta;
         const double dv = (ptr - beta * coh) / (alpha * beta * H + K);
         const double pn = ptr - K * dv;

@@ -142,17 +142,19 @@ __global__ __launch_bounds__(256) void mf_apply_kernel(int E, int N, const int32
     }
     for (int q = 0; q < Q; ++q) {
         const float* dn = dnxi + q * NN * 3;
+        const float* dj = dnxi + (Q + q) * NN * 3;  // the table J is formed with
         float H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // H[i*3+k] = sum_a p_a,i dN_a/dxi_k
-        float J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // J[k*3+c] = sum_a dN_a/dxi_k x_a,c
+        float J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // J[k*3+c] = sum_a dJ_a/dxi_k x_a,c
 #pragma unroll
         for (int a = 0; a < NN; ++a) {
             const float d[3] = {dn[a * 3], dn[a * 3 + 1], dn[a * 3 + 2]};
+            const float dk[3] = {dj[a * 3], dj[a * 3 + 1], dj[a * 3 + 2]};
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     H[i * 3 + k] += pe[a][i] * d[k];
-                    J[k * 3 + i] += d[k] * xe[a][i];
+                    J[k * 3 + i] += dk[k] * xe[a][i];
                 }
         }
         const float det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
@@ -521,7 +523,7 @@ template <typename TP>
 void launch_apply(efe_ctx* c, hipStream_t st, const TP* p, int mask, const int* flags,
                   const int* active = nullptr) {
     const int grid = round8_mf((c->E + 255) / 256);
-    const bool dp = c->material == EFE_DRUCKER_PRAGER;
+    const bool dp = c->mfNC == 11;
     switch (c->eltype) {
         case EFE_C3D8:
             if (dp) mf_apply_hex8_kernel<11, TP><<<grid, 256, 0, st>>>(MF_ARGS);
@@ -548,21 +550,23 @@ void launch_gather_op(efe_ctx* c, hipStream_t st, const float* p, float* q, doub
 }  // namespace
 
 int efe_mf_alloc(efe_ctx* c) {
-    const int NC = c->material == EFE_DRUCKER_PRAGER ? 11 : 9;
+    const int NC = c->mfNC;
     if (c->mfC) return EFE_OK;
     const size_t EQ = (size_t)c->E * c->Q;
     const int64_t L = (int64_t)c->E * c->NN;
     EFE_HIP(hipMalloc(&c->mfC, EQ * NC * sizeof(float)));
     EFE_HIP(hipMalloc(&c->mfF, (size_t)L * 3 * sizeof(float)));
     if (!c->Minv32) EFE_HIP(hipMalloc(&c->Minv32, (size_t)c->N * 9 * sizeof(float)));
-    // FP32 copy of the natural shape-function derivatives at the evaluation points
+    // FP32 copies of the natural shape-function derivatives at the evaluation points and of
+    // the table J is formed with, one after the other
     const size_t nd = (size_t)c->Q * c->NN * 3;
-    std::vector<double> h(nd);
-    std::vector<float> hf(nd);
+    std::vector<double> h(2 * nd);
+    std::vector<float> hf(2 * nd);
     EFE_HIP(hipMemcpy(h.data(), c->dNdxi, nd * sizeof(double), hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < nd; ++i) hf[i] = (float)h[i];
-    EFE_HIP(hipMalloc(&c->dnxi32, nd * sizeof(float)));
-    EFE_HIP(hipMemcpy(c->dnxi32, hf.data(), nd * sizeof(float), hipMemcpyHostToDevice));
+    EFE_HIP(hipMemcpy(h.data() + nd, c->dJdxi, nd * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < 2 * nd; ++i) hf[i] = (float)h[i];
+    EFE_HIP(hipMalloc(&c->dnxi32, 2 * nd * sizeof(float)));
+    EFE_HIP(hipMemcpy(c->dnxi32, hf.data(), 2 * nd * sizeof(float), hipMemcpyHostToDevice));
     return EFE_OK;
 }
 

@@ -15,6 +15,9 @@ from ._lib import check
 
 class DeviceSystem:
     def __init__(self, coords, conn, el_type, material, props, specimen_of_node=None, n_specimens=1, device=None):
+        """material: a material name for all elements, or one name per element (sections);
+        props: one row, or one row per element (with several materials: 6 values per row,
+        linear elastic reading the first 2)."""
         if not torch.cuda.is_available():
             raise RuntimeError("DeviceSystem needs a GPU (torch.cuda.is_available() is False)")
         self.lib = _lib.load()
@@ -23,13 +26,33 @@ class DeviceSystem:
         conn = np.ascontiguousarray(conn, dtype=np.int32)
         props = np.ascontiguousarray(props, dtype=np.float64)
         self.el_type = el_type.upper()
-        self.material = material.lower()
         self.n_nodes = coords.shape[0]
         self.n_elements = conn.shape[0]
         if self.el_type not in _lib.ELEMENT_TYPES:
             raise ValueError("element type {} not supported (C3D8, C3D20, C3D20R)".format(el_type))
-        if self.material not in _lib.MATERIALS:
-            raise ValueError("material {} not supported".format(material))
+        mat_of = None
+        if isinstance(material, str):
+            self.material = material.lower()
+            if self.material not in _lib.MATERIALS:
+                raise ValueError("material {} not supported".format(material))
+        else:  # several sections: a material per element
+            names = [str(m).lower() for m in np.asarray(material).ravel()]
+            if len(names) != self.n_elements:
+                raise ValueError("one material name per element needed")
+            bad = sorted(set(names) - set(_lib.MATERIALS))
+            if bad:
+                raise ValueError("materials {} not supported".format(bad))
+            self.material = names[0] if len(set(names)) == 1 else "sections"
+            self.materials = names
+            mat_of = np.ascontiguousarray([_lib.MATERIALS[m] for m in names], dtype=np.int32)
+            full = np.zeros((self.n_elements, 6))
+            p2 = np.atleast_2d(props)
+            if p2.shape[0] == 1:
+                p2 = np.repeat(p2, self.n_elements, axis=0)
+            if p2.shape[0] != self.n_elements:
+                raise ValueError("per-element properties need one row per element")
+            full[:, :min(6, p2.shape[1])] = p2[:, :6]
+            props = np.ascontiguousarray(full)
         nn = 8 if self.el_type == "C3D8" else 20
         if conn.shape[1] != nn:
             raise ValueError("connectivity has {} nodes per element, {} needs {}".format(conn.shape[1], el_type, nn))
@@ -41,14 +64,15 @@ class DeviceSystem:
         spec = None
         if specimen_of_node is not None:
             spec = np.ascontiguousarray(specimen_of_node, dtype=np.int32)
-        self._keep = (coords, conn, props, spec)
+        self._keep = (coords, conn, props, spec, mat_of)
         stream = torch.cuda.current_stream(self.device).cuda_stream
         desc = _lib.MeshDesc(
             n_nodes=self.n_nodes, n_elements=self.n_elements, element_type=_lib.ELEMENT_TYPES[self.el_type],
-            material=_lib.MATERIALS[self.material], coords=coords.ctypes.data, conn=conn.ctypes.data,
-            n_props=props.shape[-1], props_per_element=1 if per_el else 0, props=props.ctypes.data,
-            specimen_of_node=spec.ctypes.data if spec is not None else None, n_specimens=n_specimens,
-            stream=stream)
+            material=_lib.MATERIALS.get(self.material, 0), coords=coords.ctypes.data, conn=conn.ctypes.data,
+            n_props=props.shape[-1], props_per_element=1 if (per_el or mat_of is not None) else 0,
+            props=props.ctypes.data, specimen_of_node=spec.ctypes.data if spec is not None else None,
+            n_specimens=n_specimens, stream=stream,
+            material_of_element=mat_of.ctypes.data if mat_of is not None else None)
         self.ctx = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             check(self.lib.efe_ctx_create(ctypes.byref(desc), ctypes.byref(self.ctx)), "efe_ctx_create")
@@ -110,9 +134,14 @@ class DeviceSystem:
         check(rc, "efe_compute_residual")
         return rc, pnew.value
 
-    def body_force(self, force, PExt):
+    def body_force(self, force, PExt, elements=None):
+        """Body force (3,) per volume on `elements` (all if None), added into PExt."""
         f = (ctypes.c_double * 3)(*[float(v) for v in force])
-        check(self.lib.efe_body_force(self.ctx, f, self._ptr(PExt)), "efe_body_force")
+        if elements is None:
+            check(self.lib.efe_body_force(self.ctx, None, 0, f, self._ptr(PExt)), "efe_body_force")
+            return
+        el = np.ascontiguousarray(elements, dtype=np.int32)
+        check(self.lib.efe_body_force(self.ctx, el.ctypes.data, int(el.size), f, self._ptr(PExt)), "efe_body_force")
 
     def surface_traction(self, surface, traction, PExt):
         """Uniform traction (3,) on a surface {face ID (1..6): element indices}, added into PExt."""
@@ -230,6 +259,29 @@ class DeviceSystem:
         check(self.lib.efe_set_launch(self.ctx, int(elements_per_workgroup), int(krylov_block)), "efe_set_launch")
 
     def phase_times(self):
-        out = (ctypes.c_double * 4)()
+        """Device ms of the last call of each phase (the autotuner's probes)."""
+        out = (ctypes.c_double * 16)()
         check(self.lib.efe_get_phase_times(self.ctx, out), "efe_get_phase_times")
         return {"elements": out[0], "assembly": out[1], "krylov_iteration": out[2], "norms": out[3]}
+
+    def phase_totals(self):
+        """Device time totals since the system was created and the history averages (ms per
+        call): element kernel and assembly per tangent pass, residual-only pass, Krylov
+        iteration (CG or GMRES, without preconditioner set-up), ILU(0) factorisation per solve,
+        norms per call."""
+        out = (ctypes.c_double * 16)()
+        check(self.lib.efe_get_phase_times(self.ctx, out), "efe_get_phase_times")
+        t = list(out[4:15])
+        tot = dict(zip(("elements_ms", "element_passes", "assembly_ms", "residual_ms", "residual_passes",
+                        "krylov_ms", "krylov_iterations", "solves", "ilu_factor_ms", "norms_ms", "norm_calls"), t))
+
+        def avg(a, n):
+            return tot[a] / tot[n] if tot[n] else 0.0
+
+        tot["avg_ms"] = {"elements": avg("elements_ms", "element_passes"),
+                         "assembly": avg("assembly_ms", "element_passes"),
+                         "residual_pass": avg("residual_ms", "residual_passes"),
+                         "krylov_iteration": avg("krylov_ms", "krylov_iterations"),
+                         "ilu_factor": avg("ilu_factor_ms", "solves"),
+                         "norms": avg("norms_ms", "norm_calls")}
+        return tot

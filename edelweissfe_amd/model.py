@@ -1,8 +1,8 @@
-"""Model tree of one homogeneous hexahedral element set.
+"""Model tree of one hexahedral element set.
 
-Plays the part of edelweissfe/models/femodel.py:45-298 for the hot path:
-node coordinates, one element type with one material (optionally with
-per-element properties, as sections/sectionbase.py:114-128 allow), node and
+Plays the part of edelweissfe/models/femodel.py:45-298 for the hot path: node coordinates,
+one element type with one material or several (sections, sections/base/sectionbase.py:86-128,
+and per-element properties as materialParameterFromField gives them, :114-128), node and
 element sets, the model time and the converged node-field values U / P / dU
 (nonlinearimplicitstatic.py:288-291).  DOF numbering follows
 dofmanager.py:259-300 for a single displacement field: DOF 3 p + c for the node
@@ -51,10 +51,48 @@ class FEModel:
     def nElements(self):
         return self.connectivity.shape[0]
 
-    def assignMaterial(self, name, properties):
-        """Material by name (linearelastic, vonmises, druckerprager) and properties:
-        one row for all elements or one row per element."""
-        self.material = {"name": name.lower(), "properties": np.asarray(properties, dtype=np.float64)}
+    def assignMaterial(self, name, properties, elementSet=None):
+        """Material by name (linearelastic, vonmises, druckerprager) and properties (one row, or
+        one row per element it covers) on every element, or on the elements of `elementSet` (a
+        set name or element indices): a section (sections/base/sectionbase.py:86-128), a later
+        assignment replacing an earlier one on the same elements.  With several materials,
+        model.material holds one name and one row of 6 properties per element."""
+        props = np.asarray(properties, dtype=np.float64)
+        if elementSet is None:
+            self.material = {"name": name.lower(), "properties": props}
+            return
+        E = self.nElements
+        els = np.asarray(self.elementSets[elementSet] if isinstance(elementSet, str) else elementSet, dtype=np.int64)
+        names, rows = self._perElementMaterial()
+        r = np.atleast_2d(props)
+        if r.shape[0] == 1:
+            r = np.repeat(r, els.size, axis=0)
+        if r.shape[0] != els.size:
+            raise ValueError("one row of properties, or one per element of the set")
+        names[els] = name.lower()
+        rows[els] = 0.0
+        rows[els, :r.shape[1]] = r
+        if (names == names[0]).all() and names[0]:
+            n = 2 if names[0] == "linearelastic" else 6
+            self.material = {"name": str(names[0]), "properties": rows[:, :n].copy() if E else rows}
+        else:
+            self.material = {"name": names, "properties": rows}
+
+    def _perElementMaterial(self):
+        E = self.nElements
+        names = np.full(E, "", dtype=object)
+        rows = np.zeros((E, 6))
+        if self.material is not None:
+            names[:] = self.material["name"]
+            p = np.atleast_2d(self.material["properties"])
+            rows[:, :p.shape[1]] = p if p.shape[0] == E else np.repeat(p, E, axis=0)
+        return names, rows
+
+    def materialNames(self):
+        """The material names present (one for a homogeneous model)."""
+        if self.material is None:
+            return []
+        return sorted(set(np.atleast_1d(self.material["name"]).tolist()))
 
     def dofsOfNodeSet(self, nSet, components=(0, 1, 2)):
         nodes = self.nodeSets[nSet]

@@ -78,6 +78,61 @@ FLUX_RESIDUAL_TOL = 1e-8         # phenomena.py fluxResidualTolerance["displacem
 FLUX_RESIDUAL_TOL_ALT = 5e-3     # phenomena.py fluxResidualToleranceAlternative["displacement"]
 
 
+def _cutbackMessage(model):
+    """Text of the CutbackRequest a failing material raises: the von Mises local Newton
+    (vonmises.py:230-231), or a Drucker-Prager trial state beyond the apex of a cone with
+    zero dilatancy, which has no return."""
+    names = model.materialNames()
+    msgs = []
+    if "vonmises" in names:
+        msgs.append("Von Mises Newton failed.")
+    if "druckerprager" in names:
+        msgs.append("Drucker-Prager: trial stress beyond the apex with zero dilatancy, no return.")
+    return " / ".join(msgs) or "Material update failed."
+
+
+def _convertOption(default, value):
+    """An option value in the type of its default; strings as the reference's option step
+    action holds them (stepactions/options.py) convert too, "False" / "0" to False."""
+    if isinstance(default, bool) and isinstance(value, str):
+        v = value.strip().lower()
+        if v in ("true", "1", "yes", "on"):
+            return True
+        if v in ("false", "0", "no", "off"):
+            return False
+        raise ValueError("invalid boolean option value {:}".format(value))
+    return type(default)(value)
+
+
+_DIGESTS = []  # (array, digest) of recently hashed arrays, the arrays kept alive
+
+
+def _arrayDigest(a):
+    """Content digest of an array, remembered for the same (unchanged-identity) array object
+    so that following steps on one mesh do not hash it again."""
+    import hashlib
+
+    for arr, d in _DIGESTS:
+        if arr is a:
+            return d
+    c = np.ascontiguousarray(a)
+    h = hashlib.blake2b(digest_size=16)
+    h.update((str(c.shape) + str(c.dtype)).encode())
+    h.update(c.tobytes())
+    d = h.hexdigest()
+    _DIGESTS.insert(0, (a, d))
+    del _DIGESTS[16:]
+    return d
+
+
+def _fingerprint(model):
+    """Identity of what a device system is built from: coordinates, connectivity, element
+    type, materials and properties."""
+    names = "|".join(np.atleast_1d(model.material["name"]).astype(str).tolist())
+    return (_arrayDigest(model.coordinates), _arrayDigest(model.connectivity),
+            _arrayDigest(np.asarray(model.material["properties"])), model.elementType, names)
+
+
 class NISTGPU:
     identification = "NISTGPUSolver"
 
@@ -118,19 +173,35 @@ class NISTGPU:
         self.newtonChecks = 0      # element evaluations of the reference: one per Newton iteration
 
     def _updateOptions(self, updated):
+        """updated: a dict, or the reference's option step action (its `options` dict,
+        stepactions/options.py; nonlinearimplicitstatic.py:168-171)."""
+        updated = getattr(updated, "options", updated)
         for k, v in updated.items():
             if k not in self.NISTOptions:
                 raise AttributeError("Invalid option {:} for {:}".format(k, self.identification))
-            self.options[k] = type(self.NISTOptions[k])(v)
+            self.options[k] = _convertOption(self.NISTOptions[k], v)
 
     # ---- setup ----------------------------------------------------------------------------------------
     def _setupSystem(self, model):
+        """The device system of `model`: reused while coordinates, connectivity, element type,
+        materials and properties are the same (its state then continues when the model is the
+        one it last solved, or is set from the model's QP state, or starts at zero), rebuilt
+        otherwise."""
         mat = model.material
         if mat is None:
             raise ValueError("model has no material")
-        if self.system is None or self.system.n_nodes != model.nNodes:
+        names = np.atleast_1d(mat["name"]).astype(str)
+        if (names == "").any():
+            raise ValueError("elements without a material (no section covers them)")
+        key = _fingerprint(model)
+        if self.system is not None and getattr(self, "_systemKey", None) == "attached":
+            self._systemKey = key  # attached without a model: the system is this model's
+        if self.system is None or getattr(self, "_systemKey", None) != key:
+            if self.system is not None:
+                self.system.close()
             self.system = DeviceSystem(model.coordinates, model.connectivity, model.elementType, mat["name"],
                                        mat["properties"])
+            self._systemKey = key
             if model._qpState is not None:
                 self.system.set_qp_state(model._qpState)
         elif model.deviceSystem is self.system:
@@ -141,9 +212,11 @@ class NISTGPU:
             self.system.reset_state()
         return self.system
 
-    def attach(self, system):
-        """Use an already created DeviceSystem (mesh, pattern and buffers resident on the GPU)."""
+    def attach(self, system, model=None):
+        """Use an already created DeviceSystem (mesh, pattern and buffers resident on the GPU);
+        with `model`, the system is the one for that model's mesh and materials."""
         self.system = system
+        self._systemKey = _fingerprint(model) if model is not None else "attached"
 
     @property
     def _mixed(self):
@@ -309,7 +382,7 @@ class NISTGPU:
                 self.elementPasses += 1
             self.computationTimes["elements"] += _time.perf_counter() - tic
             if st == _lib.EFE_CUTBACK:
-                raise CutbackRequest("Von Mises Newton failed.", pnew)
+                raise CutbackRequest(_cutbackMessage(model), pnew)
             tic = _time.perf_counter()
             torch.add(P, PExt, out=R)
             if first:
@@ -337,7 +410,7 @@ class NISTGPU:
                 self.computationTimes["elements"] += _time.perf_counter() - tic
                 self.elementPasses += 1
                 if st == _lib.EFE_CUTBACK:
-                    raise CutbackRequest("Von Mises Newton failed.", pnew)
+                    raise CutbackRequest(_cutbackMessage(model), pnew)
             b = R - corr if first else R
             ddU = torch.empty(D, dtype=torch.float64, device=dev)
             tic = _time.perf_counter()
@@ -400,18 +473,19 @@ class NISTGPU:
         for nf in step.actions.get("nodeforces", {}).values():
             nodes = model.nodeSets[nf.nSet]
             idx = torch.as_tensor(model.dofsOfNodeSet(nf.nSet), device=sysm.device)
-            load = torch.as_tensor(np.tile(nf.getCurrentLoad(timeStep), (len(nodes), 1)).ravel(),
-                                   dtype=torch.float64, device=sysm.device)
+            cur = np.asarray(nf.getCurrentLoad(timeStep), dtype=float)
+            if cur.ndim == 1:  # one vector for every node of the set
+                cur = np.tile(cur, (len(nodes), 1))
+            load = torch.as_tensor(cur.ravel(), dtype=torch.float64, device=sysm.device)
             PExt[idx] += load
         tic = _time.perf_counter()
         for dl in step.actions.get("distributedload", {}).values():
             sysm.surface_traction(model.surfaces[dl.surface], dl.getCurrentLoad(timeStep), PExt)
         toc = _time.perf_counter()
         self.computationTimes["distributed loads"] += toc - tic
-        for bf in step.actions.get("bodyforce", {}).values():
-            if len(model.elementSets[bf.elSet]) != model.nElements:
-                raise NotImplementedError("body forces on element subsets")
-            sysm.body_force(bf.getCurrentLoad(timeStep), PExt)
+        for bf in step.actions.get("bodyforce", {}).values():  # computeBodyForces (:510-557)
+            els = model.elementSets[bf.elSet]
+            sysm.body_force(bf.getCurrentLoad(timeStep), PExt, None if len(els) == model.nElements else els)
         self.computationTimes["body forces"] += _time.perf_counter() - toc
         return PExt
 

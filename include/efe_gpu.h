@@ -58,6 +58,7 @@ extern "C" {
 
 #define EFE_SOLVER_CG 0
 #define EFE_SOLVER_GMRES 1
+#define EFE_SOLVER_BICGSTAB 2  /* scalar CSR solves (efe_csr_linsolve) only */
 #define EFE_PRECOND_BLOCK_JACOBI 0
 #define EFE_PRECOND_ILU0 1
 
@@ -76,6 +77,11 @@ typedef struct {
     const int32_t* specimen_of_node; /* host [n_nodes] or NULL: independent specimens (block-diagonal system) */
     int32_t n_specimens;        /* 1 when specimen_of_node is NULL */
     void* stream;               /* hipStream_t or NULL for the default stream */
+    /* Several sections (sections/base/sectionbase.py:86-128 assigns a material to each element
+     * set): host [n_elements] material of every element, or NULL (every element `material`).
+     * When given, `material` is ignored, props must be per element with n_props = 6 (linear
+     * elastic reads the first 2), and the element pass runs once per material present. */
+    const int32_t* material_of_element;
 } efe_mesh_desc;
 
 typedef struct {
@@ -146,8 +152,11 @@ int efe_compute_elements(efe_ctx* ctx, const double* U, const double* dU, const 
 int efe_compute_residual(efe_ctx* ctx, const double* U, const double* dU, const double time[2], double dT,
                          double* P, double* F, int32_t precision, double* pNewDT);
 
-/* External body-force vector (element.py:348-371), added into PExt (device [n_dof]). */
-int efe_body_force(efe_ctx* ctx, const double* force3, double* PExt);
+/* External body-force vector of a force per volume force3 (host [3]) on the elements
+ * elements[i] (host [n]; NULL: all elements), added into PExt (device [n_dof]):
+ * NIST.computeBodyForces (nonlinearimplicitstatic.py:510-557) over bForce.elementSet with the
+ * element's computeBodyForce (element.py:348-371).  An element listed twice counts once. */
+int efe_body_force(efe_ctx* ctx, const int32_t* elements, int32_t n, const double* force3, double* PExt);
 
 /* Uniform surface traction traction3 (host [3], force per area) on the faces faces[i]
  * (1..6, Abaqus numbering by local node number, as the box generator's surfaces,
@@ -219,6 +228,27 @@ int efe_set_active_specimens(efe_ctx* ctx, const uint8_t* active);
 /* Number of specimens of the context (1 for an ordinary context). */
 int efe_get_specimen_count(efe_ctx* ctx);
 
+/* The linear-solver plugin contract f(A: csr_matrix, b) -> x (config/linsolve.py:46-86, called
+ * by NIST.linearSolve, nonlinearimplicitstatic.py:727-751) on the GPU, for a scalar CSR matrix
+ * as NIST hands it over: K after applyDirichletK (nonlinearimplicitstatic.py:559-593), whose
+ * constrained rows are unit rows while their columns stay.  A row whose only stored entry is a
+ * unit diagonal is prescribed: x = b there, and its column moves to the right-hand side, so the
+ * Krylov method solves the free block (symmetric when K is) K_ff x_f = b_f - K_fd b_d.
+ * opts->solver: EFE_SOLVER_CG (symmetric K) or EFE_SOLVER_BICGSTAB; Jacobi preconditioner
+ * (opts->precond and precision are ignored: FP64); stop ||r_f||_2 <= max(rtol ||b_f'||, atol).
+ * indptr host [n+1] (0 first, non-decreasing), indices host [indptr[n]] in [0, n), data host
+ * [indptr[n]], b and x host [n].  Allocates device memory for the call and frees it. */
+int efe_csr_linsolve(int32_t n, const int32_t* indptr, const int32_t* indices, const double* data, const double* b,
+                     double* x, const efe_solve_opts* opts, efe_solve_info* info);
+
+/* efe_csr_linsolve on device arrays (indptr, indices, data, b, x device; x may not alias b),
+ * enqueued on `stream` (hipStream_t, NULL = default); the pattern is validated on the device
+ * first (EFE_ERR_ARG when indptr or an index is out of range).  Allocates its workspace for the
+ * call (efe_csr_linsolve_device_noalloc does not). */
+int efe_csr_linsolve_device(int32_t n, const int32_t* indptr, const int32_t* indices, const double* data,
+                            const double* b, double* x, void* stream, const efe_solve_opts* opts,
+                            efe_solve_info* info);
+
 /* Accepts the temporary QP state as the new reference state (femodel.py:245-258). */
 int efe_accept_state(efe_ctx* ctx);
 
@@ -232,7 +262,9 @@ int efe_get_qp_state(efe_ctx* ctx, int32_t which, double* out);
  * marmotelement/element.pxd:97-103): Ke device [n][nDof][nDof] column-major with DOFs
  * node-major (3a + i), Pe device [n][nDof] (may be NULL); nDof = 3 x nodes per element.
  * For a host-side assembly in the manner of NISTParallel (nonlinearimplicitstaticparallel.pyx),
- * or inspection.  Needs an assembled (FP64 or FP32) pass, not a matrix-free one. */
+ * or inspection.  Needs an assembled (FP64 or FP32) efe_compute_elements as the last element pass
+ * (EFE_ERR_STATE after a matrix-free or residual-only pass), and for Pe no load computation
+ * (efe_body_force, efe_surface_traction) since. */
 int efe_get_element_matrices(efe_ctx* ctx, const int32_t* elements, int32_t n, double* Ke, double* Pe);
 
 /* Device gather of QP results (utils/elementresultcollector.pyx:37-132, which collects
@@ -252,9 +284,17 @@ int efe_set_qp_state(efe_ctx* ctx, const double* in);
  * Krylov kernels; 0 keeps the current choice. */
 int efe_set_launch(efe_ctx* ctx, int32_t elements_per_workgroup, int32_t krylov_block);
 
-/* Average device time (ms) of the last call of each phase, measured with HIP events
- * on the context stream (host out[4]): element kernel, assembly, one Krylov iteration, norms. */
-int efe_get_phase_times(efe_ctx* ctx, double out[4]);
+/* Device times (ms) measured with HIP events on the context's streams (host out[16]):
+ *   out[0..3]   the last call: element kernel, assembly, one Krylov iteration (average of the
+ *               last solve), norms;
+ *   out[4..14]  totals since the context was created: element kernel of tangent passes (ms),
+ *               tangent passes (count), assembly (ms), residual-only passes (ms), residual-only
+ *               passes (count), Krylov iterations (ms, without preconditioner set-up), Krylov
+ *               iterations (count), solves (count), ILU(0) factorisations (ms), norms (ms),
+ *               norm calls (count); out[15] = 0.
+ * The totals divided by their counts are the history's per-call averages (the computation-time
+ * categories of config/timing.py, measured on the device). */
+int efe_get_phase_times(efe_ctx* ctx, double out[16]);
 
 #ifdef __cplusplus
 }

@@ -10,8 +10,19 @@ and C3D20R, vectorised over elements:
   the first two natural coordinates exchanged, i.e. quadrature point i is
   evaluated at the natural point (eta_i, xi_i, zeta_i) while its weight stays
   w_i.  Here the same quantities are formed from analytic shape-function
-  derivatives at that exchanged point; tests/test_oracle_element.py checks the
-  equivalence against an independent polynomial fit on distorted elements;
+  derivatives at that exchanged point;
+* hexa20 Jacobian, third row: the reference's hand-expanded derivative of the fitted
+  map with respect to the third natural coordinate multiplies the coefficient a5 of the
+  monomial (second coordinate x third coordinate) by z_i where the derivative gives the
+  second coordinate of the evaluation point, xi_i (_elementcomputationmatrices.py:502,
+  :514, :526; the first two rows and hexa8 are exact).  Since a5 = d2x/(d eta d zeta) at
+  the element centre = sum_a h_a x_a with h_a = d2N_a/(d eta d zeta)(0) (+-1/4 on the four
+  mid-side nodes (0, +-1, +-1), 0 elsewhere), the reference's J is the exact J at the
+  exchanged point with (zeta_i - xi_i) sum_a h_a x_a added to its third row.  B uses that
+  J with the exact natural derivatives (:820-1060), and detJ of that J enters K, P and the
+  body force.  On undistorted elements a5 = 0 and nothing changes; on distorted ones J
+  differs by up to ~10 %.  Pinned by tests/golden/element_fixtures.npz, the reference
+  element's own J, B, Ke, Pe, state and body force on distorted elements;
 * computeYourself (element.py:290-346): dEps_i = B_i dU, material update per
   QP on a copy of the reference state, K += B^T C B detJ w, P -= B^T sigma detJ w,
   strain += dEps;
@@ -126,11 +137,25 @@ def evaluation_points(el_type):
     return np.stack([eta, xi, zeta], axis=1)
 
 
+def jacobian_table(el_type):
+    """dN/dxi (Q, nn, 3) at the evaluation points, as the reference forms J from them: for
+    hexa20 the third component carries (zeta_q - xi_q) h_a in addition (module docstring)."""
+    nn = nodes_per_element(el_type)
+    xi, eta, zeta, _ = qp_table(el_type)
+    dJ = shape_derivatives(nn, evaluation_points(el_type)).copy()
+    if nn == 20:
+        nat = NAT20
+        h = np.where(nat[:, 0] == 0, nat[:, 1] * nat[:, 2] / 4.0, 0.0)  # d2N/(d eta d zeta) at 0
+        dJ[:, :, 2] += (zeta - xi)[:, None] * h[None, :]
+    return dJ
+
+
 def geometry(el_type, X):
-    """Per element and QP: J (E,Q,3,3) with J[r,c] = d x_c / d xi_r, detJ (E,Q), dN/dx (E,Q,nn,3)."""
+    """Per element and QP: J (E,Q,3,3) with J[r,c] = d x_c / d xi_r (the reference's J,
+    jacobian_table), detJ (E,Q), dN/dx (E,Q,nn,3) = J^-1 dN/dxi."""
     nn = nodes_per_element(el_type)
     dN = shape_derivatives(nn, evaluation_points(el_type))  # (Q, nn, 3)
-    J = np.einsum("qar,eac->eqrc", dN, X)
+    J = np.einsum("qar,eac->eqrc", jacobian_table(el_type), X)
     detJ = np.linalg.det(J)
     invJ = np.linalg.inv(J)
     dNdx = np.einsum("eqcr,qar->eqac", invJ, dN)

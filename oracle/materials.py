@@ -200,6 +200,12 @@ class DruckerPrager:
             dgam = Phi * A
             cone = plastic & (sqJ2 - G * dgam >= 0.0)
             apex = plastic & ~cone
+            if (apex & ~(etab > 0.0)).any():
+                # a trial state beyond the apex of a non-dilatant cone (psi = 0): the plastic
+                # flow is purely deviatoric and cannot bring p back to the apex, so no return
+                # exists; ask for a smaller increment, as the von Mises local Newton does
+                raise CutbackRequest("Drucker-Prager: trial stress beyond the apex with zero dilatancy, no return.",
+                                     0.5)
             with np.errstate(invalid="ignore", divide="ignore"):
                 n = s / snorm[:, None]  # unit deviatoric direction (tensor norm)
             # smooth cone: s = (1 - G dgam / sqrt(J2)) s_tr, p = p_tr - K etabar dgam, e += xi dgam
@@ -241,6 +247,36 @@ class DruckerPrager:
         mstate = mstate.copy()
         mstate[:, 0] = e
         return new_stress, C, mstate
+
+
+class Sections:
+    """Several materials in one model, as sections assign a material to the elements of their
+    element sets (sections/base/sectionbase.py:86-128, assignSectionPropertiesToElement):
+    materials[k] is the material of the elements with material_of_element[e] == k.  compute()
+    runs every material on the QPs of its elements, in the flattened (element, QP) order of
+    HexElements.compute; the state width is the largest of the materials' (the extra columns of
+    a material with fewer state variables stay as they are)."""
+
+    def __init__(self, materials, material_of_element, n_qp):
+        self.materials = list(materials)
+        self.of_qp = np.repeat(np.asarray(material_of_element, dtype=np.int64), n_qp)
+        self.n_state = max(m.n_state for m in self.materials)
+
+    def compute(self, stress, mstate, dstrain):
+        if stress.shape[0] != self.of_qp.size:
+            raise ValueError("Sections.compute covers the whole element set")
+        out = stress.copy()
+        C = np.empty((stress.shape[0], 6, 6))
+        ms = np.array(mstate, dtype=float, copy=True)
+        for k, mat in enumerate(self.materials):
+            sel = self.of_qp == k
+            if not sel.any():
+                continue
+            s_k, C_k, m_k = mat.compute(stress[sel], mstate[sel][:, :mat.n_state], dstrain[sel])
+            out[sel] = s_k
+            C[sel] = C_k
+            ms[np.ix_(sel, np.arange(mat.n_state))] = m_k
+        return out, C, ms
 
 
 MATERIALS = {"linearelastic": LinearElastic, "vonmises": VonMises, "druckerprager": DruckerPrager}

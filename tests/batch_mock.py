@@ -154,6 +154,9 @@ class OracleSystem(OracleBatchSystem):
     def reset_state(self):
         pass
 
+    def close(self):
+        pass
+
     def norms(self, R, ddU, F):
         Rh = R.numpy()
         dh = ddU.numpy() if ddU is not None else None
@@ -172,5 +175,5 @@ class OracleSystem(OracleBatchSystem):
 
 def attach_mock_single(solver, oracle_model):
     """Points a NISTGPU at an OracleSystem built from `oracle_model`."""
-    solver.system = OracleSystem(oracle_model)
+    solver.attach(OracleSystem(oracle_model))
     return solver.system

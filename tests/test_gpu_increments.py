@@ -61,9 +61,51 @@ def test_every_increment_matches_the_oracle(el_type, material, props, n, strain,
         assert rel_err(sig, w["stress"]) < 1e-9, inc
 
 
-@pytest.mark.parametrize("material,props,precision,linsolver", [
-    ("vonmises", J2, "fp64", "pcg"), ("vonmises", J2, "mixed", "pcg"), ("druckerprager", DP, "fp64", "gmres")])
-def test_residual_first_changes_nothing_but_the_work(material, props, precision, linsolver):
+@pytest.mark.parametrize("case", [
+    # (n, extrapolation, hardening, maxInc): the 64^3 workload's family at a size where the PCG
+    # runs hundreds of iterations per solve and SuperLU still finishes, and a history without
+    # extrapolation and with nonlinear hardening (3-4 Newton iterations in every increment)
+    ((16, 16, 16), "linear", J2, 0.1),
+    ((12, 12, 12), "none", [210000.0, 0.3, 250.0, 1000.0, 100.0, 300.0], 0.125),
+])
+def test_mid_size_history_matches_superlu_oracle(case):
+    """Inexact Krylov solves (stopped at krylovAtolFactor of the flux tolerance) against the
+    oracle's direct SuperLU solves (the reference's superlu option, config/linsolve.py:47-50) on
+    a jittered mesh: every increment takes the same Newton iterations, and U and the QP
+    stresses agree to 1e-9 after every increment."""
+    from edelweissfe_amd.solvers import NISTGPU
+
+    n, extrapolation, props, maxInc = case
+    kw = dict(maxInc=maxInc, minInc=1e-6, maxNumInc=100, maxIter=15, extrapolation=extrapolation)
+    print("case", n, extrapolation, flush=True)
+    om = oracle_model(n, (1.0, 1.0, 1.0), "C3D8", "vonmises", props, jitter=0.1, seed=0)
+    so = onist.NIST()
+    so.solve_step(om, oracle_step(uniaxial_bcs(0.01), **kw))
+    pm = device_model(n, (1.0, 1.0, 1.0), "C3D8", "vonmises", props, jitter=0.1, seed=0)
+    s = NISTGPU(precision="fp64")
+    rec = _Recorder(pm, s)
+    s.solveStep(device_step(pm, uniaxial_bcs(0.01), **kw), pm, outputmanagers=[rec])
+    print("Newton", [r["iters"] for r in s.history], "Krylov per solve", [r["krylov"] for r in s.history])
+    assert [r["iters"] for r in s.history] == [w["iters"] if w["converged"] else np.inf for w in so.increments]
+    assert max(max(r["krylov"] or [0]) for r in s.history) > 100  # solves far from a few iterations
+    want = [w for w in so.increments if w["converged"]]
+    assert len(rec.rows) == len(want) and (om.state[..., 12] > 0).all()
+    for (inc, U, sig), w in zip(rec.rows, want):
+        assert inc == w["inc"]
+        assert rel_err(U, w["U"]) < 1e-9, inc
+        assert rel_err(sig, w["stress"]) < 1e-9, inc
+
+
+@pytest.mark.parametrize("material,props,precision,linsolver,operator", [
+    ("vonmises", J2, "fp64", "pcg", "auto"),
+    # the mixed operator is pinned: "auto" chooses from the observed Krylov iterations per
+    # tangent pass, which differ between the two runs, so it could pick different operators
+    ("vonmises", J2, "mixed", "pcg", "matfree"),
+    ("vonmises", J2, "mixed", "pcg", "assembled"),
+    ("druckerprager", DP, "fp64", "gmres", "auto"),
+    ("druckerprager", DP, "mixed", "gmres", "assembled"),
+])
+def test_residual_first_changes_nothing_but_the_work(material, props, precision, linsolver, operator):
     """residualFirst (the default) checks Newton iteration 0 on a residual-only pass and forms
     the tangent only when a solve follows: the same Newton counts, U and QP state bit for bit as
     with a full element pass at every check, and fewer tangent passes."""
@@ -72,7 +114,7 @@ def test_residual_first_changes_nothing_but_the_work(material, props, precision,
     runs = []
     for residualFirst in (True, False):
         pm = device_model((4, 3, 3), (1.0, 1.0, 1.0), "C3D8", material, props, jitter=0.1, seed=6)
-        s = NISTGPU(precision=precision, linsolver=linsolver, residualFirst=residualFirst)
+        s = NISTGPU(precision=precision, linsolver=linsolver, residualFirst=residualFirst, mixedOperator=operator)
         s.solveStep(device_step(pm, uniaxial_bcs(0.006 if material == "vonmises" else 0.003), maxInc=0.1,
                                 minInc=1e-6, maxNumInc=100, maxIter=15), pm)
         runs.append((s, pm, pm.qpState.copy()))
@@ -82,3 +124,32 @@ def test_residual_first_changes_nothing_but_the_work(material, props, precision,
     assert 0 in [r["iters"] for r in s1.history]  # increments that converge at their first check
     assert np.array_equal(m1.U, m0.U) and np.array_equal(q1, q0)
     assert s1.elementPasses < s0.elementPasses
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_residual_first_with_geostatic_prestress(precision):
+    """residualFirst on the non-symmetric workload's two steps (geostatic prestress applied at
+    every iteration start, then the footing; GMRES + ILU(0)): bit for bit as full passes."""
+    from edelweissfe_amd.solvers import NISTGPU
+    from edelweissfe_amd.steps import AdaptiveStep, BodyForce, Dirichlet, Geostatic
+    from test_gpu_parity import DP as DPP, GEO, ROLLERS, _footing_nodes
+
+    slope = (GEO["p2"] - GEO["p1"]) / (GEO["h2"] - GEO["h1"])
+    runs = []
+    for residualFirst in (True, False):
+        pm = device_model((3, 3, 3), (1.0, 1.0, 1.0), "C3D20R", "druckerprager", DPP, jitter=0.1, seed=3)
+        pm.nodeSets["footing"] = _footing_nodes(pm.coordinates, pm.nodeSets["gen_top"])
+        footing = [("footing", "footing", {1: -4e-3})]
+        grav = BodyForce("gravity", "all", (0.0, -slope, 0.0), amplitude=lambda t: 1.0)
+        s = NISTGPU(precision=precision, linsolver="gmres", residualFirst=residualFirst)
+        a1 = {"dirichlet": {nm: Dirichlet(nm, ns, v) for nm, ns, v in ROLLERS}, "bodyforce": {"gravity": grav},
+              "geostatic": {"geo": Geostatic("geo", **GEO)}}
+        s.solveStep(AdaptiveStep(1, pm, a1, maxInc=1.0, minInc=1e-6, maxNumInc=100, maxIter=15), pm)
+        a2 = {"dirichlet": {nm: Dirichlet(nm, ns, v) for nm, ns, v in ROLLERS + footing},
+              "bodyforce": {"gravity": grav}}
+        s.solveStep(AdaptiveStep(2, pm, a2, maxInc=0.25, minInc=1e-6, maxNumInc=100, maxIter=15), pm)
+        runs.append((s, pm.U.copy(), pm.qpState.copy()))
+    (s1, u1, q1), (s0, u0, q0) = runs
+    assert [r["iters"] for r in s1.history] == [r["iters"] for r in s0.history]
+    assert s1.residualPasses > 0
+    assert np.array_equal(u1, u0) and np.array_equal(q1, q0)

@@ -1,5 +1,7 @@
 """Closed-form and independent checks that pin the oracle's element."""
 
+import os
+
 import numpy as np
 import pytest
 
@@ -68,10 +70,22 @@ def test_shape_functions_partition_of_unity_and_kronecker(el_type):
         assert np.allclose(element.shape_derivatives(nn, p)[:, :, r], fd, atol=1e-8)
 
 
+def _untwisted(X):
+    """The hexa20 element X moved to twist coefficient a5 = d2x/(d eta d zeta) at the centre
+    = (-x_9 + x_11 + x_13 - x_15) / 4 = 0 (0-based mid-side nodes (0, -+1, +-1)), by moving
+    node 9 alone.  The reference's hexa20 Jacobian is exact on such elements only."""
+    X = X.copy()
+    a5 = (-X[9] + X[11] + X[13] - X[15]) / 4
+    X[9] += 4 * a5
+    return X
+
+
 @pytest.mark.parametrize("el_type", ["C3D8", "C3D20", "C3D20R"])
 def test_element_stiffness_symmetry_rigid_modes_and_patch(el_type):
     nn = element.nodes_per_element(el_type)
     X = _distorted(nn, 2)
+    if nn == 20:
+        X = _untwisted(X)
     el = element.HexElements(el_type, X, np.arange(nn)[None])
     mat = materials.LinearElastic([210000.0, 0.3])
     st = np.zeros((1, el.nQP, 12))
@@ -122,3 +136,72 @@ def test_jittered_mesh_is_seeded_and_bounded():
     assert d.max() <= 0.1 * 0.25 + 1e-15 and d.max() > 0.02
     on_boundary = (np.isclose(c, 0) | np.isclose(c, 1)).any(axis=1)
     assert np.array_equal(a[on_boundary], c[on_boundary])
+
+
+FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "element_fixtures.npz")
+
+
+def _rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / np.abs(b).max())
+
+
+@pytest.mark.parametrize("el_type", ["C3D8", "C3D20", "C3D20R"])
+def test_geometry_and_body_force_equal_the_reference_element_on_distorted_elements(el_type):
+    """J, B (initializeElement, element.py:194-205) and computeBodyForce (:348-371) of the
+    reference's own element on three general distorted elements (tests/golden/
+    make_element_fixtures.py)."""
+    d = np.load(FIX)
+    X = d[el_type + "_X"]
+    nn = X.shape[1]
+    el = element.HexElements(el_type, X.reshape(-1, 3), np.arange(X.shape[0] * nn).reshape(-1, nn))
+    assert _rel(el.J, d[el_type + "_J"]) < 1e-13
+    assert _rel(element.b_operator(el.dNdx), d[el_type + "_B"]) < 1e-13
+    assert _rel(el.body_force(d["body_load"]), d[el_type + "_bodyforce"]) < 1e-13
+
+
+@pytest.mark.parametrize("el_type", ["C3D20", "C3D20R"])
+def test_compute_yourself_equals_the_reference_element_on_distorted_elements(el_type):
+    """computeYourself (element.py:290-346) with von Mises and nonlinear hardening over two
+    plastic increments, the first accepted: Ke, Pe and the temporary QP state."""
+    d = np.load(FIX)
+    X = d[el_type + "_X"]
+    nn = X.shape[1]
+    el = element.HexElements(el_type, X.reshape(-1, 3), np.arange(X.shape[0] * nn).reshape(-1, nn))
+    mat = materials.VonMises(d["vonmises_props"])
+    Ke, Pe, st = el.compute(d[el_type + "_dU1"], np.zeros((X.shape[0], el.nQP, 13)), mat)
+    assert (st[..., 12] > 0).all()  # every QP plastic
+    assert _rel(Ke, d[el_type + "_Ke1"]) < 1e-12 and _rel(Pe, d[el_type + "_Pe1"]) < 1e-12
+    assert _rel(st, d[el_type + "_state1"]) < 1e-12
+    Ke, Pe, st2 = el.compute(d[el_type + "_dU2"], st, mat)
+    assert _rel(Ke, d[el_type + "_Ke2"]) < 1e-12 and _rel(Pe, d[el_type + "_Pe2"]) < 1e-12
+    assert _rel(st2, d[el_type + "_state2"]) < 1e-12
+
+
+@pytest.mark.parametrize("el_type", ["C3D20", "C3D20R"])
+def test_exact_hexa20_jacobian_does_not_match_the_reference(el_type):
+    """The fixtures tell the reference's hexa20 Jacobian from the exact isoparametric one:
+    with the exact J (no third-row term) J and Ke differ by far more than rounding, and so
+    does the body force, through detJ."""
+    d = np.load(FIX)
+    X = d[el_type + "_X"]
+    dN = element.shape_derivatives(20, element.evaluation_points(el_type))
+    J = np.einsum("qar,eac->eqrc", dN, X)
+    assert _rel(J, d[el_type + "_J"]) > 1e-2
+    xi, eta, zeta, w = element.qp_table(el_type)
+    Nt = element.shape_functions(20, np.stack([xi, eta, zeta], axis=1))
+    bf = np.einsum("qa,eq,c->eac", Nt, np.linalg.det(J) * w[None], d["body_load"]).reshape(X.shape[0], -1)
+    assert _rel(bf, d[el_type + "_bodyforce"]) > 1e-3
+
+
+def test_reference_hexa20_is_not_rotation_free_on_twisted_elements():
+    """A property of the reference element that the oracle keeps: on an element with a
+    nonzero twist coefficient a5 a rigid rotation produces strain (the Jacobian's third row
+    is not that of the isoparametric map), while on the same element with a5 = 0 it does not."""
+    X = _distorted(20, 2)
+    mat = materials.LinearElastic([210000.0, 0.3])
+    w = np.array([0.3, -0.2, 0.5])
+    for Xe, free in ((X, False), (_untwisted(X), True)):
+        el = element.HexElements("C3D20", Xe, np.arange(20)[None])
+        u = np.cross(w, Xe).ravel()[None] * 1e-3
+        _, _, st = el.compute(u, np.zeros((1, el.nQP, 12)), mat)
+        assert (np.abs(st[..., 6:12]).max() < 1e-15) == free

@@ -148,3 +148,18 @@ def test_drucker_prager_hydrostatic_tension_returns_to_apex():
     assert np.allclose(s[3:], 0) and np.allclose(s[:3], s[0])
     assert abs(s[0] - props[4] * xi / eta) < 1e-10 * abs(s[0])
     assert np.allclose(C, 0.0)  # perfect plasticity: zero apex tangent
+
+
+def test_drucker_prager_apex_without_dilatancy_asks_for_a_cutback():
+    """psi = 0: a trial state in hydrostatic tension beyond the apex has no return (the flow is
+    purely deviatoric and cannot reduce p), so the material asks for a smaller increment instead
+    of dividing by etabar = 0; a smaller tension inside the cone still returns to the cone."""
+    from oracle.exceptions import CutbackRequest
+
+    props = [30000.0, 0.2, 30.0, 0.0, 10.0, 0.0]
+    mat = materials.DruckerPrager(props)
+    with pytest.raises(CutbackRequest) as ei:
+        _call(mat, np.zeros(6), 0.0, np.array([1e-2, 1e-2, 1e-2, 0, 0, 0]))
+    assert ei.value.cutbackSize == 0.5
+    s, C, e = _call(mat, np.zeros(6), 0.0, np.array([-2e-3, 1.5e-3, 0.5e-3, 1.0e-3, -0.5e-3, 0.8e-3]))
+    assert np.all(np.isfinite(s)) and np.all(np.isfinite(C)) and e > 0
