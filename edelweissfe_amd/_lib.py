@@ -100,6 +100,7 @@ SIGNATURES = {
     "efe_set_active_specimens": (_I, [_P, _P]),
     "efe_set_launch": (_I, [_P, _I, _I]),
     "efe_get_phase_times": (_I, [_P, ctypes.POINTER(_D)]),
+    "efe_element_compute": (_I, [_P, _P, _I, _P, _P, _P, _P, _P, _P, ctypes.POINTER(_D), _D, ctypes.POINTER(_D)]),
     "efe_csr_linsolve": (_I, [_I, _P, _P, _P, _P, _P, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]),
     "efe_csr_linsolve_device": (_I, [_I, _P, _P, _P, _P, _P, _P, ctypes.POINTER(SolveOpts),
                                      ctypes.POINTER(SolveInfo)]),

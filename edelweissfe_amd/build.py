@@ -8,8 +8,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libefe_gpu.so")
-SOURCES = ["capi.hip", "elements.hip", "krylov.hip", "gmres.hip", "batched.hip", "matfree.hip", "csrsolve.hip",
-           "pattern.cpp"]
+SOURCES = ["capi.hip", "elements.hip", "elemabi.hip", "krylov.hip", "gmres.hip", "batched.hip", "matfree.hip",
+           "csrsolve.hip", "pattern.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -33,6 +33,18 @@ struct BatchWork {
     hipGraphExec_t graph = nullptr;
     int graph_batch = 0;
     int graph_op = -1;  // precision code the captured graph was built for (operator choice)
+    // device-resident Newton rounds (efe_newton_round_batched): per specimen its Newton state
+    // (ITERATING 0, CONVERGED 1, FAILED 2, IDLE 3), iteration counter, residual growth count and
+    // previous flux residual, failure reason, first-iteration flag, limits, and work counters;
+    // per DOF the prescribed values of first iterations, the right-hand-side correction, R - corr
+    // and the Newton correction
+    int *nst = nullptr, *nit = nullptr, *ngrow = nullptr, *nreason = nullptr, *nfirst = nullptr;
+    int *nextrap = nullptr, *nhasdir = nullptr, *nmaxit = nullptr, *nmaxgrow = nullptr, *nsolve = nullptr;
+    int *npass = nullptr, *nsolves = nullptr, *nkry = nullptr, *ncount = nullptr;
+    double *ngprev = nullptr, *nacc = nullptr;
+    double *dirv = nullptr, *corr = nullptr, *rhs = nullptr, *ddu = nullptr;
+    hipGraphExec_t ngraph = nullptr;            // one captured Newton iteration
+    const void* ngraph_key[6] = {};             // dU, P, F, PExt, deltaT, precision it was built for
 };
 
 namespace {
@@ -385,25 +397,16 @@ int ensure_batch(efe_ctx* c, int prec) {
     return EFE_OK;
 }
 
+// The batched PCG from the tolerances sc[TOL2], sc[RTOL2] and the active flags already on the
+// device (after hipStreamWaitEvent of the context stream); info, spec_iters and xout as
+// efe_linsolve_batched.
 template <typename T>
-int batched_pcg(efe_ctx* c, const double* b, double* xout, const efe_solve_opts* o, const double* spec_tol,
-                const double* spec_rtol, const uint8_t* spec_active, int32_t* spec_iters, efe_solve_info* info,
-                const T* K, const T* Minv) {
+int batched_pcg_run(efe_ctx* c, const double* b, double* xout, const efe_solve_opts* o, int32_t* spec_iters,
+                    efe_solve_info* info, const T* K, const T* Minv) {
     BatchWork* w = c->bw;
     hipStream_t st = w->ks;
     const int N = c->N, S = c->nspec, D = c->D;
     std::vector<double> sc((size_t)S * NSC, 0.0);
-    std::vector<int> act(S);
-    for (int s = 0; s < S; ++s) {
-        sc[(size_t)s * NSC + TOL2] = spec_tol[s] * spec_tol[s];
-        const double rt = spec_rtol ? spec_rtol[s] : o->rtol;
-        sc[(size_t)s * NSC + RTOL2] = rt * rt;
-        act[s] = spec_active ? (spec_active[s] ? 1 : 0) : 1;
-    }
-    EFE_HIP(hipEventRecord(w->evA, c->stream));
-    EFE_HIP(hipStreamWaitEvent(st, w->evA, 0));
-    EFE_HIP(hipMemcpyAsync(w->sc, sc.data(), sc.size() * sizeof(double), hipMemcpyHostToDevice, st));
-    EFE_HIP(hipMemcpyAsync(w->active, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, st));
     b_init_kernel<T><<<gridn(N), BS, 0, st>>>(N, c->specOfNode, w->active, b, Minv, (T*)w->x, (T*)w->r, (T*)w->z,
                                               (T*)w->p, w->nv0, w->nv1);
     b_init_reduce_kernel<<<S, BS, 0, st>>>(c->specNodeOff, w->nv0, w->nv1, w->sc, w->active, w->iters);
@@ -456,6 +459,30 @@ int batched_pcg(efe_ctx* c, const double* b, double* xout, const efe_solve_opts*
     info->residual = sqrt(worst);
     info->rhs_norm = 0.0;
     return EFE_OK;
+}
+
+template <typename T>
+int batched_pcg(efe_ctx* c, const double* b, double* xout, const efe_solve_opts* o, const double* spec_tol,
+                const double* spec_rtol, const uint8_t* spec_active, int32_t* spec_iters, efe_solve_info* info,
+                const T* K, const T* Minv) {
+    BatchWork* w = c->bw;
+    hipStream_t st = w->ks;
+    const int S = c->nspec;
+    std::vector<double> sc((size_t)S * NSC, 0.0);
+    std::vector<int> act(S);
+    for (int s = 0; s < S; ++s) {
+        sc[(size_t)s * NSC + TOL2] = spec_tol[s] * spec_tol[s];
+        const double rt = spec_rtol ? spec_rtol[s] : o->rtol;
+        sc[(size_t)s * NSC + RTOL2] = rt * rt;
+        act[s] = spec_active ? (spec_active[s] ? 1 : 0) : 1;
+    }
+    EFE_HIP(hipEventRecord(w->evA, c->stream));
+    EFE_HIP(hipStreamWaitEvent(st, w->evA, 0));
+    EFE_HIP(hipMemcpyAsync(w->sc, sc.data(), sc.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    EFE_HIP(hipMemcpyAsync(w->active, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    int rc = batched_pcg_run<T>(c, b, xout, o, spec_iters, info, K, Minv);
+    EFE_HIP(hipStreamSynchronize(st));  // sc and act must outlive the copies
+    return rc;
 }
 
 }  // namespace
@@ -578,9 +605,12 @@ void efe_batched_free(efe_ctx* c) {
     void* bufs[] = {w->x,      w->r,      w->z,      w->p,          w->q,         w->nv0,
                     w->nv1,    w->sc,     w->active, w->iters,      w->nactive,   w->acc,
                     w->nout,   w->pqp,    w->rzp,    w->rrp,        w->chunkNode0, w->chunkSpec,
-                    w->specChunk};
+                    w->specChunk, w->nst, w->nit, w->ngrow, w->nreason, w->nfirst, w->nextrap, w->nhasdir,
+                    w->nmaxit, w->nmaxgrow, w->nsolve, w->npass, w->nsolves, w->nkry, w->ncount, w->ngprev,
+                    w->nacc, w->dirv, w->corr, w->rhs, w->ddu};
     for (void* p : bufs)
         if (p) hipFree(p);
+    if (w->ngraph) hipGraphExecDestroy(w->ngraph);
     if (w->hbuf) hipHostFree(w->hbuf);
     if (w->graph) hipGraphExecDestroy(w->graph);
     if (w->ks) hipStreamDestroy(w->ks);

@@ -131,8 +131,8 @@ int check_solve_args(int32_t n, const void* ip, const void* ix, const void* v, c
         efe_set_error(std::string(what) + ": invalid arguments");
         return EFE_ERR_ARG;
     }
-    if (o->solver != EFE_SOLVER_CG && o->solver != EFE_SOLVER_BICGSTAB) {
-        efe_set_error(std::string(what) + ": solver must be EFE_SOLVER_CG or EFE_SOLVER_BICGSTAB");
+    if (o->solver != EFE_SOLVER_CG && o->solver != EFE_SOLVER_BICGSTAB && o->solver != EFE_SOLVER_GMRES) {
+        efe_set_error(std::string(what) + ": solver must be EFE_SOLVER_CG, EFE_SOLVER_BICGSTAB or EFE_SOLVER_GMRES");
         return EFE_ERR_ARG;
     }
     return EFE_OK;
@@ -140,6 +140,26 @@ int check_solve_args(int32_t n, const void* ip, const void* ix, const void* v, c
 }  // namespace
 
 void efe_set_error(const std::string& msg) { g_err = msg; }
+
+// Buffers of an element pass and its assembly in `precision` (allocated once).
+int efe_prepare_elements(efe_ctx* c, int precision) {
+    const size_t nblk = (size_t)c->scratchBlocks * kBlockPitch;
+    if (!c->Kscr) EFE_HIP(hipMalloc(&c->Kscr, nblk * sizeof(double)));
+    const size_t nval = (size_t)c->hp.padded_blocks * 9;
+    if (precision == EFE_FP64 && !c->K64) {
+        EFE_HIP(hipMalloc(&c->K64, nval * sizeof(double)));
+        EFE_HIP(hipMemsetAsync(c->K64, 0, nval * sizeof(double), c->stream));
+        EFE_HIP(hipMalloc(&c->Minv64, (size_t)c->N * 9 * sizeof(double)));
+    }
+    if (precision == EFE_FP32 && !c->K32) {
+        EFE_HIP(hipMalloc(&c->K32, nval * sizeof(float)));
+        EFE_HIP(hipMemsetAsync(c->K32, 0, nval * sizeof(float), c->stream));
+        EFE_HIP(hipMalloc(&c->Minv32, (size_t)c->N * 9 * sizeof(float)));
+    }
+    if (precision == EFE_FP32_MATFREE)
+        if (int rc = efe_mf_alloc(c)) return rc;
+    return EFE_OK;
+}
 
 extern "C" {
 
@@ -201,6 +221,7 @@ int efe_ctx_create(const efe_mesh_desc* d, efe_ctx** out) {
                 ++c->ngroups;
             }
         c->material = c->groupMat[c->ngroups - 1];  // the material with the most state (LE < J2 < DP)
+        c->matOf.assign(mat_of, mat_of + d->n_elements);
         c->nstate = 12 + (mat_count[EFE_VONMISES] || mat_count[EFE_DRUCKER_PRAGER] ? 1 : 0);
     } else {
         c->groupMat[0] = c->material;
@@ -420,21 +441,7 @@ int efe_compute_elements(efe_ctx* c, const double* U, const double* dU, const do
         efe_set_error("efe_compute_elements: invalid arguments");
         return EFE_ERR_ARG;
     }
-    const size_t nblk = (size_t)c->scratchBlocks * kBlockPitch;
-    if (!c->Kscr) EFE_HIP(hipMalloc(&c->Kscr, nblk * sizeof(double)));
-    const size_t nval = (size_t)c->hp.padded_blocks * 9;
-    if (precision == EFE_FP64 && !c->K64) {
-        EFE_HIP(hipMalloc(&c->K64, nval * sizeof(double)));
-        EFE_HIP(hipMemsetAsync(c->K64, 0, nval * sizeof(double), c->stream));
-        EFE_HIP(hipMalloc(&c->Minv64, (size_t)c->N * 9 * sizeof(double)));
-    }
-    if (precision == EFE_FP32 && !c->K32) {
-        EFE_HIP(hipMalloc(&c->K32, nval * sizeof(float)));
-        EFE_HIP(hipMemsetAsync(c->K32, 0, nval * sizeof(float), c->stream));
-        EFE_HIP(hipMalloc(&c->Minv32, (size_t)c->N * 9 * sizeof(float)));
-    }
-    if (precision == EFE_FP32_MATFREE)
-        if (int rc = efe_mf_alloc(c)) return rc;
+    if (int rc = efe_prepare_elements(c, precision)) return rc;
     EFE_HIP(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
     if (c->specStatus) EFE_HIP(hipMemsetAsync(c->specStatus, 0, (size_t)c->nspec * sizeof(int), c->stream));
     EFE_HIP(hipEventRecord(c->ev[0], c->stream));
@@ -812,7 +819,7 @@ int efe_csr_linsolve(int32_t n, const int32_t* indptr, const int32_t* indices, c
     EFE_HIP(db.alloc(&dv, (size_t)nnz));
     EFE_HIP(db.alloc(&dbv, (size_t)n));
     EFE_HIP(db.alloc(&dx, (size_t)n));
-    EFE_HIP(db.alloc(&ws, efe_csr_workspace_doubles(n, o->solver)));
+    EFE_HIP(db.alloc(&ws, efe_csr_workspace_doubles(n, o->solver, o->restart)));
     hipStream_t st = nullptr;
     EFE_HIP(hipMemcpyAsync(dip, indptr, ((size_t)n + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
     EFE_HIP(hipMemcpyAsync(dix, indices, (size_t)nnz * sizeof(int32_t), hipMemcpyHostToDevice, st));
@@ -831,7 +838,7 @@ int efe_csr_linsolve_device(int32_t n, const int32_t* indptr, const int32_t* ind
     DeviceBuffers db;
     double* ws;
     int* flag;
-    EFE_HIP(db.alloc(&ws, efe_csr_workspace_doubles(n, o->solver)));
+    EFE_HIP(db.alloc(&ws, efe_csr_workspace_doubles(n, o->solver, o->restart)));
     EFE_HIP(db.alloc(&flag, 1));
     int bad = 0;
     if (int rc = efe_csr_check_pattern(n, indptr, indices, flag, st, &bad)) return rc;

@@ -6,8 +6,8 @@
 // a unit diagonal are taken as prescribed: x = b there, and their columns move to the right-hand
 // side, so the Krylov method works on the free block K_ff (symmetric when K is):
 //   K_ff x_f = b_f - K_fd b_d.
-// Methods: Jacobi-preconditioned CG (symmetric K) and BiCGStab (van der Vorst 1992, right
-// preconditioned; non-symmetric K).
+// Methods: Jacobi-preconditioned CG (symmetric K), BiCGStab (van der Vorst 1992, right
+// preconditioned) and restarted GMRES (right preconditioned) for non-symmetric K.
 //
 // Data: CSR as given (int32 indptr/indices, FP64 values).  SpMV: 16 lanes per row, each lane
 // striding the row by 16, then a fixed-order shuffle reduction; rows of a wavefront are
@@ -283,6 +283,16 @@ __global__ __launch_bounds__(kBS) void bicg_p_kernel(int n, const double* __rest
         p[i] = first ? r[i] : r[i] + beta * (p[i] - omega * v[i]);
 }
 
+__global__ void jacobi_plain_kernel(int n, const double* __restrict__ dinv, const double* __restrict__ x,
+                                    double* __restrict__ y) {
+    const int i = blockIdx.x * kBS + threadIdx.x;
+    if (i < n) y[i] = dinv[i] * x[i];
+}
+
+inline void jacobi_free_kernel(int n, const double* dinv, const double* x, double* y, hipStream_t st) {
+    jacobi_plain_kernel<<<(n + kBS - 1) / kBS, kBS, 0, st>>>(n, dinv, x, y);
+}
+
 // elementwise y = dinv x
 __global__ void jacobi_kernel(int n, const double* __restrict__ dinv, const double* __restrict__ x,
                               double* __restrict__ y, const double* __restrict__ sc) {
@@ -346,20 +356,225 @@ int efe_csr_check_pattern(int32_t n, const int32_t* ip, const int32_t* ix, int* 
     return EFE_OK;
 }
 
-size_t efe_csr_workspace_doubles(int32_t n, int32_t solver) {
-    // CG: x r z p q (5 n); BiCGStab: x r r0 p phat v shat t (8 n); plus Dinv (n), fixed flags
-    // (n bytes, rounded up), two sets of two partials (4 kMaxWG) and 16 scalars
+size_t efe_csr_workspace_doubles(int32_t n, int32_t solver, int32_t restart) {
+    // CG: x r z p q (5 n); BiCGStab: x r r0 p phat v shat t (8 n); GMRES(m): V (m+1) n, w z x rhs
+    // (4 n); plus Dinv (n), fixed flags (n bytes, rounded up), partials (4 kMaxWG, GMRES
+    // (m+2) kMaxWG), 16 scalars and, for GMRES, m+2 coefficients
+    if (solver == EFE_SOLVER_GMRES) {
+        const size_t m = restart > 0 ? restart : 50;
+        return (m + 1 + 4 + 1) * (size_t)n + ((size_t)n + 7) / 8 + (m + 2) * kMaxWG + 16 + (m + 2);
+    }
     const size_t vec = (solver == EFE_SOLVER_CG ? 5 : 8) + 1;
     return vec * (size_t)n + ((size_t)n + 7) / 8 + 4 * kMaxWG + 16;
 }
 
+namespace {
+
+// part[i][wg] = partial of V_i . w over the workgroup's rows, i < cnt (grid-stride)
+__global__ __launch_bounds__(kBS) void csr_multidot_kernel(int n, const double* __restrict__ V, int cnt,
+                                                           const double* __restrict__ w, double* __restrict__ part) {
+    __shared__ double sh[kBS];
+    for (int i = 0; i < cnt; ++i) {
+        const double* vi = V + (size_t)i * n;
+        double s = 0.0;
+        for (int r = blockIdx.x * kBS + threadIdx.x; r < n; r += gridDim.x * kBS) s += vi[r] * w[r];
+        s = wg_sum(s, sh);
+        if (threadIdx.x == 0) part[(size_t)i * kMaxWG + blockIdx.x] = s;
+    }
+}
+
+// w -= sum_i c_i V_i, i < cnt
+__global__ void csr_sub_combination_kernel(int n, const double* __restrict__ V, int cnt, const double* __restrict__ c,
+                                           double* __restrict__ w) {
+    const int r = blockIdx.x * kBS + threadIdx.x;
+    if (r >= n) return;
+    double acc = w[r];
+    for (int i = 0; i < cnt; ++i) acc -= c[i] * V[(size_t)i * n + r];
+    w[r] = acc;
+}
+
+// x += Dinv (sum_i c_i V_i), i < cnt
+__global__ void csr_add_precond_combination_kernel(int n, const double* __restrict__ V, int cnt,
+                                                   const double* __restrict__ c, const double* __restrict__ dinv,
+                                                   double* __restrict__ x) {
+    const int r = blockIdx.x * kBS + threadIdx.x;
+    if (r >= n) return;
+    double acc = 0.0;
+    for (int i = 0; i < cnt; ++i) acc += c[i] * V[(size_t)i * n + r];
+    x[r] += dinv[r] * acc;
+}
+
+// y = a x
+__global__ void csr_scale_kernel(int n, double a, const double* __restrict__ x, double* __restrict__ y) {
+    const int r = blockIdx.x * kBS + threadIdx.x;
+    if (r < n) y[r] = a * x[r];
+}
+
+// y = a - b
+__global__ void csr_sub_kernel(int n, const double* __restrict__ a, const double* __restrict__ b,
+                               double* __restrict__ y) {
+    const int r = blockIdx.x * kBS + threadIdx.x;
+    if (r < n) y[r] = a[r] - b[r];
+}
+
+// GMRES(m) (Saad & Schultz 1986) with right Jacobi preconditioning and classical Gram-Schmidt
+// applied twice; the Hessenberg least-squares problem (Givens rotations) lives on the host.
+// Partials are summed on the host in a fixed order, so the iteration is reproducible.
+int csr_gmres(int32_t n, const int32_t* ip, const int32_t* ix, const double* v, const uint8_t* fixed,
+              const double* dinv, double* x, double* rhs, double* V, double* w, double* z, double* part,
+              double* coef, int m, double tol, int maxit, int* iters, double* resid, hipStream_t st) {
+    const int gd = grid_of(n);
+    const int nwg = gd < kMaxWG ? gd : kMaxWG;
+    const int gs = (int)(((int64_t)n * kLanesPerRow + kBS - 1) / kBS);
+    std::vector<double> hp((size_t)(m + 1) * kMaxWG), H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), hc(m + 1);
+    auto norm = [&](const double* a, double* out) -> int {
+        csr_dot_kernel<<<nwg, kBS, 0, st>>>(n, a, a, part);
+        EFE_HIP(hipMemcpyAsync(hp.data(), part, nwg * sizeof(double), hipMemcpyDeviceToHost, st));
+        EFE_HIP(hipStreamSynchronize(st));
+        double s = 0.0;
+        for (int i = 0; i < nwg; ++i) s += hp[i];
+        *out = std::sqrt(s);
+        return EFE_OK;
+    };
+    int it = 0;
+    double beta = 0.0;
+    while (true) {
+        // r = rhs - A x into w
+        csr_spmv_kernel<<<gs, kBS, 0, st>>>(n, ip, ix, v, fixed, x, w, nullptr);
+        csr_sub_kernel<<<gd, kBS, 0, st>>>(n, rhs, w, w);
+        if (int rc = norm(w, &beta)) return rc;
+        *resid = beta;
+        if (!(beta > tol) || it >= maxit) break;
+        csr_scale_kernel<<<gd, kBS, 0, st>>>(n, 1.0 / beta, w, V);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int j = 0;
+        for (; j < m; ++j) {
+            jacobi_free_kernel(n, dinv, V + (size_t)j * n, z, st);
+            csr_spmv_kernel<<<gs, kBS, 0, st>>>(n, ip, ix, v, fixed, z, w, nullptr);
+            std::fill(hc.begin(), hc.end(), 0.0);
+            for (int pass = 0; pass < 2; ++pass) {
+                csr_multidot_kernel<<<nwg, kBS, 0, st>>>(n, V, j + 1, w, part);
+                for (int i = 0; i <= j; ++i)
+                    EFE_HIP(hipMemcpyAsync(hp.data() + (size_t)i * kMaxWG, part + (size_t)i * kMaxWG,
+                                           nwg * sizeof(double), hipMemcpyDeviceToHost, st));
+                EFE_HIP(hipStreamSynchronize(st));
+                std::vector<double> c(j + 1);
+                for (int i = 0; i <= j; ++i) {
+                    double s = 0.0;
+                    for (int k = 0; k < nwg; ++k) s += hp[(size_t)i * kMaxWG + k];
+                    c[i] = s;
+                    hc[i] += s;
+                }
+                EFE_HIP(hipMemcpyAsync(coef, c.data(), (j + 1) * sizeof(double), hipMemcpyHostToDevice, st));
+                csr_sub_combination_kernel<<<gd, kBS, 0, st>>>(n, V, j + 1, coef, w);
+                EFE_HIP(hipStreamSynchronize(st));  // c must outlive the copy
+            }
+            double hn = 0.0;
+            if (int rc = norm(w, &hn)) return rc;
+            hc[j + 1] = hn;
+            ++it;
+            // Givens rotations of the new column
+            for (int i = 0; i < j; ++i) {
+                const double t = cs[i] * hc[i] + sn[i] * hc[i + 1];
+                hc[i + 1] = -sn[i] * hc[i] + cs[i] * hc[i + 1];
+                hc[i] = t;
+            }
+            const double den = std::hypot(hc[j], hc[j + 1]);
+            cs[j] = den > 0.0 ? hc[j] / den : 1.0;
+            sn[j] = den > 0.0 ? hc[j + 1] / den : 0.0;
+            hc[j] = den;
+            hc[j + 1] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = hc[i];
+            *resid = std::fabs(g[j + 1]);
+            const bool stop = !(*resid > tol) || it >= maxit || !(hn > 0.0) || !(hn == hn);
+            if (!stop) csr_scale_kernel<<<gd, kBS, 0, st>>>(n, 1.0 / hn, w, V + (size_t)(j + 1) * n);
+            if (stop) {
+                ++j;
+                break;
+            }
+        }
+        // y = H^-1 g (upper triangular j x j), x += Dinv V y
+        for (int i = j - 1; i >= 0; --i) {
+            double s = g[i];
+            for (int k = i + 1; k < j; ++k) s -= H[(size_t)i * m + k] * y[k];
+            y[i] = H[(size_t)i * m + i] != 0.0 ? s / H[(size_t)i * m + i] : 0.0;
+        }
+        EFE_HIP(hipMemcpyAsync(coef, y.data(), j * sizeof(double), hipMemcpyHostToDevice, st));
+        csr_add_precond_combination_kernel<<<gd, kBS, 0, st>>>(n, V, j, coef, dinv, x);
+        EFE_HIP(hipGetLastError());
+        EFE_HIP(hipStreamSynchronize(st));  // y must outlive the copy
+        if (!(beta == beta)) break;
+    }
+    *iters = it;
+    return EFE_OK;
+}
+
+}  // namespace
+
+namespace {
+int csr_gmres_solve(int32_t n, const int32_t* ip, const int32_t* ix, const double* v, const double* b, double* xout,
+                    const efe_solve_opts* o, efe_solve_info* info, double* ws, hipStream_t st) {
+    const int m = o->restart > 0 ? o->restart : 50;
+    double* V = ws;
+    double* w = V + (size_t)(m + 1) * n;
+    double* z = w + n;
+    double* x = z + n;
+    double* rhs = x + n;
+    double* dinv = rhs + n;
+    uint8_t* fixed = (uint8_t*)(dinv + n);
+    double* part = dinv + n + ((size_t)n + 7) / 8;
+    double* coef = part + (size_t)(m + 2) * kMaxWG + 16;
+    const int gd = grid_of(n);
+    const int gs = (int)(((int64_t)n * kLanesPerRow + kBS - 1) / kBS);
+    csr_classify_kernel<<<gd, kBS, 0, st>>>(n, ip, ix, v, fixed, dinv);
+    csr_spmv_kernel<<<gs, kBS, 0, st>>>(n, ip, ix, v, fixed, b, rhs, b);  // right-hand side of the free block
+    zero_kernel<<<gd, kBS, 0, st>>>(n, x);
+    EFE_HIP(hipGetLastError());
+    const int nwg = gd < kMaxWG ? gd : kMaxWG;
+    csr_dot_kernel<<<nwg, kBS, 0, st>>>(n, rhs, rhs, part);
+    std::vector<double> hp(nwg);
+    EFE_HIP(hipMemcpyAsync(hp.data(), part, nwg * sizeof(double), hipMemcpyDeviceToHost, st));
+    EFE_HIP(hipStreamSynchronize(st));
+    double bb = 0.0;
+    for (int i = 0; i < nwg; ++i) bb += hp[i];
+    const double bnorm = std::sqrt(bb);
+    info->rhs_norm = bnorm;
+    info->iterations = 0;
+    info->status = EFE_OK;
+    info->residual = bnorm;
+    if (!(bnorm == bnorm)) {
+        info->status = EFE_NAN;
+    } else if (bnorm > 0.0) {
+        const double tol = std::fmax(o->rtol * bnorm, o->atol);
+        const int maxit = o->max_iter > 0 ? o->max_iter : 10000;
+        int it = 0;
+        double res = bnorm;
+        if (int rc = csr_gmres(n, ip, ix, v, fixed, dinv, x, rhs, V, w, z, part, coef, m, tol, maxit, &it, &res, st))
+            return rc;
+        info->iterations = it;
+        info->residual = res;
+        if (!(res == res)) info->status = EFE_NAN;
+        else if (res > tol) info->status = EFE_NOT_CONVERGED;
+    }
+    csr_finish_kernel<<<gd, kBS, 0, st>>>(n, fixed, b, x);
+    EFE_HIP(hipMemcpyAsync(xout, x, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    EFE_HIP(hipGetLastError());
+    EFE_HIP(hipStreamSynchronize(st));
+    return EFE_OK;
+}
+}  // namespace
+
 // The solve on device arrays with the caller's workspace (efe_csr_workspace_doubles doubles).
 int efe_csr_solve_ws(int32_t n, const int32_t* ip, const int32_t* ix, const double* v, const double* b, double* xout,
                      const efe_solve_opts* o, efe_solve_info* info, double* ws, hipStream_t st) {
-    if (o->solver != EFE_SOLVER_CG && o->solver != EFE_SOLVER_BICGSTAB) {
-        efe_set_error("CSR solve: the solver must be EFE_SOLVER_CG or EFE_SOLVER_BICGSTAB");
+    if (o->solver != EFE_SOLVER_CG && o->solver != EFE_SOLVER_BICGSTAB && o->solver != EFE_SOLVER_GMRES) {
+        efe_set_error("CSR solve: unknown solver");
         return EFE_ERR_ARG;
     }
+    if (o->solver == EFE_SOLVER_GMRES) return csr_gmres_solve(n, ip, ix, v, b, xout, o, info, ws, st);
     const bool cg = o->solver == EFE_SOLVER_CG;
     const int nvec = cg ? 5 : 8;
     double* vecs = ws;

@@ -119,6 +119,15 @@ struct efe_ctx {
     // matrix-free FP32 operator: [Q][NC][E] compact tangent, [E*NN][3] node-sorted force slots,
     // dnxi32 = FP32 dNdxi then dJdxi ([2][Q][NN][3])
     float *mfC = nullptr, *mfF = nullptr, *dnxi32 = nullptr;
+    // hexa8 matrix-free operator by workgroup bricks (256 consecutive elements): per workgroup its
+    // local nodes (lptr [G+1] into lnode / lpos / lincp), per local node its partial-force slot in
+    // node order (lpos) and its element-node incidences t*8+a (linc, element order, from lincp);
+    // per element its 8 local node indices (lidx); per node its partial range (pptr [N+1]) in
+    // the partial forces mfbF [L][3]
+    bool mfb = false;
+    int32_t *mfbLptr = nullptr, *mfbLnode = nullptr, *mfbLpos = nullptr, *mfbLincp = nullptr, *mfbPptr = nullptr;
+    uint16_t *mfbLinc = nullptr, *mfbLidx = nullptr;
+    float* mfbF = nullptr;
     int32_t* elemPos = nullptr;  // [NN][E] index of (e, a) in the node-sorted incidence list nlist
                                  // (force slot of the matrix-free operator; block-row (e, a) of the
                                  // element scratch starts at block elemPos * NN)
@@ -130,6 +139,7 @@ struct efe_ctx {
     int groupMat[3] = {0, 0, 0};
     int groupCount[3] = {0, 0, 0};
     int32_t* groupElems[3] = {nullptr, nullptr, nullptr};  // device, ascending; nullptr: all elements
+    std::vector<int8_t> matOf;       // host material of every element (several sections only)
     int curMat = 0;                  // the group of the current element launch
     const int32_t* curList = nullptr;
     int curCount = 0;
@@ -228,7 +238,7 @@ void efe_batched_free(efe_ctx* c);
 
 // scalar CSR Krylov solves (csrsolve.hip)
 constexpr int kCsrPartials = 1024;  // workgroup partials per dot product
-size_t efe_csr_workspace_doubles(int32_t n, int32_t solver);
+size_t efe_csr_workspace_doubles(int32_t n, int32_t solver, int32_t restart);
 int efe_csr_check_pattern(int32_t n, const int32_t* indptr, const int32_t* indices, int* flag, hipStream_t st,
                           int* bad);
 int efe_csr_solve_ws(int32_t n, const int32_t* ip, const int32_t* ix, const double* v, const double* b, double* xout,

@@ -21,6 +21,8 @@
 // element-node (connectivity) and 2 x 4 per element-node (slot index, force slot written and
 // read), plus the node vectors; against 9 x 4 + 4 bytes per 3x3 block of the assembled FP32
 // operator.  It also removes the FP32 assembly from every Newton iteration.
+#include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "efe_internal.h"
@@ -385,6 +387,178 @@ __global__ __launch_bounds__(256) void mf_apply_hex8_kernel(int E, int N, const 
     }
 }
 
+// Hexa8 product by workgroup bricks: workgroup g takes the 256 consecutive elements
+// 256 g .. 256 g + 255.  Its local nodes (the distinct nodes of those elements, ascending) are
+// staged once into LDS with coalesced loads: p (Dirichlet DOFs masked) and the FP64
+// coordinates.  Each thread takes its element's 8 nodes from LDS (local indices lidx), forms the
+// element forces exactly as mf_apply_hex8_kernel, and writes them to LDS; then each local node's
+// forces are summed in element order (incidence list linc) into one partial per (workgroup,
+// node), stored at its slot lpos of the node-ordered partial list.  The gather then sums a
+// node's partials (1 to 8, ~2.4 on a box) in workgroup order: the element-order sum of the
+// per-slot path, grouped by workgroup.  Versus one slot per (element, node): 8 p and 8 coordinate
+// gathers per element become ~2.5 staged loads, and 8 scattered 12-byte stores become ~2.5.
+constexpr int kBrick = 256;       // elements per workgroup
+constexpr int kBrickNodes = 768;  // local nodes a workgroup can stage (an interior hexa8 pencil: 650)
+
+template <int NC, typename TP>
+__global__ __launch_bounds__(kBrick) void mfb_apply_hex8_kernel(
+    int E, int N, const int32_t* __restrict__ lptr, const int32_t* __restrict__ lnode,
+    const int32_t* __restrict__ lpos, const int32_t* __restrict__ lincp, const uint16_t* __restrict__ linc,
+    const uint16_t* __restrict__ lidx, const double* __restrict__ coords, const float* __restrict__ mfC,
+    const TP* __restrict__ p, const uint8_t* __restrict__ dirflag, int mask, const int* __restrict__ flags,
+    float* __restrict__ Fpart) {
+    // LDS, reused: first the staged node data (p 3 x 768 floats, x 3 x 768 doubles), then the
+    // element forces (256 x 8 x 3 floats)
+    __shared__ double lds[3 * kBrickNodes + (3 * kBrickNodes + 1) / 2];
+    double* sx = lds;
+    float* sp = (float*)(lds + 3 * kBrickNodes);
+    float* sf = (float*)lds;
+    if (flags && flags[0]) return;
+    const int g = xcd_block_mf();
+    if (g * kBrick >= E) return;  // the grid is rounded up to a multiple of 8
+    const int t = threadIdx.x;
+    const int l0 = lptr[g], nl = lptr[g + 1] - l0;
+    for (int i = t; i < nl; i += kBrick) {
+        const int nd = lnode[l0 + i];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            float v = (float)p[3 * nd + c];
+            if (mask && dirflag[3 * nd + c]) v = 0.0f;
+            sp[c * kBrickNodes + i] = v;
+            sx[c * kBrickNodes + i] = coords[(int64_t)c * N + nd];
+        }
+    }
+    __syncthreads();
+    const int e = g * kBrick + t;
+    const bool live = e < E;
+    float cp[3][8], cx[3][8];
+    if (live) {
+        // 8 local indices in one 16-byte load
+        const uint4 li4 = *reinterpret_cast<const uint4*>(lidx + (int64_t)e * 8);
+        const uint32_t lw[4] = {li4.x, li4.y, li4.z, li4.w};
+        int loc[8];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) loc[a] = (int)((lw[a >> 1] >> (16 * (a & 1))) & 0xffffu);
+        double x0[3];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int l = loc[kHex8Canon[k]];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                cp[i][k] = sp[i * kBrickNodes + l];
+                const double xc = sx[i * kBrickNodes + l];
+                if (k == 0) x0[i] = xc;
+                cx[i][k] = (float)(xc - x0[i]);
+            }
+        }
+    }
+    __syncthreads();  // the staging area becomes the force area
+    if (live) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            walsh3(cp[i]);
+            walsh3(cx[i]);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                cp[i][k] *= 0.125f;
+                cx[i][k] *= 0.125f;
+            }
+        }
+        float fa[3][8];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) fa[i][k] = 0.0f;
+        const float gq = 0.57735026918962576451f;
+#pragma unroll 1
+        for (int q = 0; q < 8; ++q) {
+            const int r = q & 3;
+            const float x = (r < 2 ? -gq : gq), y = (q < 4 ? -gq : gq), z = ((r == 1 || r == 2) ? gq : -gq);
+            const float xy = x * y, yz = y * z, xz = x * z;
+            float H[9], J[9];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                H[i * 3 + 0] = cp[i][1] + cp[i][3] * y + cp[i][5] * z + cp[i][7] * yz;
+                H[i * 3 + 1] = cp[i][2] + cp[i][3] * x + cp[i][6] * z + cp[i][7] * xz;
+                H[i * 3 + 2] = cp[i][4] + cp[i][5] * x + cp[i][6] * y + cp[i][7] * xy;
+                J[0 * 3 + i] = cx[i][1] + cx[i][3] * y + cx[i][5] * z + cx[i][7] * yz;
+                J[1 * 3 + i] = cx[i][2] + cx[i][3] * x + cx[i][6] * z + cx[i][7] * xz;
+                J[2 * 3 + i] = cx[i][4] + cx[i][5] * x + cx[i][6] * y + cx[i][7] * xy;
+            }
+            const float det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
+                              J[2] * (J[3] * J[7] - J[4] * J[6]);
+            const float id = 1.0f / det;
+            const float iJ[9] = {(J[4] * J[8] - J[5] * J[7]) * id, (J[2] * J[7] - J[1] * J[8]) * id,
+                                 (J[1] * J[5] - J[2] * J[4]) * id, (J[5] * J[6] - J[3] * J[8]) * id,
+                                 (J[0] * J[8] - J[2] * J[6]) * id, (J[2] * J[3] - J[0] * J[5]) * id,
+                                 (J[3] * J[7] - J[4] * J[6]) * id, (J[1] * J[6] - J[0] * J[7]) * id,
+                                 (J[0] * J[4] - J[1] * J[3]) * id};
+            float gr[9];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    gr[i * 3 + j] = H[i * 3] * iJ[j * 3] + H[i * 3 + 1] * iJ[j * 3 + 1] + H[i * 3 + 2] * iJ[j * 3 + 2];
+            const float ep[6] = {gr[0], gr[4], gr[8], gr[1] + gr[3], gr[2] + gr[6], gr[5] + gr[7]};
+            float cf[NC];
+#pragma unroll
+            for (int k = 0; k < NC; ++k) cf[k] = __builtin_nontemporal_load(mfC + (int64_t)(q * NC + k) * E + e);
+            const float A = cf[0], B = cf[1], cnn = cf[2];
+            const float cn1 = NC == 11 ? cf[3] : 0.0f, c1n = NC == 11 ? cf[4] : 0.0f;
+            const float* n = cf + (NC == 11 ? 5 : 3);
+            const float tr = ep[0] + ep[1] + ep[2];
+            const float ne = n[0] * ep[0] + n[1] * ep[1] + n[2] * ep[2] + n[3] * ep[3] + n[4] * ep[4] + n[5] * ep[5];
+            const float t3 = tr * (1.0f / 3.0f);
+            const float sn = cnn * ne + cn1 * tr;
+            const float s1 = A * tr + c1n * ne;
+            float sg[6];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) sg[i] = s1 + B * (ep[i] - t3) + sn * n[i];
+#pragma unroll
+            for (int i = 3; i < 6; ++i) sg[i] = 0.5f * B * ep[i] + sn * n[i];
+            const float S[9] = {sg[0], sg[3], sg[4], sg[3], sg[1], sg[5], sg[4], sg[5], sg[2]};
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const float m0 = S[i * 3] * iJ[0] + S[i * 3 + 1] * iJ[3] + S[i * 3 + 2] * iJ[6];
+                const float m1 = S[i * 3] * iJ[1] + S[i * 3 + 1] * iJ[4] + S[i * 3 + 2] * iJ[7];
+                const float m2 = S[i * 3] * iJ[2] + S[i * 3 + 1] * iJ[5] + S[i * 3 + 2] * iJ[8];
+                fa[i][1] += m0;
+                fa[i][2] += m1;
+                fa[i][4] += m2;
+                fa[i][3] += m0 * y + m1 * x;
+                fa[i][5] += m0 * z + m2 * x;
+                fa[i][6] += m1 * z + m2 * y;
+                fa[i][7] += m0 * yz + m1 * xz + m2 * xy;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) walsh3_t(fa[i]);
+        // forces [i][t * 8 + a] (component-major: lanes t write 8 apart, then the node sums read
+        // runs of one component)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int a = kHex8Canon[k];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) sf[i * (kBrick * 8) + t * 8 + a] = 0.125f * fa[i][k];
+        }
+    }
+    __syncthreads();
+    // one partial per local node: its incidences in element order
+    for (int i = t; i < nl; i += kBrick) {
+        float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+        for (int c = lincp[l0 + i]; c < lincp[l0 + i + 1]; ++c) {
+            const int s = linc[c];
+            f0 += sf[s];
+            f1 += sf[kBrick * 8 + s];
+            f2 += sf[2 * kBrick * 8 + s];
+        }
+        float* dst = Fpart + (int64_t)lpos[l0 + i] * 3;
+        dst[0] = f0;
+        dst[1] = f1;
+        dst[2] = f2;
+    }
+}
+
 // Batched solves: q = K p on the nodes of active specimens and the per-node p.q values
 // that the per-specimen reductions sum; nodes of frozen specimens are left alone (p.q 0).
 __global__ __launch_bounds__(256) void mf_gather_batched_kernel(int N, const int32_t* __restrict__ nptr,
@@ -524,6 +698,15 @@ void launch_apply(efe_ctx* c, hipStream_t st, const TP* p, int mask, const int* 
                   const int* active = nullptr) {
     const int grid = round8_mf((c->E + 255) / 256);
     const bool dp = c->mfNC == 11;
+    if (c->mfb && !active) {  // bricks (hexa8); grid: exactly the workgroups, a multiple of 8
+#define MFB_ARGS                                                                                                 \
+    c->E, c->N, c->mfbLptr, c->mfbLnode, c->mfbLpos, c->mfbLincp, c->mfbLinc, c->mfbLidx, c->coords, c->mfC, p, \
+        c->dirflag, mask, flags, c->mfbF
+        if (dp) mfb_apply_hex8_kernel<11, TP><<<grid, kBrick, 0, st>>>(MFB_ARGS);
+        else mfb_apply_hex8_kernel<9, TP><<<grid, kBrick, 0, st>>>(MFB_ARGS);
+#undef MFB_ARGS
+        return;
+    }
     switch (c->eltype) {
         case EFE_C3D8:
             if (dp) mf_apply_hex8_kernel<11, TP><<<grid, 256, 0, st>>>(MF_ARGS);
@@ -540,13 +723,91 @@ void launch_apply(efe_ctx* c, hipStream_t st, const TP* p, int mask, const int* 
     }
 }
 
+// the per-node force list the gather sums: brick partials, or one slot per (element, node)
+inline const int32_t* gather_ptr(const efe_ctx* c, bool batched) { return (c->mfb && !batched) ? c->mfbPptr : c->nptr; }
+inline const float* gather_forces(const efe_ctx* c, bool batched) { return (c->mfb && !batched) ? c->mfbF : c->mfF; }
+
 template <int BS>
 void launch_gather_op(efe_ctx* c, hipStream_t st, const float* p, float* q, double* part, const int* flags,
                       int nwg, int kr) {
-    mf_gather_kernel<BS, 0, float, float><<<nwg, BS, 0, st>>>(c->N, c->nptr, c->nlist, c->mfF, p, c->dirflag, q,
-                                                              part, flags, kr);
+    mf_gather_kernel<BS, 0, float, float><<<nwg, BS, 0, st>>>(c->N, gather_ptr(c, false), c->nlist,
+                                                              gather_forces(c, false), p, c->dirflag, q, part,
+                                                              flags, kr);
 }
 
+}  // namespace
+
+namespace {
+// Brick tables of the hexa8 operator (single-system contexts); left off when a workgroup's
+// elements touch more than kBrickNodes nodes (a mesh in an order without locality).
+int mfb_build(efe_ctx* c) {
+    if (c->eltype != EFE_C3D8 || c->specOfNode || c->mfb) return EFE_OK;
+    const int E = c->E, N = c->N, G = (E + kBrick - 1) / kBrick;
+    std::vector<int32_t> conn((size_t)8 * E);  // [a][e]
+    EFE_HIP(hipMemcpy(conn.data(), c->conn, conn.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<int32_t> lptr(G + 1, 0), lnode, lincp, pcount(N, 0);
+    std::vector<uint16_t> linc((size_t)8 * E), lidx((size_t)8 * E);
+    lnode.reserve((size_t)3 * E);
+    lincp.reserve((size_t)3 * E + 1);
+    std::vector<int32_t> mark(N, -1);
+    std::vector<int32_t> nodes, cnt, fill;
+    for (int g = 0; g < G; ++g) {
+        const int e0 = g * kBrick, e1 = std::min(E, e0 + kBrick);
+        nodes.clear();
+        for (int e = e0; e < e1; ++e)
+            for (int a = 0; a < 8; ++a) nodes.push_back(conn[(size_t)a * E + e]);
+        std::sort(nodes.begin(), nodes.end());
+        nodes.erase(std::unique(nodes.begin(), nodes.end()), nodes.end());
+        const int nl = (int)nodes.size();
+        if (nl > kBrickNodes) return EFE_OK;  // no locality: keep the per-slot operator
+        for (int i = 0; i < nl; ++i) mark[nodes[i]] = i;
+        cnt.assign(nl + 1, 0);
+        for (int e = e0; e < e1; ++e)
+            for (int a = 0; a < 8; ++a) {
+                const int l = mark[conn[(size_t)a * E + e]];
+                lidx[(size_t)e * 8 + a] = (uint16_t)l;
+                ++cnt[l + 1];
+            }
+        for (int i = 0; i < nl; ++i) cnt[i + 1] += cnt[i];
+        const int base = g * kBrick * 8;  // linc entries of workgroup g start at its first element's
+        fill.assign(cnt.begin(), cnt.end() - 1);
+        for (int e = e0; e < e1; ++e)  // element order inside every local node's list
+            for (int a = 0; a < 8; ++a) {
+                const int l = mark[conn[(size_t)a * E + e]];
+                linc[(size_t)base + fill[l]++] = (uint16_t)((e - e0) * 8 + a);
+            }
+        for (int i = 0; i < nl; ++i) {
+            lnode.push_back(nodes[i]);
+            lincp.push_back(base + cnt[i]);
+            ++pcount[nodes[i]];
+            mark[nodes[i]] = -1;
+        }
+        lptr[g + 1] = (int32_t)lnode.size();
+    }
+    lincp.push_back(8 * E);
+    // partial slots in node order, the workgroups of a node in ascending order
+    const int64_t L = (int64_t)lnode.size();
+    std::vector<int32_t> pptr(N + 1, 0), lpos(L);
+    for (int n = 0; n < N; ++n) pptr[n + 1] = pptr[n] + pcount[n];
+    std::vector<int32_t> next(pptr.begin(), pptr.end() - 1);
+    for (int64_t i = 0; i < L; ++i) lpos[i] = next[lnode[i]]++;
+    // the end of each local node's incidence list is the start of the next one's; the last
+    // local node of a workgroup ends where the next workgroup's first begins (lincp[L] = 8 E)
+    auto up = [&](auto** dst, const auto& v) -> int {
+        using T = typename std::remove_reference<decltype(v[0])>::type;
+        EFE_HIP(hipMalloc((void**)dst, (v.size() ? v.size() : 1) * sizeof(T)));
+        if (!v.empty()) EFE_HIP(hipMemcpy((void*)*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return EFE_OK;
+    };
+    int rc;
+    if ((rc = up(&c->mfbLptr, lptr)) || (rc = up(&c->mfbLnode, lnode)) || (rc = up(&c->mfbLpos, lpos)) ||
+        (rc = up(&c->mfbLincp, lincp)) || (rc = up(&c->mfbPptr, pptr)) || (rc = up(&c->mfbLinc, linc)) ||
+        (rc = up(&c->mfbLidx, lidx)))
+        return rc;
+    EFE_HIP(hipMalloc(&c->mfbF, (size_t)L * 3 * sizeof(float)));
+    c->mfb = true;
+    return EFE_OK;
+}
 }  // namespace
 
 int efe_mf_alloc(efe_ctx* c) {
@@ -567,7 +828,7 @@ int efe_mf_alloc(efe_ctx* c) {
     for (size_t i = 0; i < 2 * nd; ++i) hf[i] = (float)h[i];
     EFE_HIP(hipMalloc(&c->dnxi32, 2 * nd * sizeof(float)));
     EFE_HIP(hipMemcpy(c->dnxi32, hf.data(), 2 * nd * sizeof(float), hipMemcpyHostToDevice));
-    return EFE_OK;
+    return mfb_build(c);
 }
 
 // Diagonal blocks, Minv, P, F and (optionally) the right-hand-side correction K[:, dir] dir_values.
@@ -580,7 +841,8 @@ int efe_launch_mf_assembly(efe_ctx* c, double* P, double* F, const double* dir_v
     if (dir_values) {
         launch_apply<double>(c, c->stream, dir_values, 0, nullptr);
         mf_gather_kernel<256, 1, double, double><<<round8_mf(grid), 256, 0, c->stream>>>(
-            c->N, c->nptr, c->nlist, c->mfF, dir_values, c->dirflag, dir_correction, nullptr, nullptr, 1);
+            c->N, gather_ptr(c, false), c->nlist, gather_forces(c, false), dir_values, c->dirflag, dir_correction,
+            nullptr, nullptr, 1);
         EFE_HIP(hipGetLastError());
     }
     c->Kprec = EFE_FP32_MATFREE;
@@ -605,7 +867,7 @@ void efe_mf_launch_operator(efe_ctx* c, hipStream_t st, const float* p, float* q
 int efe_launch_spmv_mf(efe_ctx* c, const double* x, double* y) {
     launch_apply<double>(c, c->stream, x, 1, nullptr);
     mf_gather_kernel<256, 0, double, double><<<round8_mf((c->N + 255) / 256), 256, 0, c->stream>>>(
-        c->N, c->nptr, c->nlist, c->mfF, x, c->dirflag, y, nullptr, nullptr, 1);
+        c->N, gather_ptr(c, false), c->nlist, gather_forces(c, false), x, c->dirflag, y, nullptr, nullptr, 1);
     EFE_HIP(hipGetLastError());
     return EFE_OK;
 }
@@ -623,4 +885,11 @@ void efe_mf_free(efe_ctx* c) {
     hipFree(c->mfF);
     hipFree(c->dnxi32);
     c->mfC = c->mfF = c->dnxi32 = nullptr;
+    void* b[] = {c->mfbLptr, c->mfbLnode, c->mfbLpos, c->mfbLincp, c->mfbPptr, c->mfbLinc, c->mfbLidx, c->mfbF};
+    for (void* q : b)
+        if (q) hipFree(q);
+    c->mfbLptr = c->mfbLnode = c->mfbLpos = c->mfbLincp = c->mfbPptr = nullptr;
+    c->mfbLinc = c->mfbLidx = nullptr;
+    c->mfbF = nullptr;
+    c->mfb = false;
 }

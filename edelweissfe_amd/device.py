@@ -174,6 +174,32 @@ class DeviceSystem:
                                                 ctypes.c_void_p(Pe.data_ptr())), "efe_get_element_matrices")
         return Ke, Pe
 
+    def element_compute(self, elements, dQ, stateRef, stateTemp, Pe, Ke=None, time=(0.0, 0.0), dT=0.0, QTotal=None):
+        """The native element ABI for the listed elements (efe_element_compute): dQ [n][nDof],
+        stateRef / stateTemp [n][nQP][nState], Pe [n][nDof] and Ke [n][nDof][nDof] (column-major,
+        or None) are float64 device tensors in the caller's element order.  Returns pNewDT
+        (1e36, or at most 0.5 when a material asked for a smaller increment)."""
+        el = np.ascontiguousarray(elements, dtype=np.int32)
+        nd = 3 * (8 if self.el_type == "C3D8" else 20)
+        n = el.size
+
+        def chk(t, shape, name):
+            if t is None:
+                return None
+            if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and tuple(t.shape) == shape):
+                raise ValueError("{} must be a contiguous float64 CUDA tensor of shape {}".format(name, shape))
+            return ctypes.c_void_p(t.data_ptr())
+
+        args = [chk(dQ, (n, nd), "dQ"), chk(stateRef, (n, self.n_qp, self.n_state), "stateRef"),
+                chk(stateTemp, (n, self.n_qp, self.n_state), "stateTemp"), chk(Pe, (n, nd), "Pe"),
+                chk(Ke, (n, nd, nd), "Ke")]
+        t = (ctypes.c_double * 2)(*time)
+        pnew = ctypes.c_double(1e36)
+        rc = self.lib.efe_element_compute(self.ctx, el.ctypes.data, n, chk(QTotal, (n, nd), "QTotal"), args[0],
+                                          args[1], args[2], args[3], args[4], t, float(dT), ctypes.byref(pnew))
+        check(rc, "efe_element_compute")
+        return pnew.value
+
     def export_csr_values(self):
         out = torch.empty(self.nnz, dtype=torch.float64, device=self.device)
         check(self.lib.efe_export_csr_values(self.ctx, ctypes.c_void_p(out.data_ptr())), "efe_export_csr_values")

@@ -53,11 +53,21 @@ class FEModel:
 
     def assignMaterial(self, name, properties, elementSet=None):
         """Material by name (linearelastic, vonmises, druckerprager) and properties (one row, or
-        one row per element it covers) on every element, or on the elements of `elementSet` (a
+        one row per element it covers) on every element (`name` may also be one name per element),
+        or on the elements of `elementSet` (a
         set name or element indices): a section (sections/base/sectionbase.py:86-128), a later
         assignment replacing an earlier one on the same elements.  With several materials,
         model.material holds one name and one row of 6 properties per element."""
         props = np.asarray(properties, dtype=np.float64)
+        if elementSet is None and not isinstance(name, str):  # a name per element
+            names = np.array([str(v).lower() for v in np.asarray(name).ravel()], dtype=object)
+            if names.size != self.nElements:
+                raise ValueError("one material name per element needed")
+            rows = np.zeros((self.nElements, 6))
+            p = np.atleast_2d(props)
+            rows[:, :p.shape[1]] = p if p.shape[0] == self.nElements else np.repeat(p, self.nElements, axis=0)
+            self.material = {"name": names, "properties": rows}
+            return
         if elementSet is None:
             self.material = {"name": name.lower(), "properties": props}
             return

@@ -125,16 +125,16 @@ class ModelMap:
         if key not in self._sets:
             name = "_ref_nset_%d" % len(self._sets)
             self.dm.nodeSets[name] = np.array([self.nodePos[n] for n in nodes], dtype=np.int64)
-            self._sets[key] = name
-        return self._sets[key]
+            self._sets[key] = (name, nodes)  # the set kept alive: its id stays its own
+        return self._sets[key][0]
 
     def elementSet(self, elements):
         key = ("e", id(elements))
         if key not in self._sets:
             name = "_ref_elset_%d" % len(self._sets)
             self.dm.elementSets[name] = np.array([self.elementPos[el] for el in elements], dtype=np.int64)
-            self._sets[key] = name
-        return self._sets[key]
+            self._sets[key] = (name, elements)  # the set kept alive: its id stays its own
+        return self._sets[key][0]
 
     def surface(self, surface):
         key = ("s", id(surface))
@@ -142,8 +142,8 @@ class ModelMap:
             name = "_ref_surface_%d" % len(self._sets)
             self.dm.surfaces[name] = {int(f): np.array([self.elementPos[el] for el in els], dtype=np.int64)
                                       for f, els in surface.items()}
-            self._sets[key] = name
-        return self._sets[key]
+            self._sets[key] = (name, surface)  # the set kept alive: its id stays its own
+        return self._sets[key][0]
 
     def readField(self, name):
         return np.asarray(self.field[name], dtype=float).reshape(-1)
@@ -300,7 +300,9 @@ class NISTGPUDropIn:
         state = dm.qpState
         for el, st in zip(mm.elements, state):
             ref = getattr(el, "_stateVarsRef", None)
-            if ref is not None and np.shape(ref) == st.shape:
-                ref[...] = st
+            # [QP][stress 6, strain 6, material state]: an element whose material has fewer
+            # state variables than the widest of the model takes its leading columns
+            if ref is not None and np.ndim(ref) == 2 and ref.shape[0] == st.shape[0] and ref.shape[1] <= st.shape[1]:
+                ref[...] = st[:, :ref.shape[1]]
         model.time = dm.time
         return ok, model

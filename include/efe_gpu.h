@@ -58,7 +58,7 @@ extern "C" {
 
 #define EFE_SOLVER_CG 0
 #define EFE_SOLVER_GMRES 1
-#define EFE_SOLVER_BICGSTAB 2  /* scalar CSR solves (efe_csr_linsolve) only */
+#define EFE_SOLVER_BICGSTAB 2  /* scalar CSR solves (efe_csr_linsolve*) only */
 #define EFE_PRECOND_BLOCK_JACOBI 0
 #define EFE_PRECOND_ILU0 1
 
@@ -234,8 +234,9 @@ int efe_get_specimen_count(efe_ctx* ctx);
  * constrained rows are unit rows while their columns stay.  A row whose only stored entry is a
  * unit diagonal is prescribed: x = b there, and its column moves to the right-hand side, so the
  * Krylov method solves the free block (symmetric when K is) K_ff x_f = b_f - K_fd b_d.
- * opts->solver: EFE_SOLVER_CG (symmetric K) or EFE_SOLVER_BICGSTAB; Jacobi preconditioner
- * (opts->precond and precision are ignored: FP64); stop ||r_f||_2 <= max(rtol ||b_f'||, atol).
+ * opts->solver: EFE_SOLVER_CG (symmetric K), EFE_SOLVER_GMRES (restarted GMRES(opts->restart),
+ * non-symmetric K) or EFE_SOLVER_BICGSTAB; Jacobi preconditioner (opts->precond and precision are
+ * ignored: FP64); stop ||r_f||_2 <= max(rtol ||b_f'||, atol).
  * indptr host [n+1] (0 first, non-decreasing), indices host [indptr[n]] in [0, n), data host
  * [indptr[n]], b and x host [n].  Allocates device memory for the call and frees it. */
 int efe_csr_linsolve(int32_t n, const int32_t* indptr, const int32_t* indices, const double* data, const double* b,
@@ -244,7 +245,7 @@ int efe_csr_linsolve(int32_t n, const int32_t* indptr, const int32_t* indices, c
 /* efe_csr_linsolve on device arrays (indptr, indices, data, b, x device; x may not alias b),
  * enqueued on `stream` (hipStream_t, NULL = default); the pattern is validated on the device
  * first (EFE_ERR_ARG when indptr or an index is out of range).  Allocates its workspace for the
- * call (efe_csr_linsolve_device_noalloc does not). */
+ * call. */
 int efe_csr_linsolve_device(int32_t n, const int32_t* indptr, const int32_t* indices, const double* data,
                             const double* b, double* x, void* stream, const efe_solve_opts* opts,
                             efe_solve_info* info);
@@ -266,6 +267,24 @@ int efe_get_qp_state(efe_ctx* ctx, int32_t which, double* out);
  * (EFE_ERR_STATE after a matrix-free or residual-only pass), and for Pe no load computation
  * (efe_body_force, efe_surface_traction) since. */
 int efe_get_element_matrices(efe_ctx* ctx, const int32_t* elements, int32_t n, double* Ke, double* Pe);
+
+/* The native element ABI, MarmotElement::computeYourself(QTotal, dQ, Pe, Ke, time, dT, pNewDT)
+ * (marmotelement/element.pxd:97-103; called per element by NISTParallelForMarmotElements,
+ * solvers/nonlinearimplicitstaticparallel.pyx:65-167), for the n elements elements[i] (host [n])
+ * of the context in one launch, with caller-owned device arrays in the ABI's layouts:
+ *   dQ        [n][nDof]          DOF increments, node-major (3 a + c); QTotal [n][nDof] is not read
+ *                                (the small-strain element uses the increment only, element.py:290-346)
+ *   stateRef  [n][nQP][nState]   the element's state at the increment start (Marmot's state buffer
+ *                                after _initializeStateVarsTemp; element.py:222-236 layout)
+ *   stateTemp [n][nQP][nState]   receives the updated state (may equal stateRef)
+ *   Pe        [n][nDof]          receives -sum B^T sigma detJ w (overwritten)
+ *   Ke        [n][nDof][nDof]    receives sum B^T C B detJ w column-major (overwritten), or NULL
+ * nState = n_state_per_qp of efe_get_sizes.  The element formulation, material and properties
+ * are the context's.  A material asking for a smaller increment lowers *pNewDT (in/out, as the
+ * caller's pNewDT initialised to 1e36) to at most 0.5 and returns EFE_CUTBACK. */
+int efe_element_compute(efe_ctx* ctx, const int32_t* elements, int32_t n, const double* QTotal, const double* dQ,
+                        const double* stateRef, double* stateTemp, double* Pe, double* Ke, const double time[2],
+                        double dT, double* pNewDT);
 
 /* Device gather of QP results (utils/elementresultcollector.pyx:37-132, which collects
  * element.getResultArray(result, qp) for a list of elements and quadrature points): out
