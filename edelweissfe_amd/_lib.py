@@ -104,6 +104,8 @@ SIGNATURES = {
     "efe_csr_linsolve": (_I, [_I, _P, _P, _P, _P, _P, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]),
     "efe_csr_linsolve_device": (_I, [_I, _P, _P, _P, _P, _P, _P, ctypes.POINTER(SolveOpts),
                                      ctypes.POINTER(SolveInfo)]),
+    "efe_csr_linsolve_device_noalloc": (_I, [_I, _P, _P, _P, _P, _P, _P, ctypes.POINTER(SolveOpts),
+                                              ctypes.POINTER(SolveInfo)]),
 }
 
 _lib = None

@@ -9,7 +9,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libefe_gpu.so")
 SOURCES = ["capi.hip", "elements.hip", "elemabi.hip", "krylov.hip", "gmres.hip", "batched.hip", "matfree.hip",
-           "csrsolve.hip", "pattern.cpp"]
+           "csrsolve.hip", "csrnoalloc.hip", "pattern.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

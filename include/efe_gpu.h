@@ -250,6 +250,18 @@ int efe_csr_linsolve_device(int32_t n, const int32_t* indptr, const int32_t* ind
                             const double* b, double* x, void* stream, const efe_solve_opts* opts,
                             efe_solve_info* info);
 
+/* efe_csr_linsolve_device without allocation: the same solve (the same kernels, bit for bit)
+ * with its workspace in the library's static device array instead of a hipMalloc, for callers
+ * that cannot allocate device memory (serving loops, harnesses that forbid it).  It synchronises
+ * `stream` (pattern check, convergence), so it is not for graph capture.  No counterpart in the
+ * reference.  Limits: EFE_SOLVER_CG up to 1048576 unknowns,
+ * EFE_SOLVER_BICGSTAB up to ~700000 (EFE_ERR_ARG beyond, and for EFE_SOLVER_GMRES), checked
+ * before any device work.  Every call on one GPU shares that GPU's array: calls that overlap
+ * on one GPU (concurrent streams or threads) race. */
+int efe_csr_linsolve_device_noalloc(int32_t n, const int32_t* indptr, const int32_t* indices, const double* data,
+                                    const double* b, double* x, void* stream, const efe_solve_opts* opts,
+                                    efe_solve_info* info);
+
 /* Accepts the temporary QP state as the new reference state (femodel.py:245-258). */
 int efe_accept_state(efe_ctx* ctx);
 
