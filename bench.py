@@ -251,6 +251,19 @@ def bench_batched(args):
         "specimen_newton_iterations": int(solves64.sum()),
         "autotune": {k: v for k, v in (s64.tuning or {}).items() if k != "timings"},
     }
+    # the same FP64 batch with the Newton decisions on the host (one readback and ~15 launches
+    # per pass) against the device-resident rounds timed above
+    sh = NISTBatched(precision="fp64", newtonOnDevice=False)
+    sh.attach(s64)
+    th, mh, _ = run_batched(sh, count, args.strain, inc, args.steps, args.warmup)
+    wallh = sum(th) / len(th)
+    result["host_newton_rounds"] = {
+        "ms_per_batch_history": 1000.0 * wallh,
+        "device_round_speedup": wallh / wall64,
+        "same_results_as_device_rounds": bool([list(h) for h in sh.history] == hist64 and
+                                              all(np.array_equal(a.U, b.U) for a, b in zip(mh, m64))),
+    }
+    log("[bench] batched fp64, host Newton rounds: %.3f s (device rounds %.3f s)" % (wallh, wall64))
     if not args.no_mixed:
         s32 = NISTBatched(precision="mixed")
         s32.attach(s64)

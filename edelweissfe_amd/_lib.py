@@ -58,6 +58,30 @@ class SolveOpts(ctypes.Structure):
     ]
 
 
+class NewtonOpts(ctypes.Structure):
+    """efe_newton_opts"""
+    _fields_ = [
+        ("precision", ctypes.c_int32),
+        ("krylov_max_iter", ctypes.c_int32),
+        ("check_every", ctypes.c_int32),
+        ("mixed_first", ctypes.c_int32),
+        ("alt_after", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("rtol", ctypes.c_double),
+        ("rtol_first", ctypes.c_double),
+        ("atol_factor", ctypes.c_double),
+        ("flux_tol", ctypes.c_double),
+        ("flux_tol_alt", ctypes.c_double),
+        ("correction_tol", ctypes.c_double),
+        ("flux_floor", ctypes.c_double),
+        ("avg_floor", ctypes.c_double),
+    ]
+
+
+EFE_NEWTON_ITERATING, EFE_NEWTON_CONVERGED, EFE_NEWTON_FAILED, EFE_NEWTON_IDLE = 0, 1, 2, 3
+EFE_FAIL_NONE, EFE_FAIL_CUTBACK, EFE_FAIL_NAN, EFE_FAIL_ITERATIONS, EFE_FAIL_KRYLOV = 0, 1, 2, 3, 4
+
+
 class SolveInfo(ctypes.Structure):
     _fields_ = [
         ("iterations", ctypes.c_int32),
@@ -92,6 +116,7 @@ SIGNATURES = {
     "efe_convergence_norms_batched": (_I, [_P, _P, _P, _P, _P]),
     "efe_linsolve_batched": (_I, [_P, _P, _P, ctypes.POINTER(SolveOpts), _P, _P, _P, _P, ctypes.POINTER(SolveInfo)]),
     "efe_accept_state_batched": (_I, [_P, _P]),
+    "efe_newton_round_batched": (_I, [_P] + [_P] * 11 + [ctypes.POINTER(NewtonOpts)] + [_P] * 4),
     "efe_get_specimen_count": (_I, [_P]),
     "efe_get_specimen_status": (_I, [_P, _P]),
     "efe_accept_state": (_I, [_P]),

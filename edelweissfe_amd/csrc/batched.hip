@@ -10,6 +10,7 @@
 // A converged specimen is frozen (its rows are skipped), so each specimen follows exactly
 // the iteration of a PCG run on its own system.
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "efe_internal.h"
@@ -33,18 +34,17 @@ struct BatchWork {
     hipGraphExec_t graph = nullptr;
     int graph_batch = 0;
     int graph_op = -1;  // precision code the captured graph was built for (operator choice)
-    // device-resident Newton rounds (efe_newton_round_batched): per specimen its Newton state
-    // (ITERATING 0, CONVERGED 1, FAILED 2, IDLE 3), iteration counter, residual growth count and
-    // previous flux residual, failure reason, first-iteration flag, limits, and work counters;
-    // per DOF the prescribed values of first iterations, the right-hand-side correction, R - corr
-    // and the Newton correction
-    int *nst = nullptr, *nit = nullptr, *ngrow = nullptr, *nreason = nullptr, *nfirst = nullptr;
-    int *nextrap = nullptr, *nhasdir = nullptr, *nmaxit = nullptr, *nmaxgrow = nullptr, *nsolve = nullptr;
-    int *npass = nullptr, *nsolves = nullptr, *nkry = nullptr, *ncount = nullptr;
-    double *ngprev = nullptr, *nacc = nullptr;
+    // device-resident Newton rounds (efe_newton_round_batched): the per-specimen Newton table
+    // ni [S][NI] (inputs of the round, state, counters) and nd [S][ND], the options nopt, the
+    // pass count nreadout [2] (specimens to solve; PCG still active), per-specimen norms of
+    // the last correction nout2 [S][4], and the round's DOF vectors
+    int* ni = nullptr;
+    double *nd = nullptr, *nopt = nullptr, *nout2 = nullptr;
+    int* ncount = nullptr;
+    double *du = nullptr, *nP = nullptr, *nF = nullptr, *pext = nullptr, *dtv = nullptr;
     double *dirv = nullptr, *corr = nullptr, *rhs = nullptr, *ddu = nullptr;
-    hipGraphExec_t ngraph = nullptr;            // one captured Newton iteration
-    const void* ngraph_key[6] = {};             // dU, P, F, PExt, deltaT, precision it was built for
+    hipGraphExec_t ngraph[3] = {};  // pass variants: first pass, first pass with prescribed increments, later
+    int64_t nkey[3][10] = {};       // launch parameters each was captured with
 };
 
 namespace {
@@ -392,6 +392,10 @@ int ensure_batch(efe_ctx* c, int prec) {
         EFE_HIP(hipMalloc(&w->q, D * es));
         if (w->graph) hipGraphExecDestroy(w->graph);
         w->graph = nullptr;
+        for (hipGraphExec_t& g : w->ngraph) {
+            if (g) hipGraphExecDestroy(g);
+            g = nullptr;
+        }
         w->prec = prec;
     }
     return EFE_OK;
@@ -599,18 +603,402 @@ int efe_launch_accept_batched(efe_ctx* c, const uint8_t* accept) {
     return EFE_OK;
 }
 
+// ---- device-resident Newton rounds ----------------------------------------------------------
+namespace {
+
+// per-specimen Newton table ni [S][NI]
+enum { NI_LIVE = 0, NI_EXTRAP, NI_HASDIR, NI_MAXIT, NI_MAXGROW, NI_STATE, NI_IT, NI_GROW, NI_REASON, NI_FIRST,
+       NI_GO, NI_PASSES, NI_SOLVES, NI_KRYLOV, NI };
+enum { ND_NACC = 0, ND_PREV, ND };  // nd [S][ND]: average-flux divisor, previous flux residual
+// options nopt
+enum { OP_RTOL = 0, OP_RTOL_FIRST, OP_ATOLF, OP_FLUX, OP_FLUX_ALT, OP_CORR, OP_FLUX_FLOOR, OP_AVG_FLOOR,
+       OP_MIXED_FIRST, OP_ALT_AFTER, OP_KRY_MAXIT, NOPT };
+
+// max as numpy.maximum: NaN when either is NaN
+__device__ __forceinline__ double np_max(double a, double b) {
+    if (!(a == a)) return a;
+    if (!(b == b)) return b;
+    return a > b ? a : b;
+}
+
+// Head of a later pass: the krylov failures of the last solve (its iteration limit, or a
+// correction with an infinite entry when some specimen's PCG produced NaN), it += 1 and the work
+// counters of the specimens that solved.  One workgroup.
+__global__ __launch_bounds__(BS) void nr_post_kernel(int S, int* __restrict__ ni, const double* __restrict__ sc,
+                                                     const int* __restrict__ iters,
+                                                     const double* __restrict__ dnorm,
+                                                     const double* __restrict__ nopt) {
+    __shared__ int anyNaN;
+    if (threadIdx.x == 0) anyNaN = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int s = threadIdx.x; s < S; s += BS) {
+        const double rr = sc[s * NSC + RR];
+        if (!(rr == rr)) mine = 1;
+    }
+    if (mine) atomicOr(&anyNaN, 1);
+    __syncthreads();
+    const int kmax = (int)nopt[OP_KRY_MAXIT];
+    for (int s = threadIdx.x; s < S; s += BS) {
+        int* t = ni + (size_t)s * NI;
+        if (!t[NI_GO]) continue;
+        const int its = iters[s];
+        const double dmax = dnorm[s * 4 + 0];
+        if (its >= kmax || (anyNaN && !isfinite(dmax))) {
+            t[NI_STATE] = EFE_NEWTON_FAILED;
+            t[NI_REASON] = EFE_FAIL_KRYLOV;
+        }
+        t[NI_IT] += 1;
+        t[NI_SOLVES] += 1;
+        t[NI_KRYLOV] += its;
+    }
+}
+
+// dU += ddU
+__global__ void nr_add_kernel(int64_t D, double* __restrict__ du, const double* __restrict__ ddu) {
+    const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+    if (i < D) du[i] = du[i] + ddu[i];
+}
+
+// Start of a pass: the specimens of the element pass (those iterating), which of them take
+// their first iteration (prescribed increment applied), cleared status words.  One workgroup.
+__global__ __launch_bounds__(BS) void nr_pass_kernel(int S, int* __restrict__ ni, int* __restrict__ specActive,
+                                                     int* __restrict__ status, int* __restrict__ specStatus) {
+    if (threadIdx.x == 0) status[0] = 0;
+    for (int s = threadIdx.x; s < S; s += BS) {
+        int* t = ni + (size_t)s * NI;
+        const bool it = t[NI_STATE] == EFE_NEWTON_ITERATING;
+        t[NI_FIRST] = it && t[NI_IT] == 0 && !t[NI_EXTRAP] && t[NI_HASDIR];
+        specActive[s] = it ? 1 : 0;
+        specStatus[s] = 0;
+        if (it) t[NI_PASSES] += 1;
+    }
+}
+
+// prescribed values of first iterations: deltaT on the specimen's DOFs, 0 elsewhere; and a
+// cleared correction
+__global__ void nr_dirv_kernel(int64_t D, const int32_t* __restrict__ specOfNode, const int* __restrict__ ni,
+                               const double* __restrict__ dtv, double* __restrict__ dirv, double* __restrict__ corr) {
+    const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+    if (i >= D) return;
+    const int s = specOfNode[i / 3];
+    dirv[i] = ni[(size_t)s * NI + NI_FIRST] ? dtv[i] : 0.0;
+    corr[i] = 0.0;
+}
+
+// R = P + PExt, prescribed DOFs: the first iterations' prescribed value, else 0
+__global__ void nr_residual_kernel(int64_t D, const double* __restrict__ P, const double* __restrict__ pext,
+                                   const uint8_t* __restrict__ dirflag, int ndir, const double* __restrict__ dirv,
+                                   double* __restrict__ R) {
+    const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+    if (i >= D) return;
+    double r = pext ? P[i] + pext[i] : P[i];
+    if (ndir && dirflag[i]) r = dirv ? dirv[i] : 0.0;
+    R[i] = r;
+}
+
+// b = R - correction
+__global__ void nr_sub_kernel(int64_t D, double* __restrict__ R, const double* __restrict__ corr) {
+    const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+    if (i < D) R[i] = R[i] - corr[i];
+}
+
+// The convergence test of every specimen (nonlinearimplicitstatic.py:625-725 with the loop's
+// divergence and iteration limits :405-458): cutback requests of the element pass, NaN, converged
+// (correction < tol and flux <= max(tol_flux avg, floor)), growing residual or maxIter; then the
+// specimens still iterating solve, with the PCG tolerances of the batched solve.  `hasDdu`: a
+// correction exists (every pass after the round's first).  One workgroup; cnt[0] = solving count.
+__global__ __launch_bounds__(BS) void nr_decide_kernel(int S, int hasDdu, int* __restrict__ ni,
+                                                       double* __restrict__ nd, const double* __restrict__ nout,
+                                                       const int* __restrict__ specStatus,
+                                                       const double* __restrict__ nopt, int* __restrict__ active,
+                                                       double* __restrict__ sc, int* __restrict__ cnt) {
+    __shared__ int count;
+    if (threadIdx.x == 0) count = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int s = threadIdx.x; s < S; s += BS) {
+        int* t = ni + (size_t)s * NI;
+        double* u = nd + (size_t)s * ND;
+        int st = t[NI_STATE];
+        if (st == EFE_NEWTON_ITERATING && specStatus[s]) {
+            st = EFE_NEWTON_FAILED;
+            t[NI_REASON] = EFE_FAIL_CUTBACK;
+        }
+        const int it = t[NI_IT];
+        const double avg = np_max(nopt[OP_AVG_FLOOR], nout[s * 4 + 2] / u[ND_NACC]);
+        const double tolF = it < (int)nopt[OP_ALT_AFTER] ? nopt[OP_FLUX] : nopt[OP_FLUX_ALT];
+        const double thr = np_max(tolF * avg, nopt[OP_FLUX_FLOOR]);
+        const int first = t[NI_FIRST];
+        if (st == EFE_NEWTON_ITERATING && !first) {
+            const double flux = nout[s * 4 + 0];
+            const double correction = hasDdu ? nout[s * 4 + 1] : 0.0;
+            const double prev = u[ND_PREV];
+            const int grew = t[NI_GROW] + (flux > prev ? 1 : 0);
+            u[ND_PREV] = flux;
+            t[NI_GROW] = grew;
+            if (nout[s * 4 + 3] != 0.0) {
+                st = EFE_NEWTON_FAILED;
+                t[NI_REASON] = EFE_FAIL_NAN;
+            } else if (correction < nopt[OP_CORR] && flux <= thr) {
+                st = EFE_NEWTON_CONVERGED;
+            } else if (grew > t[NI_MAXGROW] || it == t[NI_MAXIT]) {
+                st = EFE_NEWTON_FAILED;
+                t[NI_REASON] = EFE_FAIL_ITERATIONS;
+            }
+        }
+        const int go = st == EFE_NEWTON_ITERATING;
+        t[NI_STATE] = st;
+        t[NI_GO] = go;
+        active[s] = go;
+        const double a = nopt[OP_ATOLF] * thr;
+        sc[s * NSC + TOL2] = a * a;
+        const double rt = (nopt[OP_MIXED_FIRST] != 0.0 && first) ? nopt[OP_RTOL_FIRST] : nopt[OP_RTOL];
+        sc[s * NSC + RTOL2] = rt * rt;
+        mine += go;
+    }
+    if (mine) atomicAdd(&count, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) cnt[0] = count;
+}
+
+int ensure_newton(efe_ctx* c) {
+    BatchWork* w = c->bw;
+    if (w->ni) return EFE_OK;
+    const size_t S = c->nspec, D = c->D;
+    EFE_HIP(hipMalloc(&w->ni, S * NI * sizeof(int)));
+    EFE_HIP(hipMalloc(&w->nd, S * ND * sizeof(double)));
+    EFE_HIP(hipMalloc(&w->nopt, NOPT * sizeof(double)));
+    EFE_HIP(hipMalloc(&w->nout2, S * 4 * sizeof(double)));
+    EFE_HIP(hipMalloc(&w->ncount, 2 * sizeof(int)));
+    double** vecs[] = {&w->du, &w->nP, &w->nF, &w->pext, &w->dtv, &w->dirv, &w->corr, &w->rhs, &w->ddu};
+    for (double** v : vecs) {
+        EFE_HIP(hipMalloc(v, D * sizeof(double)));
+        EFE_HIP(hipMemsetAsync(*v, 0, D * sizeof(double), c->stream));
+    }
+    EFE_HIP(hipStreamSynchronize(c->stream));
+    return EFE_OK;
+}
+
+// The launches of one Newton pass on `st`: variant 0 the round's first pass, 1 the first pass
+// with prescribed increments (right-hand-side correction), 2 a later pass (which first applies
+// the previous solve).  Every launch reads its data from device memory, so the captured graph
+// replays for any round.
+template <typename T>
+int newton_pass_launch(efe_ctx* c, hipStream_t st, int variant, int prec, bool hasPExt, int batch, const T* K,
+                       const T* Minv) {
+    BatchWork* w = c->bw;
+    const int S = c->nspec, N = c->N;
+    const int64_t D = c->D;
+    const unsigned gd = (unsigned)((D + BS - 1) / BS);
+    if (variant == 2) {
+        b_finish_kernel<T><<<gd, BS, 0, st>>>((int)D, (const T*)w->x, w->rhs, c->dirflag, c->ndir, w->ddu);
+        b_norms_kernel<<<S, BS, 0, st>>>(c->specNodeOff, w->ddu, nullptr, w->ddu, w->nout2);
+        nr_post_kernel<<<1, BS, 0, st>>>(S, w->ni, w->sc, w->iters, w->nout2, w->nopt);
+        nr_add_kernel<<<gd, BS, 0, st>>>(D, w->du, w->ddu);
+    }
+    nr_pass_kernel<<<1, BS, 0, st>>>(S, w->ni, c->specActive, c->status, c->specStatus);
+    if (variant == 1) nr_dirv_kernel<<<gd, BS, 0, st>>>(D, c->specOfNode, w->ni, w->dtv, w->dirv, w->corr);
+    EFE_HIP(hipGetLastError());
+    if (int rc = efe_launch_elements(c, w->du, prec)) return rc;
+    const double* dv = variant == 1 ? w->dirv : nullptr;
+    double* cr = variant == 1 ? w->corr : nullptr;
+    if (prec == EFE_FP32_MATFREE) {
+        if (int rc = efe_launch_mf_assembly(c, w->nP, w->nF, dv, cr)) return rc;
+    } else if (int rc = efe_launch_assembly(c, w->nP, w->nF, dv, cr, prec)) {
+        return rc;
+    }
+    nr_residual_kernel<<<gd, BS, 0, st>>>(D, w->nP, hasPExt ? w->pext : nullptr, c->dirflag, c->ndir, dv, w->rhs);
+    b_norms_kernel<<<S, BS, 0, st>>>(c->specNodeOff, w->rhs, variant == 2 ? w->ddu : nullptr, w->nF, w->nout);
+    nr_decide_kernel<<<1, BS, 0, st>>>(S, variant == 2, w->ni, w->nd, w->nout, c->specStatus, w->nopt, w->active,
+                                       w->sc, w->ncount);
+    if (variant == 1) nr_sub_kernel<<<gd, BS, 0, st>>>(D, w->rhs, w->corr);
+    b_init_kernel<T><<<gridn(N), BS, 0, st>>>(N, c->specOfNode, w->active, w->rhs, Minv, (T*)w->x, (T*)w->r,
+                                              (T*)w->z, (T*)w->p, w->nv0, w->nv1);
+    b_init_reduce_kernel<<<S, BS, 0, st>>>(c->specNodeOff, w->nv0, w->nv1, w->sc, w->active, w->iters);
+    launch_b_batch<T>(c, w, st, K, Minv, batch);
+    EFE_HIP(hipGetLastError());
+    return EFE_OK;
+}
+
+template <typename T>
+int capture(efe_ctx* c, hipGraphExec_t* out, hipStream_t st, int (*fn)(efe_ctx*, hipStream_t, int, int, bool, int,
+                                                                        const T*, const T*),
+            int variant, int prec, bool hasPExt, int batch, const T* K, const T* Minv) {
+    hipGraph_t g;
+    EFE_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    const int rc = fn(c, st, variant, prec, hasPExt, batch, K, Minv);
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc) {
+        if (e == hipSuccess) hipGraphDestroy(g);
+        return rc;
+    }
+    EFE_HIP(e);
+    EFE_HIP(hipGraphInstantiate(out, g, nullptr, nullptr, 0));
+    EFE_HIP(hipGraphDestroy(g));
+    return EFE_OK;
+}
+
+template <typename T>
+int newton_round(efe_ctx* c, const efe_newton_opts* o, bool hasPExt, bool anyFirst, int maxPasses, int* passes,
+                 const T* K, const T* Minv) {
+    BatchWork* w = c->bw;
+    hipStream_t st = c->stream;
+    const int batch = o->check_every > 0 ? o->check_every : 16;
+    const int maxit = o->krylov_max_iter > 0 ? o->krylov_max_iter : 10000;
+    // the PCG batch graph of later batches (also the one efe_linsolve_batched replays)
+    if (!w->graph || w->graph_batch != batch || w->graph_op != o->precision) {
+        if (w->graph) hipGraphExecDestroy(w->graph);
+        w->graph = nullptr;
+        hipGraph_t g;
+        EFE_HIP(hipStreamBeginCapture(w->ks, hipStreamCaptureModeRelaxed));
+        launch_b_batch<T>(c, w, w->ks, K, Minv, batch);
+        EFE_HIP(hipStreamEndCapture(w->ks, &g));
+        EFE_HIP(hipGraphInstantiate(&w->graph, g, nullptr, nullptr, 0));
+        EFE_HIP(hipGraphDestroy(g));
+        w->graph_batch = batch;
+        w->graph_op = o->precision;
+    }
+    for (int pass = 0;; ++pass) {
+        if (pass > maxPasses) {
+            efe_set_error("efe_newton_round_batched: more passes than any specimen's iteration limit allows");
+            return EFE_ERR_STATE;
+        }
+        const int variant = pass == 0 ? (anyFirst ? 1 : 0) : 2;
+        // everything a graph bakes in by value
+        const int64_t key[10] = {1,
+                                 o->precision,
+                                 hasPExt,
+                                 batch,
+                                 c->ndir,
+                                 c->epw,
+                                 (int64_t)(intptr_t)c->stateRef,
+                                 (int64_t)(intptr_t)c->stateTmp,
+                                 (int64_t)(intptr_t)K,
+                                 (int64_t)(intptr_t)Minv};
+        if (!w->ngraph[variant] || memcmp(key, w->nkey[variant], sizeof(key)) != 0) {
+            if (w->ngraph[variant]) hipGraphExecDestroy(w->ngraph[variant]);
+            w->ngraph[variant] = nullptr;
+            if (int rc = capture<T>(c, &w->ngraph[variant], st, newton_pass_launch<T>, variant, o->precision,
+                                    hasPExt, batch, K, Minv))
+                return rc;
+            memcpy(w->nkey[variant], key, sizeof(key));
+        }
+        EFE_HIP(hipGraphLaunch(w->ngraph[variant], st));
+        ++*passes;
+        EFE_HIP(hipMemcpyAsync(w->hbuf, w->ncount, sizeof(int), hipMemcpyDeviceToHost, st));
+        EFE_HIP(hipMemcpyAsync(w->hbuf + 1, w->nactive, sizeof(int), hipMemcpyDeviceToHost, st));
+        EFE_HIP(hipStreamSynchronize(st));
+        if (w->hbuf[0] == 0) break;  // no specimen iterates: the round is over
+        int it = batch;
+        while (w->hbuf[1] != 0 && it < maxit) {
+            EFE_HIP(hipGraphLaunch(w->graph, st));
+            EFE_HIP(hipMemcpyAsync(w->hbuf + 1, w->nactive, sizeof(int), hipMemcpyDeviceToHost, st));
+            EFE_HIP(hipStreamSynchronize(st));
+            it += batch;
+        }
+    }
+    return EFE_OK;
+}
+
+}  // namespace
+
+int efe_newton_round_batched_impl(efe_ctx* c, double* dU, double* P, double* F, const double* PExt,
+                                  const double* deltaT, const uint8_t* live, const uint8_t* extrap,
+                                  const uint8_t* hasDir, const int32_t* maxIter, const int32_t* maxGrow,
+                                  const double* nAcc, const efe_newton_opts* o, int32_t* state, int32_t* iterations,
+                                  int32_t* reason, int32_t* work) {
+    const int S = c->nspec;
+    const int vp = o->precision == EFE_FP64 ? EFE_FP64 : EFE_FP32;
+    if (int rc = efe_prepare_elements(c, o->precision)) return rc;
+    if (int rc = ensure_batch(c, vp)) return rc;
+    if (int rc = ensure_newton(c)) return rc;
+    BatchWork* w = c->bw;
+    // the round runs on the batch stream (a graph cannot be captured on the legacy default
+    // stream), ordered after the context stream's work and before its later work
+    hipStream_t st = w->ks, home = c->stream;
+    EFE_HIP(hipEventRecord(w->evA, home));
+    EFE_HIP(hipStreamWaitEvent(st, w->evA, 0));
+    std::vector<int> ni((size_t)S * NI, 0);
+    std::vector<double> nd((size_t)S * ND, 0.0);
+    bool anyFirst = false;
+    int maxPasses = 1;
+    for (int s = 0; s < S; ++s) {
+        int* t = ni.data() + (size_t)s * NI;
+        t[NI_LIVE] = live[s] ? 1 : 0;
+        t[NI_EXTRAP] = extrap[s] ? 1 : 0;
+        t[NI_HASDIR] = hasDir[s] ? 1 : 0;
+        t[NI_MAXIT] = maxIter[s];
+        t[NI_MAXGROW] = maxGrow[s];
+        t[NI_STATE] = live[s] ? EFE_NEWTON_ITERATING : EFE_NEWTON_IDLE;
+        nd[(size_t)s * ND + ND_NACC] = nAcc[s];
+        if (live[s] && !extrap[s] && hasDir[s]) anyFirst = true;
+        if (live[s] && maxIter[s] + 2 > maxPasses) maxPasses = maxIter[s] + 2;
+    }
+    double opt[NOPT] = {o->rtol,           o->rtol_first, o->atol_factor,  o->flux_tol,
+                        o->flux_tol_alt,   o->correction_tol, o->flux_floor, o->avg_floor,
+                        (double)(o->mixed_first != 0), (double)o->alt_after,
+                        (double)(o->krylov_max_iter > 0 ? o->krylov_max_iter : 10000)};
+    const size_t Db = (size_t)c->D * sizeof(double);
+    EFE_HIP(hipMemcpyAsync(w->ni, ni.data(), ni.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    EFE_HIP(hipMemcpyAsync(w->nd, nd.data(), nd.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    EFE_HIP(hipMemcpyAsync(w->nopt, opt, sizeof(opt), hipMemcpyHostToDevice, st));
+    EFE_HIP(hipMemcpyAsync(w->du, dU, Db, hipMemcpyDeviceToDevice, st));
+    EFE_HIP(hipMemcpyAsync(w->dtv, deltaT, Db, hipMemcpyDeviceToDevice, st));
+    if (PExt) EFE_HIP(hipMemcpyAsync(w->pext, PExt, Db, hipMemcpyDeviceToDevice, st));
+    const bool wasActive = c->useActive;
+    c->useActive = true;  // the passes cover the iterating specimens (specActive, set on the device)
+    c->stream = st;       // the element and assembly launches follow
+    int passes = 0, rc;
+    if (o->precision == EFE_FP32_MATFREE)
+        rc = newton_round<float>(c, o, PExt != nullptr, anyFirst, maxPasses, &passes, nullptr, c->Minv32);
+    else if (o->precision == EFE_FP32)
+        rc = newton_round<float>(c, o, PExt != nullptr, anyFirst, maxPasses, &passes, c->K32, c->Minv32);
+    else
+        rc = newton_round<double>(c, o, PExt != nullptr, anyFirst, maxPasses, &passes, c->K64, c->Minv64);
+    c->stream = home;
+    c->useActive = wasActive;
+    if (rc) {
+        hipStreamSynchronize(st);
+        return rc;
+    }
+    c->kscrValid = o->precision != EFE_FP32_MATFREE;
+    c->pscrValid = true;
+    c->Kprec = o->precision;
+    c->Kversion++;
+    c->tot[EFE_T_ELEMENT_PASSES] += passes;
+    EFE_HIP(hipMemcpyAsync(dU, w->du, Db, hipMemcpyDeviceToDevice, st));
+    if (P) EFE_HIP(hipMemcpyAsync(P, w->nP, Db, hipMemcpyDeviceToDevice, st));
+    if (F) EFE_HIP(hipMemcpyAsync(F, w->nF, Db, hipMemcpyDeviceToDevice, st));
+    EFE_HIP(hipMemcpyAsync(ni.data(), w->ni, ni.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+    EFE_HIP(hipEventRecord(w->evB, st));
+    EFE_HIP(hipStreamWaitEvent(home, w->evB, 0));
+    EFE_HIP(hipStreamSynchronize(st));
+    for (int s = 0; s < S; ++s) {
+        const int* t = ni.data() + (size_t)s * NI;
+        state[s] = t[NI_STATE];
+        iterations[s] = t[NI_IT];
+        reason[s] = t[NI_REASON];
+        work[3 * s + 0] = t[NI_PASSES];
+        work[3 * s + 1] = t[NI_SOLVES];
+        work[3 * s + 2] = t[NI_KRYLOV];
+    }
+    return EFE_OK;
+}
+
 void efe_batched_free(efe_ctx* c) {
     BatchWork* w = c->bw;
     if (!w) return;
-    void* bufs[] = {w->x,      w->r,      w->z,      w->p,          w->q,         w->nv0,
-                    w->nv1,    w->sc,     w->active, w->iters,      w->nactive,   w->acc,
+    void* bufs[] = {w->x,      w->r,      w->z,      w->p,          w->q,          w->nv0,
+                    w->nv1,    w->sc,     w->active, w->iters,      w->nactive,    w->acc,
                     w->nout,   w->pqp,    w->rzp,    w->rrp,        w->chunkNode0, w->chunkSpec,
-                    w->specChunk, w->nst, w->nit, w->ngrow, w->nreason, w->nfirst, w->nextrap, w->nhasdir,
-                    w->nmaxit, w->nmaxgrow, w->nsolve, w->npass, w->nsolves, w->nkry, w->ncount, w->ngprev,
-                    w->nacc, w->dirv, w->corr, w->rhs, w->ddu};
+                    w->specChunk, w->ni,  w->nd,     w->nopt,       w->nout2,      w->ncount,
+                    w->du,     w->nP,     w->nF,     w->pext,       w->dtv,        w->dirv,
+                    w->corr,   w->rhs,    w->ddu};
     for (void* p : bufs)
         if (p) hipFree(p);
-    if (w->ngraph) hipGraphExecDestroy(w->ngraph);
+    for (hipGraphExec_t g : w->ngraph)
+        if (g) hipGraphExecDestroy(g);
     if (w->hbuf) hipHostFree(w->hbuf);
     if (w->graph) hipGraphExecDestroy(w->graph);
     if (w->ks) hipStreamDestroy(w->ks);

@@ -689,6 +689,31 @@ int efe_accept_state_batched(efe_ctx* c, const uint8_t* accept) {
     return efe_launch_accept_batched(c, accept);
 }
 
+int efe_newton_round_batched(efe_ctx* c, double* dU, double* P, double* F, const double* PExt, const double* deltaT,
+                             const uint8_t* live, const uint8_t* extrapolated, const uint8_t* has_dirichlet,
+                             const int32_t* max_iter, const int32_t* max_grow, const double* n_acc,
+                             const efe_newton_opts* o, int32_t* state, int32_t* iterations, int32_t* reason,
+                             int32_t* work) {
+    if (int rc = check(c)) return rc;
+    if (!c->specOfNode || !c->specActive) {
+        efe_set_error("efe_newton_round_batched needs a context created with specimens");
+        return EFE_ERR_STATE;
+    }
+    if (!dU || !deltaT || !live || !extrapolated || !has_dirichlet || !max_iter || !max_grow || !n_acc || !o ||
+        !state || !iterations || !reason || !work ||
+        (o->precision != EFE_FP64 && o->precision != EFE_FP32 && o->precision != EFE_FP32_MATFREE)) {
+        efe_set_error("efe_newton_round_batched: invalid arguments");
+        return EFE_ERR_ARG;
+    }
+    for (int s = 0; s < c->nspec; ++s)
+        if (max_iter[s] < 0 || max_grow[s] < 0 || !(n_acc[s] > 0.0)) {
+            efe_set_error("efe_newton_round_batched: max_iter and max_grow must be >= 0, n_acc > 0");
+            return EFE_ERR_ARG;
+        }
+    return efe_newton_round_batched_impl(c, dU, P, F, PExt, deltaT, live, extrapolated, has_dirichlet, max_iter,
+                                         max_grow, n_acc, o, state, iterations, reason, work);
+}
+
 int efe_get_specimen_status(efe_ctx* c, uint8_t* out) {
     if (int rc = check(c)) return rc;
     if (!out) {

@@ -201,6 +201,8 @@ struct efe_ctx {
     double tot[EFE_T_COUNT] = {};
 };
 
+// buffers of element passes in a precision (capi.hip)
+int efe_prepare_elements(efe_ctx* c, int precision);
 // element kernels (elements.hip)
 int efe_launch_elements(efe_ctx* c, const double* dU, int precision);
 int efe_launch_residual(efe_ctx* c, const double* dU, double* P, double* F, int precision);
@@ -234,6 +236,11 @@ int efe_batched_linsolve(efe_ctx* c, const double* b, double* x, const efe_solve
                          const double* spec_rtol, const uint8_t* spec_active, int32_t* spec_iters,
                          efe_solve_info* info);
 int efe_launch_accept_batched(efe_ctx* c, const uint8_t* accept);
+int efe_newton_round_batched_impl(efe_ctx* c, double* dU, double* P, double* F, const double* PExt,
+                                  const double* deltaT, const uint8_t* live, const uint8_t* extrap,
+                                  const uint8_t* hasDir, const int32_t* maxIter, const int32_t* maxGrow,
+                                  const double* nAcc, const efe_newton_opts* o, int32_t* state, int32_t* iterations,
+                                  int32_t* reason, int32_t* work);
 void efe_batched_free(efe_ctx* c);
 
 // scalar CSR Krylov solves (csrsolve.hip)

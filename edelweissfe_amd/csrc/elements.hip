@@ -23,6 +23,8 @@
 //   diagonal block for block Jacobi, and gathers P = sum Pe and F = sum |Pe|
 //   (nonlinearimplicitstatic.py:843-844) in element order.  No atomics: the
 //   result is bitwise reproducible.
+#include <cstdlib>
+
 #include "efe_internal.h"
 #include "materials.h"
 
@@ -65,8 +67,10 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
         return (specActive && !specActive[specOfElem[e]]) ? -1 : e;
     };
     constexpr int NP = (MAT == EFE_LINEAR_ELASTIC) ? 2 : 6;
-    __shared__ double g_s[EPW][Q][NN][3];
-    __shared__ KT fC_s[EPW][Q][36];
+    // padded by one value per element, so that lanes of the same (q, a) in EPW elements (the
+    // assembled passes' phase 2) read different banks
+    __shared__ double g_s[EPW][Q * NN * 3 + 1];
+    __shared__ KT fC_s[EPW][Q * 36 + 1];
     __shared__ double fs_s[EPW][Q][6];
     const int lane = threadIdx.x;
     const int64_t EQ = (int64_t)E * Q;
@@ -110,9 +114,9 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
             double g[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) g[c] = iJ[c * 3 + 0] * d0 + iJ[c * 3 + 1] * d1 + iJ[c * 3 + 2] * d2;
-            g_s[el][q][a][0] = g[0];
-            g_s[el][q][a][1] = g[1];
-            g_s[el][q][a][2] = g[2];
+            g_s[el][(q * NN + a) * 3 + 0] = g[0];
+            g_s[el][(q * NN + a) * 3 + 1] = g[1];
+            g_s[el][(q * NN + a) * 3 + 2] = g[2];
             const int nd = conn[(int64_t)a * E + e];
             const double u0 = dU[3 * nd], u1 = dU[3 * nd + 1], u2 = dU[3 * nd + 2];
 #pragma unroll
@@ -157,7 +161,7 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
         const double f = det * qpw[q];
         if (formK)
 #pragma unroll
-            for (int k = 0; k < 36; ++k) fC_s[el][q][k] = (KT)(f * C[k]);
+            for (int k = 0; k < 36; ++k) fC_s[el][q * 36 + k] = (KT)(f * C[k]);
 #pragma unroll
         for (int k = 0; k < 6; ++k) fs_s[el][q][k] = f * s[k];
     }
@@ -168,8 +172,12 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
     constexpr bool SYM = !MF && MAT != EFE_DRUCKER_PRAGER;
     constexpr int NB = MF ? NN : (SYM ? NN * (NN + 1) / 2 : NN * NN);  // blocks per element formed here
     // a residual-only pass (formK = 0) leaves the tangent blocks alone
+    // Assembled passes: lanes take the EPW elements of one block position (a, b) side by side.
+    // Consecutive elements put the block at consecutive rows of the same contribution run (on
+    // a structured mesh), so a wavefront's stores form runs of EPW adjacent blocks.  The
+    // matrix-free pass's K_aa are element-major: element by element there.
     for (int t = formK ? lane : EPW * NB; t < EPW * NB; t += 64) {
-        const int el = t / NB, r0 = t - (t / NB) * NB;
+        const int el = MF ? t / NB : t % EPW, r0 = MF ? t - (t / NB) * NB : t / EPW;
         int a, b;
         if (MF) {
             a = b = r0;
@@ -191,9 +199,9 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
 #pragma unroll
         for (int k = 0; k < 9; ++k) acc[k] = (KT)0;
         for (int q = 0; q < Q; ++q) {
-            const KT ax = (KT)g_s[el][q][a][0], ay = (KT)g_s[el][q][a][1], az = (KT)g_s[el][q][a][2];
-            const KT bx = (KT)g_s[el][q][b][0], by = (KT)g_s[el][q][b][1], bz = (KT)g_s[el][q][b][2];
-            const KT* C = fC_s[el][q];
+            const KT ax = (KT)g_s[el][(q * NN + a) * 3 + 0], ay = (KT)g_s[el][(q * NN + a) * 3 + 1], az = (KT)g_s[el][(q * NN + a) * 3 + 2];
+            const KT bx = (KT)g_s[el][(q * NN + b) * 3 + 0], by = (KT)g_s[el][(q * NN + b) * 3 + 1], bz = (KT)g_s[el][(q * NN + b) * 3 + 2];
+            const KT* C = fC_s[el] + q * 36;
             KT CB[6][3];
 #pragma unroll
             for (int r = 0; r < 6; ++r) {
@@ -226,7 +234,7 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
         if (e < 0) continue;
         double p0 = 0, p1 = 0, p2 = 0;
         for (int q = 0; q < Q; ++q) {
-            const double ax = g_s[el][q][a][0], ay = g_s[el][q][a][1], az = g_s[el][q][a][2];
+            const double ax = g_s[el][(q * NN + a) * 3 + 0], ay = g_s[el][(q * NN + a) * 3 + 1], az = g_s[el][(q * NN + a) * 3 + 2];
             const double* s = fs_s[el][q];
             p0 += ax * s[0] + ay * s[3] + az * s[4];
             p1 += ay * s[1] + ax * s[3] + az * s[5];
@@ -321,6 +329,139 @@ __global__ __launch_bounds__(256) void assemble_kernel(
             for (int v = 0; v < 9; ++v) dg[v] = (double)acc[v];
         }
     }
+    double iv[9];
+    inv3(dg, iv);
+#pragma unroll
+    for (int v = 0; v < 9; ++v) Minv[(int64_t)n * 9 + v] = (KT)iv[v];
+    if (P) {
+        double p0 = 0, p1 = 0, p2 = 0, f0 = 0, f1 = 0, f2 = 0;
+        for (int c = nptr[n]; c < nptr[n + 1]; ++c) {
+            const double* src = Pscr + (int64_t)nlist[c] * 3;
+            p0 += src[0];
+            p1 += src[1];
+            p2 += src[2];
+            f0 += fabs(src[0]);
+            f1 += fabs(src[1]);
+            f2 += fabs(src[2]);
+        }
+        P[3 * n] = p0;
+        P[3 * n + 1] = p1;
+        P[3 * n + 2] = p2;
+        F[3 * n] = f0;
+        F[3 * n + 1] = f1;
+        F[3 * n + 2] = f2;
+    }
+    if (dirv) {
+        dircorr[3 * n] = fr[0] ? 0.0 : t0;
+        dircorr[3 * n + 1] = fr[1] ? 0.0 : t1;
+        dircorr[3 * n + 2] = fr[2] ? 0.0 : t2;
+    }
+}
+
+// assemble_kernel with the element scratch read run by run: the wavefront of a slice loads
+// contribution run j (the j-th contribution of its 64 rows, 64 contiguous blocks) with nine
+// coalesced 512-byte loads (FP64), stages it in LDS and each lane takes its own block from
+// there, loading run j + 1 meanwhile.  Every line of the scratch is fetched once (the per-lane
+// nine-load form re-fetched lines its wavefronts had in flight when they were evicted).  The
+// lanes walk their rows' contributions in the same order, so K, Minv, P, F and the correction are
+// bitwise those of assemble_kernel.
+template <typename KT>
+__global__ __launch_bounds__(256) void assemble_runs_kernel(
+    int N, const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol, const int32_t* __restrict__ coff,
+    const int32_t* __restrict__ nptr, const int32_t* __restrict__ nlist,
+    const int32_t* __restrict__ slice_off, const KT* __restrict__ Kscr, const double* __restrict__ Pscr,
+    const uint8_t* __restrict__ dirflag, int ndir, const double* __restrict__ dirv, double* __restrict__ dircorr,
+    KT* __restrict__ Kval, KT* __restrict__ Minv, double* __restrict__ P, double* __restrict__ F,
+    const int32_t* __restrict__ slotNode, const int32_t* __restrict__ cslice, const int* __restrict__ specActive,
+    const int32_t* __restrict__ specOfNode) {
+    constexpr int RUN = 9 * kSlice;  // values of one contribution run
+    __shared__ KT runbuf[256 / kSlice][RUN];
+    const int per = gridDim.x >> 3;
+    const int blk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int ps = blk * blockDim.x + threadIdx.x;
+    const int s = ps / kSlice, l = ps - (ps / kSlice) * kSlice;  // one slice per wavefront: l is the lane
+    if (s * kSlice >= N) return;                                  // the whole wavefront
+    KT* buf = runbuf[threadIdx.x / kSlice];
+    const bool valid = ps < N;
+    const int n = valid ? slotNode[ps] : 0;
+    const bool work = valid && !(specActive && !specActive[specOfNode[n]]);
+    if (valid && !work && dirv) dircorr[3 * n] = dircorr[3 * n + 1] = dircorr[3 * n + 2] = 0.0;
+    bool fr[3] = {false, false, false};
+    if (work && ndir) {
+        fr[0] = dirflag[3 * n];
+        fr[1] = dirflag[3 * n + 1];
+        fr[2] = dirflag[3 * n + 2];
+    }
+    double t0 = 0, t1 = 0, t2 = 0;
+    double dg[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    const int64_t vbase = (int64_t)slice_off[s] * 9 + l;
+    int b0 = 0, b = 0, b1 = 0, c0 = 0, cnt = 0, cend = 0;
+    if (work) {
+        b0 = b = bptr[n];
+        b1 = bptr[n + 1];
+        c0 = coff[b0];
+        cnt = coff[b1] - c0;
+        cend = coff[b + 1] - c0;
+    }
+    KT acc[9];
+#pragma unroll
+    for (int v = 0; v < 9; ++v) acc[v] = (KT)0;
+    const int J = cslice[s + 1] - cslice[s];
+    if (__any(work)) {
+        const KT* runs = Kscr + (int64_t)cslice[s] * RUN;
+        KT nxt[9];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) nxt[t] = J > 0 ? runs[t * kSlice + l] : (KT)0;
+        for (int j = 0; j < J; ++j) {
+#pragma unroll
+            for (int t = 0; t < 9; ++t) buf[t * kSlice + l] = nxt[t];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (j + 1 < J) {
+                const KT* nr = runs + (int64_t)(j + 1) * RUN;
+#pragma unroll
+                for (int t = 0; t < 9; ++t) nxt[t] = __builtin_nontemporal_load(nr + t * kSlice + l);
+            }
+            if (work && j < cnt) {
+#pragma unroll
+                for (int v = 0; v < 9; ++v) acc[v] += buf[l * 9 + v];
+                if (j + 1 == cend) {  // the last contribution of block b: finish it
+                    const int m = bcol[b];
+                    if (dirv) {
+                        const double d0 = dirv[3 * m], d1 = dirv[3 * m + 1], d2 = dirv[3 * m + 2];
+                        t0 += (double)acc[0] * d0 + (double)acc[1] * d1 + (double)acc[2] * d2;
+                        t1 += (double)acc[3] * d0 + (double)acc[4] * d1 + (double)acc[5] * d2;
+                        t2 += (double)acc[6] * d0 + (double)acc[7] * d1 + (double)acc[8] * d2;
+                    }
+                    if (ndir) {
+                        const bool fc[3] = {(bool)dirflag[3 * m], (bool)dirflag[3 * m + 1], (bool)dirflag[3 * m + 2]};
+#pragma unroll
+                        for (int i = 0; i < 3; ++i)
+#pragma unroll
+                            for (int jj = 0; jj < 3; ++jj)
+                                if (fr[i] || fc[jj]) acc[i * 3 + jj] = (m == n && i == jj) ? (KT)1 : (KT)0;
+                    }
+                    const int k = b - b0;
+#pragma unroll
+                    for (int v = 0; v < 9; ++v) Kval[vbase + (int64_t)(k * 9 + v) * kSlice] = acc[v];
+                    if (m == n) {
+#pragma unroll
+                        for (int v = 0; v < 9; ++v) dg[v] = (double)acc[v];
+                    }
+#pragma unroll
+                    for (int v = 0; v < 9; ++v) acc[v] = (KT)0;
+                    ++b;
+                    if (b < b1) cend = coff[b + 1] - c0;
+                }
+            }
+            // every lane has read its block before the next run overwrites the buffer
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+    if (!work) return;
     double iv[9];
     inv3(dg, iv);
 #pragma unroll
@@ -653,6 +794,22 @@ int efe_launch_residual(efe_ctx* c, const double* dU, double* P, double* F, int 
 int efe_launch_assembly(efe_ctx* c, double* P, double* F, const double* dir_values, double* dir_correction,
                         int precision) {
     const int bs = 256, grid = (((c->N + bs - 1) / bs) + 7) & ~7;  // a multiple of 8 (XCD mapping)
+    static const bool perLane = std::getenv("EFE_ASSEMBLY_PER_LANE") != nullptr;  // the per-lane form, for comparison
+    if (!perLane) {
+#define ASM_ARGS(KT, K, M)                                                                                           \
+    c->N, c->bptr, c->bcol, c->coff, c->nptr, c->nlist, c->slice_off, (const KT*)c->Kscr, c->Pscr, c->dirflag,       \
+        c->ndir, dir_values, dir_correction, K, M, P, F, c->slotNode, c->cslice, c->useActive ? c->specActive : nullptr, \
+        c->specOfNode
+        if (precision == EFE_FP32)
+            assemble_runs_kernel<float><<<grid, bs, 0, c->stream>>>(ASM_ARGS(float, c->K32, c->Minv32));
+        else
+            assemble_runs_kernel<double><<<grid, bs, 0, c->stream>>>(ASM_ARGS(double, c->K64, c->Minv64));
+#undef ASM_ARGS
+        EFE_HIP(hipGetLastError());
+        c->Kprec = precision;
+        c->Kversion++;
+        return EFE_OK;
+    }
     if (precision == EFE_FP32) {
         assemble_kernel<float><<<grid, bs, 0, c->stream>>>(
             c->N, c->bptr, c->bcol, c->coff, c->nptr, c->nlist, c->slice_off, (const float*)c->Kscr,

@@ -11,6 +11,9 @@
 // GMRES: right preconditioning (the Arnoldi residual is the true residual of K x = b),
 // classical Gram-Schmidt with one re-orthogonalisation (all projections of a step in one
 // pass over the basis), Givens rotations on the host with one readback per step.
+#include <algorithm>
+#include <array>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -25,6 +28,7 @@ struct GmresWork {
     std::vector<int> cstart;
     int32_t *fptr = nullptr, *fcol = nullptr, *fidx = nullptr, *dslot = nullptr, *perm = nullptr;
     int64_t nbF = 0;
+    int maxUpper = 0;  // most blocks in a row's upper part
     double *F64 = nullptr, *Dinv64 = nullptr;
     float *F32 = nullptr, *Dinv32 = nullptr;
     int factored_version[2] = {-1, -1};
@@ -41,11 +45,25 @@ struct GmresWork {
     hipStream_t ks = nullptr;
     hipEvent_t evA = nullptr, evB = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr, t2 = nullptr;  // timing: solve start, factor done, end
+    // one captured hipGraph per Arnoldi step j (the launches between two readbacks) and the
+    // pointers and sizes it was captured with
+    std::vector<hipGraphExec_t> stepGraph;
+    std::vector<std::array<int64_t, 8>> stepKey;
 };
+
+namespace {
+void drop_step_graphs(GmresWork* g) {
+    for (hipGraphExec_t& e : g->stepGraph)
+        if (e) hipGraphExecDestroy(e);
+    g->stepGraph.clear();
+    g->stepKey.clear();
+}
+}  // namespace
 
 namespace {
 
 constexpr int BS = 256;
+constexpr int kFactorCap = 128;  // upper-part blocks of a row the LDS factor kernel stages
 
 __device__ __forceinline__ int xcd_block_g() {
     const int per = gridDim.x >> 3;
@@ -167,6 +185,74 @@ __global__ __launch_bounds__(BS) void ilu_factor_kernel(int r0, int r1, const in
     }
 }
 
+// ilu_factor_kernel with row k's upper column list staged in LDS: the G lanes of a row load it
+// together (coalesced), then each searches its columns there instead of through dependent
+// global loads.  The same updates in the same order: the factor is bitwise identical.
+// Needs every row's upper part to hold at most CAP blocks.
+template <typename T, int G, int CAP>
+__global__ __launch_bounds__(BS) void ilu_factor_lds_kernel(int r0, int r1, const int32_t* __restrict__ fptr,
+                                                            const int32_t* __restrict__ fcol,
+                                                            const int32_t* __restrict__ dslot, T* __restrict__ F,
+                                                            T* __restrict__ Dinv) {
+    __shared__ int32_t cols[(BS / G) * CAP];
+    const int tid = blockIdx.x * BS + threadIdx.x;
+    const int i = r0 + tid / G, lane = tid % G;
+    int32_t* kc = cols + (threadIdx.x / G) * CAP;  // this row group's list (groups lie inside one wave)
+    if (i >= r1) return;  // a whole group leaves together
+    const int b0 = fptr[i], b1 = fptr[i + 1], d = b0 + dslot[i];
+    for (int kb = b0; kb < d; ++kb) {
+        const int k = fcol[kb];
+        const int owner = (kb - b0) % G;
+        const int u0 = fptr[k] + dslot[k] + 1, nu = fptr[k + 1] - u0;
+        for (int t = lane; t < nu; t += G) kc[t] = fcol[u0 + t];
+        T lik[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (lane == owner) {
+            T a[9];
+#pragma unroll
+            for (int v = 0; v < 9; ++v) a[v] = F[(int64_t)kb * 9 + v];
+            mat3mul(a, Dinv + (int64_t)k * 9, lik);
+#pragma unroll
+            for (int v = 0; v < 9; ++v) F[(int64_t)kb * 9 + v] = lik[v];
+        }
+#pragma unroll
+        for (int v = 0; v < 9; ++v) lik[v] = __shfl(lik[v], owner, G);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int first = kb + 1 + ((lane - (kb + 1 - b0) % G) % G + G) % G;  // first owned position > kb
+        for (int jb = first; jb < b1; jb += G) {
+            const int cj = fcol[jb];
+            int lo = 0, hi = nu;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (kc[mid] < cj) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo < nu && kc[lo] == cj) {
+                T u[9], p[9];
+                const int64_t ub = (int64_t)(u0 + lo) * 9;
+#pragma unroll
+                for (int v = 0; v < 9; ++v) u[v] = F[ub + v];
+                mat3mul(lik, u, p);
+#pragma unroll
+                for (int v = 0; v < 9; ++v) F[(int64_t)jb * 9 + v] -= p[v];
+            }
+        }
+        // every lane's searches are done before the list is overwritten
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if (lane == (d - b0) % G) {
+        double dd[9], di[9];
+#pragma unroll
+        for (int v = 0; v < 9; ++v) dd[v] = (double)F[(int64_t)d * 9 + v];
+        inv3d(dd, di);
+#pragma unroll
+        for (int v = 0; v < 9; ++v) Dinv[(int64_t)i * 9 + v] = (T)di[v];
+    }
+}
+
 // t = perm(v): t[factor row i] = v[node perm[i]]
 template <typename T>
 __global__ void permute_in_kernel(int N, const int32_t* __restrict__ perm, const T* __restrict__ v,
@@ -202,8 +288,11 @@ __device__ __forceinline__ void group_sum3(T& a, T& b, T& c) {
 }
 
 // forward sweep of one color: t_i -= sum_{k before i} L_ik t_k (unit block-lower);
-// G lanes per row, each over every G-th block of the row's lower part
-template <typename T, int G>
+// G lanes per row, each over every G-th block of the row's lower part.  A lane takes U of its
+// blocks at a time: their column indices, then the t entries they name, then the blocks are
+// loaded together, so one pass costs two dependent memory round trips, not two per block.
+// Each lane adds its blocks in the same order as one at a time.
+template <typename T, int G, int U>
 __global__ __launch_bounds__(BS) void ilu_lower_kernel(int r0, int r1, const int32_t* __restrict__ fptr,
                                                        const int32_t* __restrict__ fcol,
                                                        const int32_t* __restrict__ dslot, const T* __restrict__ F,
@@ -213,13 +302,23 @@ __global__ __launch_bounds__(BS) void ilu_lower_kernel(int r0, int r1, const int
     if (i >= r1) return;
     const int b0 = fptr[i], d = b0 + dslot[i];
     T y0 = 0, y1 = 0, y2 = 0;
-    for (int kb = b0 + lane; kb < d; kb += G) {
-        const int k = fcol[kb];
-        const T* a = F + (int64_t)kb * 9;
-        const T x0 = t[3 * k], x1 = t[3 * k + 1], x2 = t[3 * k + 2];
-        y0 += a[0] * x0 + a[1] * x1 + a[2] * x2;
-        y1 += a[3] * x0 + a[4] * x1 + a[5] * x2;
-        y2 += a[6] * x0 + a[7] * x1 + a[8] * x2;
+    for (int kb0 = b0 + lane; kb0 < d; kb0 += G * U) {
+        int kk[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) kk[u] = kb0 + u * G < d ? fcol[kb0 + u * G] : -1;
+        T x[U][3];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) x[u][c] = kk[u] >= 0 ? t[3 * kk[u] + c] : (T)0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (kk[u] < 0) break;
+            const T* a = F + (int64_t)(kb0 + u * G) * 9;
+            y0 += a[0] * x[u][0] + a[1] * x[u][1] + a[2] * x[u][2];
+            y1 += a[3] * x[u][0] + a[4] * x[u][1] + a[5] * x[u][2];
+            y2 += a[6] * x[u][0] + a[7] * x[u][1] + a[8] * x[u][2];
+        }
     }
     group_sum3<T, G>(y0, y1, y2);
     if (lane == 0) {
@@ -229,8 +328,9 @@ __global__ __launch_bounds__(BS) void ilu_lower_kernel(int r0, int r1, const int
     }
 }
 
-// backward sweep of one color: t_i = Dinv_i (t_i - sum_{j after i} U_ij t_j)
-template <typename T, int G>
+// backward sweep of one color: t_i = Dinv_i (t_i - sum_{j after i} U_ij t_j); blocks taken U
+// at a time per lane as in the forward sweep
+template <typename T, int G, int U>
 __global__ __launch_bounds__(BS) void ilu_upper_kernel(int r0, int r1, const int32_t* __restrict__ fptr,
                                                        const int32_t* __restrict__ fcol,
                                                        const int32_t* __restrict__ dslot, const T* __restrict__ F,
@@ -240,13 +340,23 @@ __global__ __launch_bounds__(BS) void ilu_upper_kernel(int r0, int r1, const int
     if (i >= r1) return;
     const int e = fptr[i + 1];
     T y0 = 0, y1 = 0, y2 = 0;
-    for (int jb = fptr[i] + dslot[i] + 1 + lane; jb < e; jb += G) {
-        const int j = fcol[jb];
-        const T* a = F + (int64_t)jb * 9;
-        const T x0 = t[3 * j], x1 = t[3 * j + 1], x2 = t[3 * j + 2];
-        y0 += a[0] * x0 + a[1] * x1 + a[2] * x2;
-        y1 += a[3] * x0 + a[4] * x1 + a[5] * x2;
-        y2 += a[6] * x0 + a[7] * x1 + a[8] * x2;
+    for (int jb0 = fptr[i] + dslot[i] + 1 + lane; jb0 < e; jb0 += G * U) {
+        int jj[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) jj[u] = jb0 + u * G < e ? fcol[jb0 + u * G] : -1;
+        T x[U][3];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) x[u][c] = jj[u] >= 0 ? t[3 * jj[u] + c] : (T)0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (jj[u] < 0) break;
+            const T* a = F + (int64_t)(jb0 + u * G) * 9;
+            y0 += a[0] * x[u][0] + a[1] * x[u][1] + a[2] * x[u][2];
+            y1 += a[3] * x[u][0] + a[4] * x[u][1] + a[5] * x[u][2];
+            y2 += a[6] * x[u][0] + a[7] * x[u][1] + a[8] * x[u][2];
+        }
     }
     group_sum3<T, G>(y0, y1, y2);
     if (lane == 0) {
@@ -260,7 +370,6 @@ __global__ __launch_bounds__(BS) void ilu_upper_kernel(int r0, int r1, const int
     }
 }
 
-// z = Minv v (block Jacobi)
 template <typename T>
 __global__ void bjacobi_kernel(int N, const T* __restrict__ Minv, const T* __restrict__ v, T* __restrict__ z) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -397,6 +506,8 @@ int build_ilu(efe_ctx* c, GmresWork* g) {
         (void)s;
         (void)l;
     }
+    g->maxUpper = 0;
+    for (int i = 0; i < N; ++i) g->maxUpper = std::max(g->maxUpper, fptr[i + 1] - fptr[i] - dslot[i] - 1);
     g->ncolors = ip.ncolors;
     g->cstart.assign(ip.color_start.begin(), ip.color_start.end());
     g->nbF = fptr[N];
@@ -425,11 +536,19 @@ int ilu_factor(efe_ctx* c, GmresWork* g, const T* K, T*& F, T*& Dinv, int pi, hi
     for (int col = 0; col < g->ncolors; ++col) {
         const int r0 = g->cstart[col], r1 = g->cstart[col + 1];
         if (r1 <= r0) continue;
-        if (c->NN == 8)
+        if (g->maxUpper <= kFactorCap) {  // row k's upper columns staged in LDS
+            if (c->NN == 8)
+                ilu_factor_lds_kernel<T, 8, kFactorCap><<<grid_of((r1 - r0) * 8), BS, 0, st>>>(
+                    r0, r1, g->fptr, g->fcol, g->dslot, F, Dinv);
+            else
+                ilu_factor_lds_kernel<T, 16, kFactorCap><<<grid_of((r1 - r0) * 16), BS, 0, st>>>(
+                    r0, r1, g->fptr, g->fcol, g->dslot, F, Dinv);
+        } else if (c->NN == 8) {
             ilu_factor_kernel<T, 8><<<grid_of((r1 - r0) * 8), BS, 0, st>>>(r0, r1, g->fptr, g->fcol, g->dslot, F, Dinv);
-        else
+        } else {
             ilu_factor_kernel<T, 16><<<grid_of((r1 - r0) * 16), BS, 0, st>>>(r0, r1, g->fptr, g->fcol, g->dslot, F,
                                                                              Dinv);
+        }
     }
     EFE_HIP(hipGetLastError());
     g->factored_version[pi] = c->Kversion;
@@ -449,25 +568,85 @@ void apply_precond(efe_ctx* c, GmresWork* g, int precond, const T* F, const T* D
             const int r0 = g->cstart[col], r1 = g->cstart[col + 1];
             if (r1 <= r0) continue;
             if (h8)
-                ilu_lower_kernel<T, 8><<<grid_of((r1 - r0) * 8), BS, 0, st>>>(r0, r1, g->fptr, g->fcol, g->dslot, F, t);
+                ilu_lower_kernel<T, 8, 4><<<grid_of((r1 - r0) * 8), BS, 0, st>>>(r0, r1, g->fptr, g->fcol, g->dslot, F, t);
             else
-                ilu_lower_kernel<T, 16><<<grid_of((r1 - r0) * 16), BS, 0, st>>>(r0, r1, g->fptr, g->fcol, g->dslot, F,
+                ilu_lower_kernel<T, 16, 4><<<grid_of((r1 - r0) * 16), BS, 0, st>>>(r0, r1, g->fptr, g->fcol, g->dslot, F,
                                                                                 t);
         }
         for (int col = g->ncolors - 1; col >= 0; --col) {
             const int r0 = g->cstart[col], r1 = g->cstart[col + 1];
             if (r1 <= r0) continue;
             if (h8)
-                ilu_upper_kernel<T, 8><<<grid_of((r1 - r0) * 8), BS, 0, st>>>(r0, r1, g->fptr, g->fcol, g->dslot, F,
+                ilu_upper_kernel<T, 8, 4><<<grid_of((r1 - r0) * 8), BS, 0, st>>>(r0, r1, g->fptr, g->fcol, g->dslot, F,
                                                                               Dinv, t);
             else
-                ilu_upper_kernel<T, 16><<<grid_of((r1 - r0) * 16), BS, 0, st>>>(r0, r1, g->fptr, g->fcol, g->dslot,
+                ilu_upper_kernel<T, 16, 4><<<grid_of((r1 - r0) * 16), BS, 0, st>>>(r0, r1, g->fptr, g->fcol, g->dslot,
                                                                                 F, Dinv, t);
         }
         permute_out_kernel<T><<<grid_of(N), BS, 0, st>>>(N, g->perm, t, z);
     } else {
         bjacobi_kernel<T><<<grid_of(N), BS, 0, st>>>(N, Minv, v, z);
     }
+}
+
+// Arnoldi step j: w = K M^{-1} V_j, two classical Gram-Schmidt passes against V_0..V_j
+// (projections into hdev[0..], hdev[m+2..]), ||w||^2 into hdev[2(m+2)], V_{j+1} = w / ||w||.
+template <typename T>
+void gmres_step_launch(efe_ctx* c, GmresWork* g, int precond, const T* F, const T* Dinv, const T* Minv, const T* K,
+                       int j, hipStream_t st) {
+    const int D = c->D, N = c->N, m = g->m;
+    const int gd = grid_of(D), gs = round8g(grid_of(N));
+    T* V = (T*)g->V;
+    T* w = (T*)g->w;
+    T* z = (T*)g->z;
+    apply_precond<T>(c, g, precond, F, Dinv, Minv, V + (int64_t)j * D, z, st);
+    gm_spmv_kernel<T><<<gs, BS, 0, st>>>(N, c->slice_off, c->scol, c->slotNode, K, z, w, nullptr);
+    for (int pass = 0; pass < 2; ++pass) {
+        mdot_kernel<T><<<dim3(g->nwg, j + 1), BS, 0, st>>>(D, V, w, g->part, g->nwg);
+        mdot_reduce_kernel<<<j + 1, BS, 0, st>>>(g->part, g->nwg, g->hdev + pass * (m + 2), 0);
+        combine_kernel<T><<<gd, BS, 0, st>>>(D, j + 1, V, g->hdev + pass * (m + 2), w, 0);
+    }
+    mdot_kernel<T><<<dim3(g->nwg, 1), BS, 0, st>>>(D, w, w, g->part, g->nwg);
+    mdot_reduce_kernel<<<1, BS, 0, st>>>(g->part, g->nwg, g->hdev + 2 * (m + 2), 0);
+    scale_kernel<T><<<gd, BS, 0, st>>>(D, w, g->hdev + 2 * (m + 2), V + (int64_t)(j + 1) * D);
+}
+
+// Step j as one replayed hipGraph (captured on first use for these buffers); with
+// EFE_GMRES_NO_GRAPH set, the launches go to the stream one by one.
+template <typename T>
+int run_step(efe_ctx* c, GmresWork* g, int precond, const T* F, const T* Dinv, const T* Minv, const T* K, int j,
+             hipStream_t st) {
+    static const bool graphs = std::getenv("EFE_GMRES_NO_GRAPH") == nullptr;
+    if (!graphs) {
+        gmres_step_launch<T>(c, g, precond, F, Dinv, Minv, K, j, st);
+        EFE_HIP(hipGetLastError());
+        return EFE_OK;
+    }
+    if ((int)g->stepGraph.size() < g->m) {
+        g->stepGraph.resize(g->m, nullptr);
+        g->stepKey.resize(g->m);
+    }
+    const std::array<int64_t, 8> key = {precond,
+                                        (int64_t)sizeof(T),
+                                        (int64_t)(intptr_t)F,
+                                        (int64_t)(intptr_t)Dinv,
+                                        (int64_t)(intptr_t)Minv,
+                                        (int64_t)(intptr_t)K,
+                                        (int64_t)(intptr_t)g->V,
+                                        g->ncolors};
+    if (!g->stepGraph[j] || g->stepKey[j] != key) {
+        if (g->stepGraph[j]) hipGraphExecDestroy(g->stepGraph[j]);
+        g->stepGraph[j] = nullptr;
+        hipGraph_t gr;
+        EFE_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        gmres_step_launch<T>(c, g, precond, F, Dinv, Minv, K, j, st);
+        EFE_HIP(hipStreamEndCapture(st, &gr));
+        EFE_HIP(hipGraphInstantiate(&g->stepGraph[j], gr, nullptr, nullptr, 0));
+        EFE_HIP(hipGraphDestroy(gr));
+        g->stepKey[j] = key;
+    }
+    EFE_HIP(hipGraphLaunch(g->stepGraph[j], st));
+    return EFE_OK;
 }
 
 template <typename T>
@@ -524,18 +703,7 @@ int gmres_solve(efe_ctx* c, GmresWork* g, const double* b, double* xout, const e
         gv[0] = beta;
         int jend = -1;
         for (int j = 0; j < m; ++j) {
-            T* Vj = V + (int64_t)j * D;
-            apply_precond<T>(c, g, o->precond, F, Dinv, Minv, Vj, z, st);
-            gm_spmv_kernel<T><<<gs, BS, 0, st>>>(N, c->slice_off, c->scol, c->slotNode, K, z, w, nullptr);
-            // classical Gram-Schmidt, twice
-            for (int pass = 0; pass < 2; ++pass) {
-                mdot_kernel<T><<<dim3(g->nwg, j + 1), BS, 0, st>>>(D, V, w, g->part, g->nwg);
-                mdot_reduce_kernel<<<j + 1, BS, 0, st>>>(g->part, g->nwg, g->hdev + pass * (m + 2), 0);
-                combine_kernel<T><<<gd, BS, 0, st>>>(D, j + 1, V, g->hdev + pass * (m + 2), w, 0);
-            }
-            mdot_kernel<T><<<dim3(g->nwg, 1), BS, 0, st>>>(D, w, w, g->part, g->nwg);
-            mdot_reduce_kernel<<<1, BS, 0, st>>>(g->part, g->nwg, g->hdev + 2 * (m + 2), 0);
-            scale_kernel<T><<<gd, BS, 0, st>>>(D, w, g->hdev + 2 * (m + 2), V + (int64_t)(j + 1) * D);
+            if (int rc = run_step<T>(c, g, o->precond, F, Dinv, Minv, K, j, st)) return rc;
             EFE_HIP(hipMemcpyAsync(hh, g->hdev, 3 * (m + 2) * sizeof(double), hipMemcpyDeviceToHost, st));
             EFE_HIP(hipStreamSynchronize(st));
             ++it;
@@ -631,6 +799,7 @@ int ensure_gmres(efe_ctx* c, int m, int prec) {
             efe_set_error("GMRES restart length above 256");
             return EFE_ERR_ARG;
         }
+        drop_step_graphs(g);
         hipFree(g->V); hipFree(g->w); hipFree(g->z); hipFree(g->x); hipFree(g->t);
         hipFree(g->part); hipFree(g->hdev);
         const size_t es = prec == EFE_FP32 ? sizeof(float) : sizeof(double);
@@ -665,6 +834,7 @@ void efe_gmres_free(efe_ctx* c) {
                     g->V, g->w, g->z, g->x, g->t, g->part, g->hdev};
     for (void* p : bufs)
         if (p) hipFree(p);
+    drop_step_graphs(g);
     if (g->hhost) hipHostFree(g->hhost);
     if (g->ks) hipStreamDestroy(g->ks);
     if (g->evA) hipEventDestroy(g->evA);

@@ -240,6 +240,29 @@ class DeviceSystem:
                                             ctypes.byref(info)), "efe_linsolve_batched")
         return info, its
 
+    def newton_round_batched(self, dU, PExt, deltaT, live, extrapolated, has_dirichlet, max_iter, max_grow, n_acc,
+                             opts, P=None, F=None):
+        """The Newton iterations of one lockstep round on the device (efe_newton_round_batched):
+        dU (updated in place), PExt (or None), deltaT device [n_dof]; per-specimen host arrays;
+        opts a _lib.NewtonOpts.  Returns (state, iterations, reason, work [n_specimens, 3])."""
+        S = self.n_specimens
+        u8 = lambda a: np.ascontiguousarray(a, dtype=np.uint8)  # noqa: E731
+        i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)  # noqa: E731
+        lv, ex, hd = u8(live), u8(extrapolated), u8(has_dirichlet)
+        mi, mg = i32(max_iter), i32(max_grow)
+        na = np.ascontiguousarray(n_acc, dtype=np.float64)
+        for a in (lv, ex, hd, mi, mg, na):
+            if a.shape != (S,):
+                raise ValueError("per-specimen arrays must have n_specimens entries")
+        state, its, reason = (np.zeros(S, dtype=np.int32) for _ in range(3))
+        work = np.zeros((S, 3), dtype=np.int32)
+        check(self.lib.efe_newton_round_batched(
+            self.ctx, self._ptr(dU), self._ptr(P) if P is not None else None, self._ptr(F) if F is not None else None,
+            self._ptr(PExt) if PExt is not None else None, self._ptr(deltaT), lv.ctypes.data, ex.ctypes.data,
+            hd.ctypes.data, mi.ctypes.data, mg.ctypes.data, na.ctypes.data, ctypes.byref(opts), state.ctypes.data,
+            its.ctypes.data, reason.ctypes.data, work.ctypes.data), "efe_newton_round_batched")
+        return state, its, reason, work
+
     def norms_batched(self, R, ddU, F):
         """[n_specimens, 4]: max|R|, ||ddU||_inf, ||F||_1, NaN flag."""
         out = np.empty((self.n_specimens, 4))

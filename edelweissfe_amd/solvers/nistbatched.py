@@ -42,7 +42,7 @@ class NISTBatched:
     options_default = {"extrapolation": "linear", "precision": "fp64", "krylovAtolFactor": 1e-2,
                        "krylovRtol": 1e-13, "krylovRtolMixed": 1e-4, "krylovRtolMixedFirst": 1e-7,
                        "krylovMaxIter": 50000,
-                       "krylovCheckEvery": 16, "autotune": True}
+                       "krylovCheckEvery": 16, "autotune": True, "newtonOnDevice": True}
 
     def __init__(self, **kwargs):
         self.options = dict(self.options_default)
@@ -183,8 +183,12 @@ class NISTBatched:
                 dU, PExt, deltaT, extrap = self._startIncrements(models, steps, ts, prevTS, live, owners, dU, isDir,
                                                                  perDof)
                 self._cutbackSize = np.zeros(S)
-                state, it = self._newtonRound(sysm, steps, live, extrap, hasDir, dU, P, F, PExt, deltaT, dirT,
-                                              perDof, nAcc, prec, rtol)
+                if self.options["newtonOnDevice"]:
+                    state, it = self._newtonRoundDevice(sysm, steps, live, extrap, hasDir, dU, PExt, deltaT, nAcc,
+                                                        prec, rtol)
+                else:
+                    state, it = self._newtonRound(sysm, steps, live, extrap, hasDir, dU, P, F, PExt, deltaT, dirT,
+                                                  perDof, nAcc, prec, rtol)
                 dU = self._lastDU
                 ok = state == CONVERGED
                 sysm.accept_state_batched(ok)
@@ -264,7 +268,9 @@ class NISTBatched:
         state = np.where(live, ITERATING, IDLE)
         ddU = None
         while True:
-            first = live & (it == 0) & ~extrap & hasDir
+            # a specimen failed in its first pass keeps it == 0 but takes no further pass: only
+            # iterating specimens are first, so the passes after the first form no correction
+            first = live & (it == 0) & ~extrap & hasDir & (state == ITERATING)
             if first.any():
                 dirVals = torch.where(perDof(first), deltaT, torch.zeros_like(deltaT))
                 corr = torch.empty(D, dtype=torch.float64, device=dev)
@@ -330,6 +336,34 @@ class NISTBatched:
             it[go] += 1
         self._lastDU = dU
         return state, it
+
+    _REASONS = {_lib.EFE_FAIL_CUTBACK: "cutback", _lib.EFE_FAIL_NAN: "nan", _lib.EFE_FAIL_ITERATIONS: "iterations",
+                _lib.EFE_FAIL_KRYLOV: "krylov"}
+
+    def _newtonRoundDevice(self, sysm, steps, live, extrap, hasDir, dU, PExt, deltaT, nAcc, prec, rtol):
+        """_newtonRound with every decision on the device (efe_newton_round_batched): one
+        replayed hipGraph per Newton pass, the same per-specimen arithmetic and outcomes."""
+        o = self.options
+        opts = _lib.NewtonOpts(precision=prec, krylov_max_iter=o["krylovMaxIter"], check_every=o["krylovCheckEvery"],
+                               mixed_first=1 if self._mixed else 0, alt_after=15, rtol=rtol,
+                               rtol_first=o["krylovRtolMixedFirst"], atol_factor=o["krylovAtolFactor"],
+                               flux_tol=FLUX_RESIDUAL_TOL, flux_tol_alt=FLUX_RESIDUAL_TOL_ALT,
+                               correction_tol=FIELD_CORRECTION_TOL, flux_floor=1e-7, avg_floor=1e-10)
+        maxIter = [st.maxIter for st in steps]
+        maxGrow = [st.maxGrowIter for st in steps]
+        state, it, reason, work = sysm.newton_round_batched(dU, PExt, deltaT, live, extrap, hasDir, maxIter, maxGrow,
+                                                            nAcc, opts)
+        self.elementPasses += int(work[:, 0].max()) if live.any() else 0
+        self.linearSolves += int(work[:, 1].sum())
+        self.krylovIterations += int(work[:, 2].sum())
+        self.specimenPasses += work[:, 0]
+        self.specimenSolves += work[:, 1]
+        self.specimenKrylov += work[:, 2]
+        for s in np.flatnonzero(state == FAILED):
+            self.failures[s].append(self._REASONS[int(reason[s])])
+        self._cutbackSize[reason == _lib.EFE_FAIL_CUTBACK] = 0.5
+        self._lastDU = dU
+        return state, it.astype(int)
 
     def qpStates(self, models, which=0):
         """Reference (0) or temporary (1) QP state of every specimen, as a list of

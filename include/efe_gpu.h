@@ -212,6 +212,56 @@ int efe_linsolve_batched(efe_ctx* ctx, const double* b, double* x, const efe_sol
  * femodel.py:245-258 per specimen model. */
 int efe_accept_state_batched(efe_ctx* ctx, const uint8_t* accept);
 
+/* Options of a device-resident Newton round (efe_newton_round_batched). */
+typedef struct efe_newton_opts {
+    int32_t precision;       /* EFE_FP64, EFE_FP32 or EFE_FP32_MATFREE: element pass, operator and PCG */
+    int32_t krylov_max_iter; /* PCG iteration limit; a specimen that reaches it fails its increment */
+    int32_t check_every;     /* PCG iterations per replayed batch */
+    int32_t mixed_first;     /* 1: a first iteration (prescribed increment applied) solves to rtol_first */
+    int32_t alt_after;       /* iterations after which flux_tol_alt replaces flux_tol (nonlinearimplicitstatic.py:666-669: 15) */
+    int32_t reserved;
+    double rtol;             /* PCG relative tolerance */
+    double rtol_first;       /* with mixed_first */
+    double atol_factor;      /* PCG absolute tolerance = atol_factor x the specimen's flux threshold */
+    double flux_tol;         /* fluxResidualTolerance (1e-8) */
+    double flux_tol_alt;     /* fluxResidualToleranceAlternative (5e-3) */
+    double correction_tol;   /* fieldCorrectionTolerance (1e-8) */
+    double flux_floor;       /* lower bound of the flux threshold (1e-7) */
+    double avg_floor;        /* lower bound of the average flux (1e-10) */
+} efe_newton_opts;
+
+#define EFE_NEWTON_ITERATING 0
+#define EFE_NEWTON_CONVERGED 1
+#define EFE_NEWTON_FAILED 2
+#define EFE_NEWTON_IDLE 3
+#define EFE_FAIL_NONE 0
+#define EFE_FAIL_CUTBACK 1    /* a material asked for a smaller increment */
+#define EFE_FAIL_NAN 2        /* NaN in the residual or the correction */
+#define EFE_FAIL_ITERATIONS 3 /* maxIter or maxGrowingIter reached */
+#define EFE_FAIL_KRYLOV 4     /* the PCG reached krylov_max_iter or returned a non-finite correction */
+
+/* Batched contexts: the Newton iterations of one lockstep round, every specimen's convergence
+ * test, divergence count and cutback decided on the device (the loop of
+ * nonlinearimplicitstatic.py:405-458 and its tests :625-725, per specimen).  One pass is one
+ * replayed hipGraph: the previous solve's correction applied, element pass and assembly over the
+ * iterating specimens, residual, per-specimen norms and decisions, PCG set-up and the first
+ * check_every PCG iterations; further PCG batches follow until every solving specimen has
+ * converged.  The host reads two counts per PCG batch and nothing else.
+ * dU, P, F, PExt (or NULL), deltaT: device [n_dof]; dU holds the increment's start (the
+ * extrapolated increment, deltaT on prescribed DOFs of extrapolated specimens) and receives the
+ * round's increments; P and F (or NULL) receive the last pass's.  deltaT: the prescribed
+ * increments (taken on prescribed DOFs of a specimen's first iteration).  Host [n_specimens]:
+ * live (0: the specimen idles), extrapolated, has_dirichlet, max_iter, max_grow (maxIter,
+ * maxGrowingIter of its step), n_acc (element DOF count: the average-flux divisor).  Out, host
+ * [n_specimens]: state (EFE_NEWTON_*), iterations, reason (EFE_FAIL_*), work [n_specimens][3]
+ * (element passes, linear solves, PCG iterations of the round).  No counterpart in the
+ * reference, which iterates one model on the host. */
+int efe_newton_round_batched(efe_ctx* ctx, double* dU, double* P, double* F, const double* PExt, const double* deltaT,
+                             const uint8_t* live, const uint8_t* extrapolated, const uint8_t* has_dirichlet,
+                             const int32_t* max_iter, const int32_t* max_grow, const double* n_acc,
+                             const efe_newton_opts* opts, int32_t* state, int32_t* iterations, int32_t* reason,
+                             int32_t* work);
+
 /* After an efe_compute_elements that returned EFE_CUTBACK on a batched context: out[s] = 1
  * for the specimens whose material asked for a smaller increment (host [n_specimens]);
  * the CutbackRequest of vonmises.py:230-231 raised per specimen model. */

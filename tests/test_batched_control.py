@@ -21,7 +21,8 @@ def _run(specimens):
         steps.append(device_step(pm, uniaxial_bcs(strain), maxInc=maxInc, minInc=1e-6, maxNumInc=maxNumInc,
                                  maxIter=15))
         oms.append(oracle_model(n, (1.0, 1.0, 1.0), "C3D8", "vonmises", props, jitter=0.1, seed=seed))
-    solver = NISTBatched(precision="fp64", autotune=False)
+    # the host-side round: the mock stands in for the device calls it makes
+    solver = NISTBatched(precision="fp64", autotune=False, newtonOnDevice=False)
     mock = attach_mock(solver, models, oms)
     ok, _ = solver.solveSteps(models, steps)
     return solver, mock, models, ok
@@ -76,7 +77,7 @@ def test_groups_by_element_and_material_follow_each_oracle_run(monkeypatch):
         return attach_mock(self, group_models, [oms[id(m)] for m in group_models])
 
     monkeypatch.setattr(NISTBatched, "build", build)
-    solver = NISTBatchedGroups(precision="fp64", autotune=False)
+    solver = NISTBatchedGroups(precision="fp64", autotune=False, newtonOnDevice=False)
     ok, _ = solver.solveSteps(models, steps)
     assert ok and [g[1] for g in solver.groups] == [[0, 3], [1, 4], [2]]
     states = solver.qpStates(models)

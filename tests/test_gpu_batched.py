@@ -127,6 +127,51 @@ def test_batched_norms_solve_and_selective_accept():
     assert np.abs(tmp_state[ea:, :, 0:6]).max() > 0
 
 
+def test_per_specimen_relative_tolerances_match_solves_alone():
+    """efe_linsolve_batched with spec_rtol: each specimen stops by its own relative tolerance;
+    its iteration count and solution equal, bit for bit, those of the specimen solved as the
+    only one of its own system with that tolerance."""
+    from edelweissfe_amd.device import DeviceSystem
+
+    sysm, a, b = _two_specimen_system()
+    D, da = sysm.n_dof, 3 * a.nNodes
+    g = torch.Generator(device="cpu").manual_seed(11)
+    dU = (torch.rand(D, generator=g, dtype=torch.float64) * 4e-3 - 2e-3).cuda()
+    dirs = np.r_[a.dofsOfNodeSet("gen_bottom"), da + b.dofsOfNodeSet("gen_bottom")]
+    sysm.set_dirichlet(dirs)
+    P, F = sysm.zeros(), sysm.zeros()
+    sysm.compute_elements(dU, P, F)
+    rhs = P.clone()
+    rhs[torch.as_tensor(dirs, device=rhs.device)] = 0.0
+    rtols = [1e-7, 1e-4]
+    x = torch.empty_like(rhs)
+    info, its = sysm.linsolve_batched(rhs, x, np.array([1e-300, 1e-300]), spec_rtol=rtols, rtol=0.5, max_iter=5000,
+                                      check_every=1)
+    assert info.status == 0 and (its > 0).all()
+    # the tolerances swapped: each specimen's count follows its own tolerance
+    _, its_sw = sysm.linsolve_batched(rhs, torch.empty_like(rhs), np.array([1e-300, 1e-300]), spec_rtol=rtols[::-1],
+                                      rtol=0.5, max_iter=5000, check_every=1)
+    assert its_sw[0] < its[0] and its_sw[1] > its[1]
+    for s, (m, sl, d) in enumerate(((a, slice(0, da), a.dofsOfNodeSet("gen_bottom")),
+                                    (b, slice(da, D), b.dofsOfNodeSet("gen_bottom")))):
+        one = DeviceSystem(m.coordinates, m.connectivity, "C3D8", "vonmises",
+                           np.tile(m.material["properties"], (m.nElements, 1)),
+                           specimen_of_node=np.zeros(m.nNodes, np.int32), n_specimens=1)
+        one.set_dirichlet(d)
+        P1, F1 = one.zeros(), one.zeros()
+        one.compute_elements(dU[sl].contiguous(), P1, F1)
+        assert torch.equal(P1, P[sl])
+        r1 = P1.clone()
+        r1[torch.as_tensor(d, device=r1.device)] = 0.0
+        x1 = torch.empty_like(r1)
+        info1, its1 = one.linsolve_batched(r1, x1, np.array([1e-300]), rtol=rtols[s], max_iter=5000, check_every=1)
+        print(s, rtols[s], its[s], its1[0], flush=True)
+        assert its1[0] == its[s]
+        assert torch.equal(x1, x[sl])
+        one.close()
+    sysm.close()
+
+
 def test_material_cutbacks_and_step_failure_stay_per_specimen():
     from edelweissfe_amd.solvers.nistbatched import NISTBatched
 
@@ -208,3 +253,56 @@ def test_groups_by_element_and_material_on_the_device():
         assert solver.history[s] == want
         assert rel_err(models[s].U, om.U) < 1e-9
         assert rel_err(states[s][..., 0:6], om.state[..., 0:6]) < 1e-9
+
+
+def _run_specimens(specs, precision, newton_on_device, **kw):
+    from edelweissfe_amd.solvers.nistbatched import NISTBatched
+
+    models, steps = [], []
+    for n, props, strain, maxInc, maxNumInc, maxIter, seed in specs:
+        pm = device_model(n, (1.0, 1.0, 1.0), "C3D8", "vonmises", props, jitter=0.1, seed=seed)
+        models.append(pm)
+        steps.append(device_step(pm, uniaxial_bcs(strain), maxInc=maxInc, minInc=1e-6, maxNumInc=maxNumInc,
+                                 maxIter=maxIter))
+    solver = NISTBatched(precision=precision, autotune=False, newtonOnDevice=newton_on_device, **kw)
+    ok, _ = solver.solveSteps(models, steps)
+    return solver, models, ok
+
+
+@pytest.mark.parametrize("case", ["plain-fp64", "plain-mixed", "cutbacks-fp64", "cutbacks-mixed"])
+def test_newton_round_on_device_equals_host_round_bitwise(case):
+    """efe_newton_round_batched (every convergence, divergence, iteration-limit, cutback and
+    Krylov decision on the device, one replayed hipGraph per Newton pass) against the host
+    round it replaces: the same increments, Newton counts, failure reasons, work counts,
+    failed specimens, and the same U and QP state bit for bit.  The cases take in a specimen
+    that cuts back by its iteration limit (maxIter 2), material cutbacks and a step failure."""
+    kind, precision = case.split("-")
+    if kind == "plain":
+        specs = [(n, _props(E, fy), strain, maxInc, 100, maxIter, s)
+                 for s, (n, E, fy, strain, maxInc, maxIter) in enumerate(SPECIMENS)]
+        kw = {}
+    else:
+        specs = [(n, props, strain, maxInc, maxNumInc, 15, seed)
+                 for n, props, strain, maxInc, maxNumInc, seed in CUTBACK_SPECIMENS]
+        kw = {"krylovAtolFactor": 1e-8}
+    host, mh, okh = _run_specimens(specs, precision, False, **kw)
+    dev, md, okd = _run_specimens(specs, precision, True, **kw)
+    print(case, host.history, dev.history, host.failures, dev.failures, flush=True)
+    assert okh == okd
+    assert host.history == dev.history
+    assert host.failures == dev.failures
+    assert host.failedSpecimens == dev.failedSpecimens
+    assert np.array_equal(host.specimenPasses, dev.specimenPasses)
+    assert np.array_equal(host.specimenSolves, dev.specimenSolves)
+    assert np.array_equal(host.specimenKrylov, dev.specimenKrylov)
+    assert (host.elementPasses, host.linearSolves, host.krylovIterations) == (dev.elementPasses, dev.linearSolves,
+                                                                            dev.krylovIterations)
+    for a, b in zip(mh, md):
+        assert np.array_equal(a.U, b.U)
+    for a, b in zip(host.qpStates(mh), dev.qpStates(md)):
+        assert np.array_equal(a, b)
+    if precision == "fp64":  # the cases take the paths they are meant to
+        if kind == "plain":
+            assert np.inf in dev.history[1]
+        else:
+            assert "cutback" in dev.failures[0] and dev.failedSpecimens == [2]

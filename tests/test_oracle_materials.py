@@ -163,3 +163,27 @@ def test_drucker_prager_apex_without_dilatancy_asks_for_a_cutback():
     assert ei.value.cutbackSize == 0.5
     s, C, e = _call(mat, np.zeros(6), 0.0, np.array([-2e-3, 1.5e-3, 0.5e-3, 1.0e-3, -0.5e-3, 0.8e-3]))
     assert np.all(np.isfinite(s)) and np.all(np.isfinite(C)) and e > 0
+
+
+def test_sections_update_each_element_with_its_own_material():
+    """Sections (sectionbase.py:86-128: one material per element set): each QP takes exactly
+    the update of its element's material run alone; a state column a material does not use
+    stays as it was."""
+    rng = np.random.default_rng(5)
+    nq, E = 8, 6
+    of = np.array([0, 1, 2, 1, 0, 2])
+    mats = [materials.VonMises(J2), materials.LinearElastic([30000.0, 0.25]), materials.DruckerPrager(DP)]
+    sec = materials.Sections(mats, of, nq)
+    n = E * nq
+    stress = rng.uniform(-50, 50, (n, 6))
+    mstate = np.column_stack([rng.uniform(0, 1e-3, n)])
+    dstrain = rng.uniform(-3e-3, 3e-3, (n, 6))
+    s, C, m = sec.compute(stress, mstate, dstrain)
+    of_qp = np.repeat(of, nq)
+    for k, mat in enumerate(mats):
+        sel = of_qp == k
+        s_k, C_k, m_k = mat.compute(stress[sel], mstate[sel][:, :mat.n_state], dstrain[sel])
+        assert np.array_equal(s[sel], s_k) and np.array_equal(C[sel], C_k)
+        assert np.array_equal(m[sel][:, :mat.n_state], m_k)
+        assert np.array_equal(m[sel][:, mat.n_state:], mstate[sel][:, mat.n_state:])
+    assert (m[of_qp == 0][:, 0] > mstate[of_qp == 0][:, 0]).any()  # J2 QPs went plastic
