@@ -157,6 +157,29 @@ def profile_spmv(system, n_launch=20, copy_mb=512):
         "iterations" % (n_launch, alg, n_launch, copy_mb, n_launch))
 
 
+def profile_elements(system, n_pass=3, n_spmv=5):
+    """Counter-collection mode of the footing workload: n_pass FP64 and n_pass assembled FP32
+    element passes (with assembly) of the C3D20R Drucker-Prager operator at a random increment
+    of +-1e-3 per DOF (seed 3: plastic QPs on the cone and at the apex), then n_spmv FP64 SpMVs."""
+    import torch
+
+    from edelweissfe_amd import _lib
+
+    g = torch.Generator(device="cpu").manual_seed(3)
+    dU = (torch.rand(system.n_dof, generator=g, dtype=torch.float64) * 2e-3 - 1e-3).cuda()
+    P, F = system.zeros(), system.zeros()
+    for prec in (_lib.EFE_FP64, _lib.EFE_FP32):
+        for _ in range(n_pass):
+            system.compute_elements(dU, P, F, precision=prec)
+    x = torch.randn(system.n_dof, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    system.compute_elements(dU, P, F)
+    for _ in range(n_spmv):
+        system.spmv(x, y)
+    torch.cuda.synchronize()
+    log("[profile] %d FP64 and %d FP32 element passes with assembly, %d FP64 SpMVs" % (n_pass, n_pass, n_spmv))
+
+
 def load_traffic(name):
     p = os.path.join(ROOT, "profiles", name)
     if os.path.exists(p):
@@ -404,6 +427,9 @@ def bench_footing(args):
     system = DeviceSystem(template.coordinates, template.connectivity, "C3D20R", "druckerprager", DP)
     torch.cuda.synchronize()
     log("[bench] device system ready: %d DOFs, %d nnz (%.1f s)" % (system.n_dof, system.nnz, time.perf_counter() - t0))
+    if args.profile_spmv:
+        profile_elements(system)
+        return
     pt0 = system.phase_totals()
     t64, s64, m64 = run_footing(system, template, steps_for, "fp64", args.steps, args.warmup)
     pt1 = system.phase_totals()
@@ -504,7 +530,9 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--profile-spmv", action="store_true", help="counter-collection mode (no JSON line)")
+    ap.add_argument("--profile-spmv", action="store_true",
+                    help="counter-collection mode (no JSON line): cube, the operators of the load history; "
+                         "footing, the C3D20R element passes and SpMV")
     ap.add_argument("--settlement", type=float, default=0.005)
     ap.add_argument("--workload", choices=("cube", "lognormal", "batched", "footing"), default="cube",
                     help="cube: 64^3 J2 load history (default); lognormal: 96^3 J2 with per-element lognormal "

@@ -29,6 +29,7 @@ struct GmresWork {
     int32_t *fptr = nullptr, *fcol = nullptr, *fidx = nullptr, *dslot = nullptr, *perm = nullptr;
     int64_t nbF = 0;
     int maxUpper = 0;  // most blocks in a row's upper part
+    int maxRow = 0;    // most blocks in a row
     double *F64 = nullptr, *Dinv64 = nullptr;
     float *F32 = nullptr, *Dinv32 = nullptr;
     int factored_version[2] = {-1, -1};
@@ -250,6 +251,104 @@ __global__ __launch_bounds__(BS) void ilu_factor_lds_kernel(int r0, int r1, cons
         inv3d(dd, di);
 #pragma unroll
         for (int v = 0; v < 9; ++v) Dinv[(int64_t)i * 9 + v] = (T)di[v];
+    }
+}
+
+// ilu_factor_lds_kernel with the row's own blocks held in registers: the G lanes of a row own
+// its blocks b0 + lane + t G (t < NT) from the start, update them there and write them once at
+// the end.  What remains per elimination step k is k's upper column list (LDS), Dinv_k and the
+// U_kj blocks the row meets.  The same updates in the same order as ilu_factor_kernel: the
+// factor is bitwise identical.  Needs rows of at most G NT blocks and upper parts of at most
+// CAP blocks.
+template <typename T, int G, int NT, int CAP>
+__global__ __launch_bounds__(BS) void ilu_factor_reg_kernel(int r0, int r1, const int32_t* __restrict__ fptr,
+                                                            const int32_t* __restrict__ fcol,
+                                                            const int32_t* __restrict__ dslot, T* __restrict__ F,
+                                                            T* __restrict__ Dinv) {
+    __shared__ int32_t cols[(BS / G) * CAP];
+    const int tid = blockIdx.x * BS + threadIdx.x;
+    const int i = r0 + tid / G, lane = tid % G;
+    int32_t* kc = cols + (threadIdx.x / G) * CAP;
+    if (i >= r1) return;  // a whole group leaves together
+    const int b0 = fptr[i], b1 = fptr[i + 1], d = b0 + dslot[i];
+    T blk[NT][9];
+    int col[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int jb = b0 + lane + t * G;
+        col[t] = jb < b1 ? fcol[jb] : -1;
+#pragma unroll
+        for (int v = 0; v < 9; ++v) blk[t][v] = jb < b1 ? F[(int64_t)jb * 9 + v] : (T)0;
+    }
+    for (int kb = b0; kb < d; ++kb) {
+        const int rel = kb - b0, owner = rel % G, tk = rel / G;
+        int ck = 0;
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+            if (t == tk) ck = col[t];
+        const int k = __shfl(ck, owner, G);
+        const int u0 = fptr[k] + dslot[k] + 1, nu = fptr[k + 1] - u0;
+        for (int t = lane; t < nu; t += G) kc[t] = fcol[u0 + t];
+        T lik[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (lane == owner) {
+            T a[9];
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+                if (t == tk)
+#pragma unroll
+                    for (int v = 0; v < 9; ++v) a[v] = blk[t][v];
+            mat3mul(a, Dinv + (int64_t)k * 9, lik);
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+                if (t == tk)
+#pragma unroll
+                    for (int v = 0; v < 9; ++v) blk[t][v] = lik[v];
+        }
+#pragma unroll
+        for (int v = 0; v < 9; ++v) lik[v] = __shfl(lik[v], owner, G);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int jb = b0 + lane + t * G;
+            if (jb <= kb || jb >= b1) continue;
+            const int cj = col[t];
+            int lo = 0, hi = nu;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (kc[mid] < cj) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo < nu && kc[lo] == cj) {
+                T u[9], p[9];
+                const int64_t ub = (int64_t)(u0 + lo) * 9;
+#pragma unroll
+                for (int v = 0; v < 9; ++v) u[v] = F[ub + v];
+                mat3mul(lik, u, p);
+#pragma unroll
+                for (int v = 0; v < 9; ++v) blk[t][v] -= p[v];
+            }
+        }
+        // every lane's searches are done before the list is overwritten
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int jb = b0 + lane + t * G;
+        if (jb >= b1) continue;
+#pragma unroll
+        for (int v = 0; v < 9; ++v) F[(int64_t)jb * 9 + v] = blk[t][v];
+        if (jb == d) {
+            double dd[9], di[9];
+#pragma unroll
+            for (int v = 0; v < 9; ++v) dd[v] = (double)blk[t][v];
+            inv3d(dd, di);
+#pragma unroll
+            for (int v = 0; v < 9; ++v) Dinv[(int64_t)i * 9 + v] = (T)di[v];
+        }
     }
 }
 
@@ -507,7 +606,11 @@ int build_ilu(efe_ctx* c, GmresWork* g) {
         (void)l;
     }
     g->maxUpper = 0;
-    for (int i = 0; i < N; ++i) g->maxUpper = std::max(g->maxUpper, fptr[i + 1] - fptr[i] - dslot[i] - 1);
+    g->maxRow = 0;
+    for (int i = 0; i < N; ++i) {
+        g->maxUpper = std::max(g->maxUpper, fptr[i + 1] - fptr[i] - dslot[i] - 1);
+        g->maxRow = std::max(g->maxRow, fptr[i + 1] - fptr[i]);
+    }
     g->ncolors = ip.ncolors;
     g->cstart.assign(ip.color_start.begin(), ip.color_start.end());
     g->nbF = fptr[N];
@@ -536,7 +639,16 @@ int ilu_factor(efe_ctx* c, GmresWork* g, const T* K, T*& F, T*& Dinv, int pi, hi
     for (int col = 0; col < g->ncolors; ++col) {
         const int r0 = g->cstart[col], r1 = g->cstart[col + 1];
         if (r1 <= r0) continue;
-        if (g->maxUpper <= kFactorCap) {  // row k's upper columns staged in LDS
+        // the row in registers: hexa8 rows have up to 27 blocks, hexa20 rows up to 81.  (Issuing
+        // the next steps' row-k loads ahead, with a second LDS list buffer, was measured: no
+        // gain at 48^3 hexa20, the extra registers cost as much as the hidden latency.)
+        if (g->maxUpper <= kFactorCap && g->maxRow <= 32) {
+            ilu_factor_reg_kernel<T, 16, 2, kFactorCap><<<grid_of((r1 - r0) * 16), BS, 0, st>>>(
+                r0, r1, g->fptr, g->fcol, g->dslot, F, Dinv);
+        } else if (g->maxUpper <= kFactorCap && g->maxRow <= 96) {
+            ilu_factor_reg_kernel<T, 32, 3, kFactorCap><<<grid_of((r1 - r0) * 32), BS, 0, st>>>(
+                r0, r1, g->fptr, g->fcol, g->dslot, F, Dinv);
+        } else if (g->maxUpper <= kFactorCap) {  // row k's upper columns staged in LDS
             if (c->NN == 8)
                 ilu_factor_lds_kernel<T, 8, kFactorCap><<<grid_of((r1 - r0) * 8), BS, 0, st>>>(
                     r0, r1, g->fptr, g->fcol, g->dslot, F, Dinv);
