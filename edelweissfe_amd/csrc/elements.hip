@@ -67,10 +67,8 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
         return (specActive && !specActive[specOfElem[e]]) ? -1 : e;
     };
     constexpr int NP = (MAT == EFE_LINEAR_ELASTIC) ? 2 : 6;
-    // padded by one value per element, so that lanes of the same (q, a) in EPW elements (the
-    // assembled passes' phase 2) read different banks
-    __shared__ double g_s[EPW][Q * NN * 3 + 1];
-    __shared__ KT fC_s[EPW][Q * 36 + 1];
+    __shared__ double g_s[EPW][Q * NN * 3];
+    __shared__ KT fC_s[EPW][Q * 36];
     __shared__ double fs_s[EPW][Q][6];
     const int lane = threadIdx.x;
     const int64_t EQ = (int64_t)E * Q;
