@@ -67,8 +67,11 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
         return (specActive && !specActive[specOfElem[e]]) ? -1 : e;
     };
     constexpr int NP = (MAT == EFE_LINEAR_ELASTIC) ? 2 : 6;
+    // the tangent of every QP in compact form, w detJ [A, B, c_nn, (c_n1, c_1n)] then n:
+    // C = A 1(x)1 + B Idev + c_nn n(x)n (+ c_n1 n(x)1 + c_1n 1(x)n for Drucker-Prager)
+    constexpr int NCC = (MAT == EFE_DRUCKER_PRAGER) ? 11 : 9;
     __shared__ double g_s[EPW][Q * NN * 3];
-    __shared__ KT fC_s[EPW][Q * 36];
+    __shared__ KT fC_s[EPW][Q * NCC];
     __shared__ double fs_s[EPW][Q][6];
     const int lane = threadIdx.x;
     const int64_t EQ = (int64_t)E * Q;
@@ -135,9 +138,9 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
         for (int k = 0; k < NP; ++k) p[k] = propStride ? props[(int64_t)k * propStride + e] : props[k];
         double C[36], cc[11];
         int st = 0;
-        if (MAT == EFE_LINEAR_ELASTIC) st = efe::le_update(p, s, de, C, MF ? cc : nullptr);
-        else if (MAT == EFE_VONMISES) st = efe::vm_update(p, s, ms, de, C, MF ? cc : nullptr);
-        else st = efe::dp_update(p, s, ms, de, C, MF ? cc : nullptr);
+        if (MAT == EFE_LINEAR_ELASTIC) st = efe::le_update(p, s, de, C, cc);
+        else if (MAT == EFE_VONMISES) st = efe::vm_update(p, s, ms, de, C, cc);
+        else st = efe::dp_update(p, s, ms, de, C, cc);
         if (st) {
             *status = 1;
             if (specStatus) specStatus[specOfElem[e]] = 1;  // which specimen asked (batched contexts)
@@ -157,9 +160,12 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
         if (MatState<MAT>::n) sTmp[12 * EQ + sidx] = ms;
         (void)NS;
         const double f = det * qpw[q];
-        if (formK)
+        if (formK) {
 #pragma unroll
-            for (int k = 0; k < 36; ++k) fC_s[el][q * 36 + k] = (KT)(f * C[k]);
+            for (int k = 0; k < NCC - 6; ++k) fC_s[el][q * NCC + k] = (KT)(f * cc[k]);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) fC_s[el][q * NCC + NCC - 6 + k] = (KT)cc[5 + k];
+        }
 #pragma unroll
         for (int k = 0; k < 6; ++k) fs_s[el][q][k] = f * s[k];
     }
@@ -198,13 +204,30 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
         for (int q = 0; q < Q; ++q) {
             const KT ax = (KT)g_s[el][(q * NN + a) * 3 + 0], ay = (KT)g_s[el][(q * NN + a) * 3 + 1], az = (KT)g_s[el][(q * NN + a) * 3 + 2];
             const KT bx = (KT)g_s[el][(q * NN + b) * 3 + 0], by = (KT)g_s[el][(q * NN + b) * 3 + 1], bz = (KT)g_s[el][(q * NN + b) * 3 + 2];
-            const KT* C = fC_s[el] + q * 36;
+            // CB[:, j] = C B_b e_j from the compact tangent: the strain of column j has the
+            // normal component b_j and two engineering shears (12/13, 12/23, 13/23)
+            const KT* cq = fC_s[el] + q * NCC;
+            const KT cA = cq[0], cB = cq[1], cnn = cq[2];
+            const KT cn1 = NCC == 11 ? cq[3] : (KT)0, c1n = NCC == 11 ? cq[4] : (KT)0;
+            const KT* n = cq + NCC - 6;
+            const KT bv[3] = {bx, by, bz};
             KT CB[6][3];
 #pragma unroll
-            for (int r = 0; r < 6; ++r) {
-                CB[r][0] = C[r * 6 + 0] * bx + C[r * 6 + 3] * by + C[r * 6 + 4] * bz;
-                CB[r][1] = C[r * 6 + 1] * by + C[r * 6 + 3] * bx + C[r * 6 + 5] * bz;
-                CB[r][2] = C[r * 6 + 2] * bz + C[r * 6 + 4] * bx + C[r * 6 + 5] * by;
+            for (int j = 0; j < 3; ++j) {
+                KT v[6] = {0, 0, 0, 0, 0, 0};
+                v[j] = bv[j];
+                if (j == 0) { v[3] = by; v[4] = bz; }
+                if (j == 1) { v[3] = bx; v[5] = bz; }
+                if (j == 2) { v[4] = bx; v[5] = by; }
+                const KT tr = bv[j];
+                KT ne = n[j] * bv[j];
+#pragma unroll
+                for (int k = 3; k < 6; ++k) ne += n[k] * v[k];
+                const KT sn = cnn * ne + cn1 * tr, s1 = cA * tr + c1n * ne, t3 = tr * (KT)(1.0 / 3.0);
+#pragma unroll
+                for (int i = 0; i < 3; ++i) CB[i][j] = s1 + cB * (v[i] - t3) + sn * n[i];
+#pragma unroll
+                for (int k = 3; k < 6; ++k) CB[k][j] = (KT)0.5 * cB * v[k] + sn * n[k];
             }
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
