@@ -26,6 +26,7 @@ struct GmresWork {
     bool ilu_ready = false;
     int ncolors = 0;
     std::vector<int> cstart;
+    std::vector<int> clow, cup;  // per color: most blocks before / after the diagonal in a row
     int32_t *fptr = nullptr, *fcol = nullptr, *fidx = nullptr, *dslot = nullptr, *perm = nullptr;
     int64_t nbF = 0;
     int maxUpper = 0;  // most blocks in a row's upper part
@@ -391,11 +392,13 @@ __device__ __forceinline__ void group_sum3(T& a, T& b, T& c) {
 // blocks at a time: their column indices, then the t entries they name, then the blocks are
 // loaded together, so one pass costs two dependent memory round trips, not two per block.
 // Each lane adds its blocks in the same order as one at a time.
+// With Dinv given (the last color, whose rows have no blocks after them), the backward sweep's
+// t_i = Dinv_i (t_i - 0) follows in the same launch.
 template <typename T, int G, int U>
 __global__ __launch_bounds__(BS) void ilu_lower_kernel(int r0, int r1, const int32_t* __restrict__ fptr,
                                                        const int32_t* __restrict__ fcol,
                                                        const int32_t* __restrict__ dslot, const T* __restrict__ F,
-                                                       T* __restrict__ t) {
+                                                       T* __restrict__ t, const T* __restrict__ Dinv) {
     const int tid = blockIdx.x * BS + threadIdx.x;
     const int i = r0 + tid / G, lane = tid % G;
     if (i >= r1) return;
@@ -421,9 +424,19 @@ __global__ __launch_bounds__(BS) void ilu_lower_kernel(int r0, int r1, const int
     }
     group_sum3<T, G>(y0, y1, y2);
     if (lane == 0) {
-        t[3 * i] -= y0;
-        t[3 * i + 1] -= y1;
-        t[3 * i + 2] -= y2;
+        y0 = t[3 * i] - y0;
+        y1 = t[3 * i + 1] - y1;
+        y2 = t[3 * i + 2] - y2;
+        if (Dinv) {
+            const T* m = Dinv + (int64_t)i * 9;
+            const T z0 = y0 - (T)0, z1 = y1 - (T)0, z2 = y2 - (T)0;  // as the backward sweep forms them
+            y0 = m[0] * z0 + m[1] * z1 + m[2] * z2;
+            y1 = m[3] * z0 + m[4] * z1 + m[5] * z2;
+            y2 = m[6] * z0 + m[7] * z1 + m[8] * z2;
+        }
+        t[3 * i] = y0;
+        t[3 * i + 1] = y1;
+        t[3 * i + 2] = y2;
     }
 }
 
@@ -613,6 +626,13 @@ int build_ilu(efe_ctx* c, GmresWork* g) {
     }
     g->ncolors = ip.ncolors;
     g->cstart.assign(ip.color_start.begin(), ip.color_start.end());
+    g->clow.assign(g->ncolors, 0);
+    g->cup.assign(g->ncolors, 0);
+    for (int col = 0; col < g->ncolors; ++col)
+        for (int i = g->cstart[col]; i < g->cstart[col + 1]; ++i) {
+            g->clow[col] = std::max(g->clow[col], dslot[i]);
+            g->cup[col] = std::max(g->cup[col], fptr[i + 1] - fptr[i] - dslot[i] - 1);
+        }
     g->nbF = fptr[N];
     EFE_HIP(hipMalloc(&g->fptr, fptr.size() * sizeof(int32_t)));
     EFE_HIP(hipMalloc(&g->fcol, fcol.size() * sizeof(int32_t)));
@@ -676,18 +696,26 @@ void apply_precond(efe_ctx* c, GmresWork* g, int precond, const T* F, const T* D
         T* t = (T*)g->t;
         permute_in_kernel<T><<<grid_of(N), BS, 0, st>>>(N, g->perm, v, t);
         const bool h8 = c->NN == 8;
+        const int last = g->ncolors - 1;
+        // the last color's backward sweep rides on its forward sweep when its rows have no
+        // blocks after them; a color whose rows have no blocks before them (the first) has no
+        // forward sweep
+        const bool fuseLast = last >= 0 && g->cup[last] == 0;
         for (int col = 0; col < g->ncolors; ++col) {
             const int r0 = g->cstart[col], r1 = g->cstart[col + 1];
-            if (r1 <= r0) continue;
+            const bool fuse = fuseLast && col == last;
+            if (r1 <= r0 || (g->clow[col] == 0 && !fuse)) continue;
+            const T* D = fuse ? Dinv : nullptr;
             if (h8)
-                ilu_lower_kernel<T, 8, 4><<<grid_of((r1 - r0) * 8), BS, 0, st>>>(r0, r1, g->fptr, g->fcol, g->dslot, F, t);
+                ilu_lower_kernel<T, 8, 4><<<grid_of((r1 - r0) * 8), BS, 0, st>>>(r0, r1, g->fptr, g->fcol, g->dslot, F, t,
+                                                                              D);
             else
                 ilu_lower_kernel<T, 16, 4><<<grid_of((r1 - r0) * 16), BS, 0, st>>>(r0, r1, g->fptr, g->fcol, g->dslot, F,
-                                                                                t);
+                                                                                t, D);
         }
         for (int col = g->ncolors - 1; col >= 0; --col) {
             const int r0 = g->cstart[col], r1 = g->cstart[col + 1];
-            if (r1 <= r0) continue;
+            if (r1 <= r0 || (fuseLast && col == last)) continue;
             if (h8)
                 ilu_upper_kernel<T, 8, 4><<<grid_of((r1 - r0) * 8), BS, 0, st>>>(r0, r1, g->fptr, g->fcol, g->dslot, F,
                                                                               Dinv, t);

@@ -152,3 +152,48 @@ def test_entry_points_reject_bad_arguments_and_keep_the_context_usable():
     with pytest.raises(Exception, match="matrix-free"):
         sysm.element_matrices([0])
     sysm.close()
+
+
+def test_newton_round_rejects_bad_arguments():
+    """efe_newton_round_batched: a context without specimens, a missing array, an unknown
+    precision and negative limits are refused before any device work; the context stays usable."""
+    from edelweissfe_amd import _lib
+    from edelweissfe_amd.device import DeviceSystem
+
+    lib = _lib.load()
+    pm = device_model((2, 2, 2), (1.0, 1.0, 1.0), "C3D8", "linearelastic", [1.0, 0.3])
+    dev = torch.device("cuda")
+    S = 2
+    spec = np.r_[np.zeros(pm.nNodes, np.int32), np.ones(pm.nNodes, np.int32)]
+    coords = np.vstack([pm.coordinates, pm.coordinates + 2.0])
+    conn = np.vstack([pm.connectivity, pm.connectivity + pm.nNodes])
+    batch = DeviceSystem(coords, conn, "C3D8", "linearelastic", [1.0, 0.3], specimen_of_node=spec, n_specimens=S)
+    single = DeviceSystem(pm.coordinates, pm.connectivity, "C3D8", "linearelastic", [1.0, 0.3])
+    o = _lib.NewtonOpts(precision=_lib.EFE_FP64, krylov_max_iter=100, check_every=16, alt_after=15, rtol=1e-10,
+                        rtol_first=1e-10, atol_factor=1e-2, flux_tol=1e-8, flux_tol_alt=5e-3, correction_tol=1e-8,
+                        flux_floor=1e-7, avg_floor=1e-10)
+    u8 = np.ones(S, np.uint8)
+    i32 = np.full(S, 5, np.int32)
+    na = np.ones(S)
+    outs = [np.zeros(S, np.int32) for _ in range(3)] + [np.zeros((S, 3), np.int32)]
+
+    def call(sysm, dU, live=u8, maxit=i32, nacc=na, opts=o):
+        dT = torch.zeros_like(dU)
+        return lib.efe_newton_round_batched(
+            sysm.ctx, ctypes.c_void_p(dU.data_ptr()), None, None, None, ctypes.c_void_p(dT.data_ptr()),
+            live.ctypes.data if live is not None else None, u8.ctypes.data, u8.ctypes.data, maxit.ctypes.data,
+            i32.ctypes.data, nacc.ctypes.data, ctypes.byref(opts), *[a.ctypes.data for a in outs])
+
+    dU1 = torch.zeros(single.n_dof, dtype=torch.float64, device=dev)
+    assert call(single, dU1) == _lib.EFE_ERR_STATE and "specimens" in _err(lib)
+    dU = torch.zeros(batch.n_dof, dtype=torch.float64, device=dev)
+    assert call(batch, dU, live=None) == _lib.EFE_ERR_ARG
+    bad = _lib.NewtonOpts(precision=7)
+    assert call(batch, dU, opts=bad) == _lib.EFE_ERR_ARG
+    assert call(batch, dU, maxit=np.array([5, -1], np.int32)) == _lib.EFE_ERR_ARG and "max_iter" in _err(lib)
+    assert call(batch, dU, nacc=np.array([1.0, 0.0])) == _lib.EFE_ERR_ARG
+    # still usable: a round with no live specimen takes one pass and reports every specimen idle
+    assert call(batch, dU, live=np.zeros(S, np.uint8)) == _lib.EFE_OK
+    assert (outs[0] == _lib.EFE_NEWTON_IDLE).all() and (outs[3] == 0).all()
+    batch.close()
+    single.close()
