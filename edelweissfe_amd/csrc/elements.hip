@@ -56,16 +56,6 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
                                                      const int* __restrict__ specActive, int formK,
                                                      const int32_t* __restrict__ elist, int ecount, int ncw) {
     constexpr int NS = 12 + MatState<MAT>::n;
-    const int e0 = blockIdx.x * EPW;
-    // Element el of this workgroup: entry e0 + el of the group's element list (elist, or all
-    // elements in order when it is null); -1 past the end and, in batched contexts, for
-    // specimens the pass skips.
-    auto elem = [&](int el) {
-        const int i = e0 + el;
-        if (i >= ecount) return -1;
-        const int e = elist ? elist[i] : i;
-        return (specActive && !specActive[specOfElem[e]]) ? -1 : e;
-    };
     constexpr int NP = (MAT == EFE_LINEAR_ELASTIC) ? 2 : 6;
     // the tangent of every QP in compact form, w detJ [A, B, c_nn, (c_n1, c_1n)] then n:
     // C = A 1(x)1 + B Idev + c_nn n(x)n (+ c_n1 n(x)1 + c_1n 1(x)n for Drucker-Prager)
@@ -75,6 +65,18 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
     __shared__ double fs_s[EPW][Q][6];
     const int lane = threadIdx.x;
     const int64_t EQ = (int64_t)E * Q;
+    // Grid-stride over the element list: a grid smaller than the list (a persistent launch)
+    // takes EPW elements per workgroup and round.
+    for (int e0 = blockIdx.x * EPW; e0 < ecount; e0 += gridDim.x * EPW) {
+    // Element el of this round: entry e0 + el of the group's element list (elist, or all
+    // elements in order when it is null); -1 past the end and, in batched contexts, for
+    // specimens the pass skips.
+    auto elem = [&](int el) {
+        const int i = e0 + el;
+        if (i >= ecount) return -1;
+        const int e = elist ? elist[i] : i;
+        return (specActive && !specActive[specOfElem[e]]) ? -1 : e;
+    };
 
     for (int t = lane; t < EPW * Q; t += 64) {
         const int el = t / Q, q = t - (t / Q) * Q;
@@ -264,6 +266,8 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
         dst[0] = -p0;
         dst[1] = -p1;
         dst[2] = -p2;
+    }
+    __syncthreads();  // the LDS tiles are refilled by the next round
     }
 }
 
@@ -724,8 +728,10 @@ __global__ void export_csr_kernel(int N, const int32_t* __restrict__ bptr, const
 
 template <int NN, int Q, int MAT, typename KT, int EPW>
 void launch_elem_t(efe_ctx* c, const double* dU) {
-    const int grid = (c->curCount + EPW - 1) / EPW;
+    static const int cap = std::getenv("EFE_ELEMENT_GRID_CAP") ? std::atoi(std::getenv("EFE_ELEMENT_GRID_CAP")) : 0;
+    int grid = (c->curCount + EPW - 1) / EPW;
     if (grid == 0) return;
+    if (cap > 0 && grid > cap) grid = cap;  // persistent: the workgroups loop over the list
     const int propStride = c->props_per_element ? c->E : 0;
     const int* act = c->useActive ? c->specActive : nullptr;
     if (c->mfPass)
