@@ -227,7 +227,7 @@ int efe_ctx_create(const efe_mesh_desc* d, efe_ctx** out) {
         c->groupMat[0] = c->material;
         c->groupCount[0] = c->E;
     }
-    c->mfNC = (mat_of ? mat_count[EFE_DRUCKER_PRAGER] > 0 : d->material == EFE_DRUCKER_PRAGER) ? 11 : 9;
+    c->mfNC = (mat_of ? mat_count[EFE_DRUCKER_PRAGER] > 0 : d->material == EFE_DRUCKER_PRAGER) ? 10 : 8;
     c->stream = (hipStream_t)d->stream;
     c->nspec = d->specimen_of_node ? d->n_specimens : 1;
     std::string err;

@@ -132,8 +132,8 @@ struct efe_ctx {
                                  // (force slot of the matrix-free operator; block-row (e, a) of the
                                  // element scratch starts at block elemPos * NN)
     bool mfPass = false;             // the current element launch is a matrix-free pass
-    int mfNC = 9;                    // compact tangent values per QP of the matrix-free operator
-                                     // (11 when a Drucker-Prager section is present)
+    int mfNC = 8;                    // compact tangent values per QP of the matrix-free operator
+                                     // (10 when a Drucker-Prager section is present)
     // material groups (sections): the element kernel runs once per group over its element list
     int ngroups = 1;
     int groupMat[3] = {0, 0, 0};

@@ -147,12 +147,14 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
             *status = 1;
             if (specStatus) specStatus[specOfElem[e]] = 1;  // which specimen asked (batched contexts)
         }
-        if (MF && formK) {  // ncw = 9 [A, B, cnn, n] or 11 [A, B, cnn, cn1, c1n, n] (cn1 = c1n = 0 but DP)
+        if (MF && formK) {  // ncw = 8 [A, B, cnn, n] or 10 [A, B, cnn, cn1, c1n, n] (cn1 = c1n = 0 but DP);
+            // n is deviatoric (n_33 = -n_11 - n_22): its five other components are stored
             const double fq = det * qpw[q];
             float* dc = mfC + (int64_t)q * ncw * E + e;
-            for (int k = 0; k < ncw - 6; ++k) dc[(int64_t)k * E] = (float)(fq * cc[k]);
+            for (int k = 0; k < ncw - 5; ++k) dc[(int64_t)k * E] = (float)(fq * cc[k]);
+            const int nk[5] = {0, 1, 3, 4, 5};
 #pragma unroll
-            for (int k = 0; k < 6; ++k) dc[(int64_t)(ncw - 6 + k) * E] = (float)cc[5 + k];
+            for (int k = 0; k < 5; ++k) dc[(int64_t)(ncw - 5 + k) * E] = (float)cc[5 + nk[k]];
         }
 #pragma unroll
         for (int c = 0; c < 6; ++c) {

@@ -179,8 +179,9 @@ __global__ __launch_bounds__(256) void mf_apply_kernel(int E, int N, const int32
 #pragma unroll
         for (int k = 0; k < NC; ++k) cf[k] = __builtin_nontemporal_load(mfC + (int64_t)(q * NC + k) * E + e);
         const float A = cf[0], B = cf[1], cnn = cf[2];
-        const float cn1 = NC == 11 ? cf[3] : 0.0f, c1n = NC == 11 ? cf[4] : 0.0f;
-        const float* n = cf + (NC == 11 ? 5 : 3);
+        const float cn1 = NC == 10 ? cf[3] : 0.0f, c1n = NC == 10 ? cf[4] : 0.0f;
+        const float* nc = cf + (NC == 10 ? 5 : 3);
+        const float n[6] = {nc[0], nc[1], -(nc[0] + nc[1]), nc[2], nc[3], nc[4]};  // n deviatoric
         const float tr = ep[0] + ep[1] + ep[2];
         const float ne = n[0] * ep[0] + n[1] * ep[1] + n[2] * ep[2] + n[3] * ep[3] + n[4] * ep[4] + n[5] * ep[5];
         const float t3 = tr * (1.0f / 3.0f);
@@ -346,8 +347,9 @@ __global__ __launch_bounds__(256) void mf_apply_hex8_kernel(int E, int N, const 
 #pragma unroll
         for (int k = 0; k < NC; ++k) cf[k] = __builtin_nontemporal_load(mfC + (int64_t)(q * NC + k) * E + e);
         const float A = cf[0], B = cf[1], cnn = cf[2];
-        const float cn1 = NC == 11 ? cf[3] : 0.0f, c1n = NC == 11 ? cf[4] : 0.0f;
-        const float* n = cf + (NC == 11 ? 5 : 3);
+        const float cn1 = NC == 10 ? cf[3] : 0.0f, c1n = NC == 10 ? cf[4] : 0.0f;
+        const float* nc = cf + (NC == 10 ? 5 : 3);
+        const float n[6] = {nc[0], nc[1], -(nc[0] + nc[1]), nc[2], nc[3], nc[4]};  // n deviatoric
         const float tr = ep[0] + ep[1] + ep[2];
         const float ne = n[0] * ep[0] + n[1] * ep[1] + n[2] * ep[2] + n[3] * ep[3] + n[4] * ep[4] + n[5] * ep[5];
         const float t3 = tr * (1.0f / 3.0f);
@@ -504,8 +506,9 @@ __global__ __launch_bounds__(kBrick) void mfb_apply_hex8_kernel(
 #pragma unroll
             for (int k = 0; k < NC; ++k) cf[k] = __builtin_nontemporal_load(mfC + (int64_t)(q * NC + k) * E + e);
             const float A = cf[0], B = cf[1], cnn = cf[2];
-            const float cn1 = NC == 11 ? cf[3] : 0.0f, c1n = NC == 11 ? cf[4] : 0.0f;
-            const float* n = cf + (NC == 11 ? 5 : 3);
+            const float cn1 = NC == 10 ? cf[3] : 0.0f, c1n = NC == 10 ? cf[4] : 0.0f;
+            const float* nc = cf + (NC == 10 ? 5 : 3);
+            const float n[6] = {nc[0], nc[1], -(nc[0] + nc[1]), nc[2], nc[3], nc[4]};  // n deviatoric
             const float tr = ep[0] + ep[1] + ep[2];
             const float ne = n[0] * ep[0] + n[1] * ep[1] + n[2] * ep[2] + n[3] * ep[3] + n[4] * ep[4] + n[5] * ep[5];
             const float t3 = tr * (1.0f / 3.0f);
@@ -697,28 +700,28 @@ template <typename TP>
 void launch_apply(efe_ctx* c, hipStream_t st, const TP* p, int mask, const int* flags,
                   const int* active = nullptr) {
     const int grid = round8_mf((c->E + 255) / 256);
-    const bool dp = c->mfNC == 11;
+    const bool dp = c->mfNC == 10;
     if (c->mfb && !active) {  // bricks (hexa8); grid: exactly the workgroups, a multiple of 8
 #define MFB_ARGS                                                                                                 \
     c->E, c->N, c->mfbLptr, c->mfbLnode, c->mfbLpos, c->mfbLincp, c->mfbLinc, c->mfbLidx, c->coords, c->mfC, p, \
         c->dirflag, mask, flags, c->mfbF
-        if (dp) mfb_apply_hex8_kernel<11, TP><<<grid, kBrick, 0, st>>>(MFB_ARGS);
-        else mfb_apply_hex8_kernel<9, TP><<<grid, kBrick, 0, st>>>(MFB_ARGS);
+        if (dp) mfb_apply_hex8_kernel<10, TP><<<grid, kBrick, 0, st>>>(MFB_ARGS);
+        else mfb_apply_hex8_kernel<8, TP><<<grid, kBrick, 0, st>>>(MFB_ARGS);
 #undef MFB_ARGS
         return;
     }
     switch (c->eltype) {
         case EFE_C3D8:
-            if (dp) mf_apply_hex8_kernel<11, TP><<<grid, 256, 0, st>>>(MF_ARGS);
-            else mf_apply_hex8_kernel<9, TP><<<grid, 256, 0, st>>>(MF_ARGS);
+            if (dp) mf_apply_hex8_kernel<10, TP><<<grid, 256, 0, st>>>(MF_ARGS);
+            else mf_apply_hex8_kernel<8, TP><<<grid, 256, 0, st>>>(MF_ARGS);
             break;
         case EFE_C3D20:
-            if (dp) mf_apply_kernel<20, 27, 11, TP><<<grid, 256, 0, st>>>(MF_ARGS);
-            else mf_apply_kernel<20, 27, 9, TP><<<grid, 256, 0, st>>>(MF_ARGS);
+            if (dp) mf_apply_kernel<20, 27, 10, TP><<<grid, 256, 0, st>>>(MF_ARGS);
+            else mf_apply_kernel<20, 27, 8, TP><<<grid, 256, 0, st>>>(MF_ARGS);
             break;
         default:
-            if (dp) mf_apply_kernel<20, 8, 11, TP><<<grid, 256, 0, st>>>(MF_ARGS);
-            else mf_apply_kernel<20, 8, 9, TP><<<grid, 256, 0, st>>>(MF_ARGS);
+            if (dp) mf_apply_kernel<20, 8, 10, TP><<<grid, 256, 0, st>>>(MF_ARGS);
+            else mf_apply_kernel<20, 8, 8, TP><<<grid, 256, 0, st>>>(MF_ARGS);
             break;
     }
 }
