@@ -508,6 +508,11 @@ __global__ __launch_bounds__(BS) void gm_spmv_kernel(int N, const int32_t* __res
     const int32_t* cp = scol + o0 + l;
     const T* kp = K + (int64_t)o0 * 9 + l;
     T y0 = 0, y1 = 0, y2 = 0;
+    // unrolled so that the loads of several blocks are in flight together; measured on the 48^3
+    // hexa20 operator: FP64 455 / 438 / 491 / 452 us at 1 / 2 / 3 / 4 blocks, FP32 243 / 214 /
+    // 209 / 210 us
+    constexpr int kUnroll = sizeof(T) == 8 ? 2 : 3;
+#pragma unroll kUnroll
     for (int k = 0; k < deg; ++k) {
         const int m = __builtin_nontemporal_load(cp + k * kSlice);
         const T x0 = x[3 * m], x1 = x[3 * m + 1], x2 = x[3 * m + 2];
