@@ -125,6 +125,8 @@ struct efe_ctx {
     // per element its 8 local node indices (lidx); per node its partial range (pptr [N+1]) in
     // the partial forces mfbF [L][3]
     bool mfb = false;
+    int mfbG = 0;                // workgroups (bricks); in batched contexts a brick holds one specimen's elements
+    int32_t* mfbE0 = nullptr;    // [G+1] first element of each brick
     int32_t *mfbLptr = nullptr, *mfbLnode = nullptr, *mfbLpos = nullptr, *mfbLincp = nullptr, *mfbPptr = nullptr;
     uint16_t *mfbLinc = nullptr, *mfbLidx = nullptr;
     float* mfbF = nullptr;

@@ -408,7 +408,8 @@ __global__ __launch_bounds__(kBrick) void mfb_apply_hex8_kernel(
     const int32_t* __restrict__ lpos, const int32_t* __restrict__ lincp, const uint16_t* __restrict__ linc,
     const uint16_t* __restrict__ lidx, const double* __restrict__ coords, const float* __restrict__ mfC,
     const TP* __restrict__ p, const uint8_t* __restrict__ dirflag, int mask, const int* __restrict__ flags,
-    float* __restrict__ Fpart) {
+    float* __restrict__ Fpart, int G, const int32_t* __restrict__ e0s, const int* __restrict__ active,
+    const int32_t* __restrict__ specOfElem) {
     // LDS, reused: first the staged node data (p 3 x 768 floats, x 3 x 768 doubles), then the
     // element forces (256 x 8 x 3 floats)
     __shared__ double lds[3 * kBrickNodes + (3 * kBrickNodes + 1) / 2];
@@ -417,7 +418,11 @@ __global__ __launch_bounds__(kBrick) void mfb_apply_hex8_kernel(
     float* sf = (float*)lds;
     if (flags && flags[0]) return;
     const int g = xcd_block_mf();
-    if (g * kBrick >= E) return;  // the grid is rounded up to a multiple of 8
+    if (g >= G) return;  // the grid is rounded up to a multiple of 8
+    const int eb = e0s[g], ee = e0s[g + 1];
+    // batched solves: a brick of a frozen specimen leaves its partials alone (the gather skips
+    // that specimen's nodes)
+    if (active && !active[specOfElem[eb]]) return;
     const int t = threadIdx.x;
     const int l0 = lptr[g], nl = lptr[g + 1] - l0;
     for (int i = t; i < nl; i += kBrick) {
@@ -431,8 +436,8 @@ __global__ __launch_bounds__(kBrick) void mfb_apply_hex8_kernel(
         }
     }
     __syncthreads();
-    const int e = g * kBrick + t;
-    const bool live = e < E;
+    const int e = eb + t;
+    const bool live = e < ee;
     float cp[3][8], cx[3][8];
     if (live) {
         // 8 local indices in one 16-byte load
@@ -701,12 +706,13 @@ void launch_apply(efe_ctx* c, hipStream_t st, const TP* p, int mask, const int* 
                   const int* active = nullptr) {
     const int grid = round8_mf((c->E + 255) / 256);
     const bool dp = c->mfNC == 10;
-    if (c->mfb && !active) {  // bricks (hexa8); grid: exactly the workgroups, a multiple of 8
+    if (c->mfb) {  // bricks (hexa8); grid: the bricks, rounded up to a multiple of 8
+        const int gb = round8_mf(c->mfbG);
 #define MFB_ARGS                                                                                                 \
     c->E, c->N, c->mfbLptr, c->mfbLnode, c->mfbLpos, c->mfbLincp, c->mfbLinc, c->mfbLidx, c->coords, c->mfC, p, \
-        c->dirflag, mask, flags, c->mfbF
-        if (dp) mfb_apply_hex8_kernel<10, TP><<<grid, kBrick, 0, st>>>(MFB_ARGS);
-        else mfb_apply_hex8_kernel<8, TP><<<grid, kBrick, 0, st>>>(MFB_ARGS);
+        c->dirflag, mask, flags, c->mfbF, c->mfbG, c->mfbE0, active, c->specOfElem
+        if (dp) mfb_apply_hex8_kernel<10, TP><<<gb, kBrick, 0, st>>>(MFB_ARGS);
+        else mfb_apply_hex8_kernel<8, TP><<<gb, kBrick, 0, st>>>(MFB_ARGS);
 #undef MFB_ARGS
         return;
     }
@@ -727,25 +733,41 @@ void launch_apply(efe_ctx* c, hipStream_t st, const TP* p, int mask, const int* 
 }
 
 // the per-node force list the gather sums: brick partials, or one slot per (element, node)
-inline const int32_t* gather_ptr(const efe_ctx* c, bool batched) { return (c->mfb && !batched) ? c->mfbPptr : c->nptr; }
-inline const float* gather_forces(const efe_ctx* c, bool batched) { return (c->mfb && !batched) ? c->mfbF : c->mfF; }
+inline const int32_t* gather_ptr(const efe_ctx* c) { return c->mfb ? c->mfbPptr : c->nptr; }
+inline const float* gather_forces(const efe_ctx* c) { return c->mfb ? c->mfbF : c->mfF; }
 
 template <int BS>
 void launch_gather_op(efe_ctx* c, hipStream_t st, const float* p, float* q, double* part, const int* flags,
                       int nwg, int kr) {
-    mf_gather_kernel<BS, 0, float, float><<<nwg, BS, 0, st>>>(c->N, gather_ptr(c, false), c->nlist,
-                                                              gather_forces(c, false), p, c->dirflag, q, part,
+    mf_gather_kernel<BS, 0, float, float><<<nwg, BS, 0, st>>>(c->N, gather_ptr(c), c->nlist,
+                                                              gather_forces(c), p, c->dirflag, q, part,
                                                               flags, kr);
 }
 
 }  // namespace
 
 namespace {
-// Brick tables of the hexa8 operator (single-system contexts); left off when a workgroup's
-// elements touch more than kBrickNodes nodes (a mesh in an order without locality).
+// Brick tables of the hexa8 operator; left off when a workgroup's elements touch more than
+// kBrickNodes nodes (a mesh in an order without locality).  Bricks are runs of up to kBrick
+// consecutive elements; in batched contexts every specimen's elements start a new brick, so
+// that a specimen's partial sums are grouped the same alone and in any batch (its elements
+// must then be one contiguous run per specimen, else the per-slot operator stays).
 int mfb_build(efe_ctx* c) {
-    if (c->eltype != EFE_C3D8 || c->specOfNode || c->mfb) return EFE_OK;
-    const int E = c->E, N = c->N, G = (E + kBrick - 1) / kBrick;
+    if (c->eltype != EFE_C3D8 || c->mfb) return EFE_OK;
+    const int E = c->E, N = c->N;
+    std::vector<int32_t> e0s(1, 0);
+    if (c->specOfNode) {
+        std::vector<int32_t> se(E);
+        EFE_HIP(hipMemcpy(se.data(), c->specOfElem, (size_t)E * sizeof(int32_t), hipMemcpyDeviceToHost));
+        for (int e = 1; e < E; ++e) {
+            if (se[e] < se[e - 1]) return EFE_OK;  // specimens' elements not in runs
+            if (se[e] != se[e - 1] || e - e0s.back() == kBrick) e0s.push_back(e);
+        }
+    } else {
+        for (int e = kBrick; e < E; e += kBrick) e0s.push_back(e);
+    }
+    e0s.push_back(E);
+    const int G = (int)e0s.size() - 1;
     std::vector<int32_t> conn((size_t)8 * E);  // [a][e]
     EFE_HIP(hipMemcpy(conn.data(), c->conn, conn.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
     std::vector<int32_t> lptr(G + 1, 0), lnode, lincp, pcount(N, 0);
@@ -755,7 +777,7 @@ int mfb_build(efe_ctx* c) {
     std::vector<int32_t> mark(N, -1);
     std::vector<int32_t> nodes, cnt, fill;
     for (int g = 0; g < G; ++g) {
-        const int e0 = g * kBrick, e1 = std::min(E, e0 + kBrick);
+        const int e0 = e0s[g], e1 = e0s[g + 1];
         nodes.clear();
         for (int e = e0; e < e1; ++e)
             for (int a = 0; a < 8; ++a) nodes.push_back(conn[(size_t)a * E + e]);
@@ -772,7 +794,7 @@ int mfb_build(efe_ctx* c) {
                 ++cnt[l + 1];
             }
         for (int i = 0; i < nl; ++i) cnt[i + 1] += cnt[i];
-        const int base = g * kBrick * 8;  // linc entries of workgroup g start at its first element's
+        const int base = e0 * 8;  // linc entries of workgroup g start at its first element's
         fill.assign(cnt.begin(), cnt.end() - 1);
         for (int e = e0; e < e1; ++e)  // element order inside every local node's list
             for (int a = 0; a < 8; ++a) {
@@ -805,9 +827,10 @@ int mfb_build(efe_ctx* c) {
     int rc;
     if ((rc = up(&c->mfbLptr, lptr)) || (rc = up(&c->mfbLnode, lnode)) || (rc = up(&c->mfbLpos, lpos)) ||
         (rc = up(&c->mfbLincp, lincp)) || (rc = up(&c->mfbPptr, pptr)) || (rc = up(&c->mfbLinc, linc)) ||
-        (rc = up(&c->mfbLidx, lidx)))
+        (rc = up(&c->mfbLidx, lidx)) || (rc = up(&c->mfbE0, e0s)))
         return rc;
     EFE_HIP(hipMalloc(&c->mfbF, (size_t)L * 3 * sizeof(float)));
+    c->mfbG = G;
     c->mfb = true;
     return EFE_OK;
 }
@@ -844,7 +867,7 @@ int efe_launch_mf_assembly(efe_ctx* c, double* P, double* F, const double* dir_v
     if (dir_values) {
         launch_apply<double>(c, c->stream, dir_values, 0, nullptr);
         mf_gather_kernel<256, 1, double, double><<<round8_mf(grid), 256, 0, c->stream>>>(
-            c->N, gather_ptr(c, false), c->nlist, gather_forces(c, false), dir_values, c->dirflag, dir_correction,
+            c->N, gather_ptr(c), c->nlist, gather_forces(c), dir_values, c->dirflag, dir_correction,
             nullptr, nullptr, 1);
         EFE_HIP(hipGetLastError());
     }
@@ -870,7 +893,7 @@ void efe_mf_launch_operator(efe_ctx* c, hipStream_t st, const float* p, float* q
 int efe_launch_spmv_mf(efe_ctx* c, const double* x, double* y) {
     launch_apply<double>(c, c->stream, x, 1, nullptr);
     mf_gather_kernel<256, 0, double, double><<<round8_mf((c->N + 255) / 256), 256, 0, c->stream>>>(
-        c->N, gather_ptr(c, false), c->nlist, gather_forces(c, false), x, c->dirflag, y, nullptr, nullptr, 1);
+        c->N, gather_ptr(c), c->nlist, gather_forces(c), x, c->dirflag, y, nullptr, nullptr, 1);
     EFE_HIP(hipGetLastError());
     return EFE_OK;
 }
@@ -879,8 +902,8 @@ int efe_launch_spmv_mf(efe_ctx* c, const double* x, double* y) {
 void efe_mf_launch_operator_batched(efe_ctx* c, hipStream_t st, const float* p, float* q, double* npq,
                                     const int* active) {
     launch_apply<float>(c, st, p, 1, nullptr, active);
-    mf_gather_batched_kernel<<<(c->N + 255) / 256, 256, 0, st>>>(c->N, c->nptr, c->mfF, p, c->dirflag, c->specOfNode,
-                                                                 active, q, npq);
+    mf_gather_batched_kernel<<<(c->N + 255) / 256, 256, 0, st>>>(c->N, gather_ptr(c), gather_forces(c), p, c->dirflag,
+                                                                 c->specOfNode, active, q, npq);
 }
 
 void efe_mf_free(efe_ctx* c) {
@@ -888,10 +911,11 @@ void efe_mf_free(efe_ctx* c) {
     hipFree(c->mfF);
     hipFree(c->dnxi32);
     c->mfC = c->mfF = c->dnxi32 = nullptr;
-    void* b[] = {c->mfbLptr, c->mfbLnode, c->mfbLpos, c->mfbLincp, c->mfbPptr, c->mfbLinc, c->mfbLidx, c->mfbF};
+    void* b[] = {c->mfbLptr, c->mfbLnode, c->mfbLpos, c->mfbLincp, c->mfbPptr, c->mfbLinc, c->mfbLidx, c->mfbF,
+                 c->mfbE0};
     for (void* q : b)
         if (q) hipFree(q);
-    c->mfbLptr = c->mfbLnode = c->mfbLpos = c->mfbLincp = c->mfbPptr = nullptr;
+    c->mfbLptr = c->mfbLnode = c->mfbLpos = c->mfbLincp = c->mfbPptr = c->mfbE0 = nullptr;
     c->mfbLinc = c->mfbLidx = nullptr;
     c->mfbF = nullptr;
     c->mfb = false;
