@@ -63,6 +63,11 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
     __shared__ double g_s[EPW][Q * NN * 3];
     __shared__ KT fC_s[EPW][Q * NCC];
     __shared__ double fs_s[EPW][Q][6];
+    // assembled passes: blocks staged for word-by-word stores, and their positions (and those
+    // of their transposes, -1: none)
+    constexpr int kStage = 32;
+    __shared__ KT stg_s[MF ? 1 : kStage][9];
+    __shared__ int spos_s[MF ? 1 : 2 * kStage];
     const int lane = threadIdx.x;
     const int64_t EQ = (int64_t)E * Q;
     // Grid-stride over the element list: a grid smaller than the list (a persistent launch)
@@ -182,15 +187,21 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
     // a residual-only pass (formK = 0) leaves the tangent blocks alone
     // Lanes take an element's blocks one after the other.  (Taking the EPW elements of one
     // block position side by side, so that the stores form runs of adjacent blocks, was
-    // measured 8 % slower at 64^3: the tangent reads lose their broadcast.)
-    for (int t = formK ? lane : EPW * NB; t < EPW * NB; t += 64) {
+    // measured 8 % slower at 64^3: the tangent reads lose their broadcast.)  The assembled
+    // passes write their blocks through LDS, kStage blocks (and their transposes) at a time:
+    // the wavefront then stores consecutive words, about seven 72-byte blocks per store
+    // instruction instead of one scattered block per lane.  Measured on the 64^3 hexa8 J2
+    // tangent pass: 1.00 ms with per-lane stores, 0.95 / 0.91 / 0.88 / 0.95 ms staging 8 / 16 /
+    // 32 / 64 blocks (0.53 ms with no block stores at all); 48^3 hexa20R DP: 2.34 -> 2.27 ms.
+    const int ntask = formK ? EPW * NB : 0;
+    for (int t0 = 0; t0 < ntask; t0 += 64) {
+        const int t = t0 + lane;
         const int el = t / NB, r0 = t - (t / NB) * NB;
-        int a, b;
+        int a = 0, b = 0;
         if (MF) {
             a = b = r0;
         } else if (SYM) {  // r0 -> (a, b), upper triangle row by row
             int r = r0;
-            a = 0;
             while (r >= NN - a) {
                 r -= NN - a;
                 ++a;
@@ -200,55 +211,81 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
             a = r0 / NN;
             b = r0 - a * NN;
         }
-        const int e = elem(el);
-        if (e < 0) continue;
+        const int e = t < ntask ? elem(el) : -1;
         KT acc[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) acc[k] = (KT)0;
-        for (int q = 0; q < Q; ++q) {
-            const KT ax = (KT)g_s[el][(q * NN + a) * 3 + 0], ay = (KT)g_s[el][(q * NN + a) * 3 + 1], az = (KT)g_s[el][(q * NN + a) * 3 + 2];
-            const KT bx = (KT)g_s[el][(q * NN + b) * 3 + 0], by = (KT)g_s[el][(q * NN + b) * 3 + 1], bz = (KT)g_s[el][(q * NN + b) * 3 + 2];
-            // CB[:, j] = C B_b e_j from the compact tangent: the strain of column j has the
-            // normal component b_j and two engineering shears (12/13, 12/23, 13/23)
-            const KT* cq = fC_s[el] + q * NCC;
-            const KT cA = cq[0], cB = cq[1], cnn = cq[2];
-            const KT cn1 = NCC == 11 ? cq[3] : (KT)0, c1n = NCC == 11 ? cq[4] : (KT)0;
-            const KT* n = cq + NCC - 6;
-            const KT bv[3] = {bx, by, bz};
-            KT CB[6][3];
+        if (e >= 0) {
+            for (int q = 0; q < Q; ++q) {
+                const KT ax = (KT)g_s[el][(q * NN + a) * 3 + 0], ay = (KT)g_s[el][(q * NN + a) * 3 + 1], az = (KT)g_s[el][(q * NN + a) * 3 + 2];
+                const KT bx = (KT)g_s[el][(q * NN + b) * 3 + 0], by = (KT)g_s[el][(q * NN + b) * 3 + 1], bz = (KT)g_s[el][(q * NN + b) * 3 + 2];
+                // CB[:, j] = C B_b e_j from the compact tangent: the strain of column j has the
+                // normal component b_j and two engineering shears (12/13, 12/23, 13/23)
+                const KT* cq = fC_s[el] + q * NCC;
+                const KT cA = cq[0], cB = cq[1], cnn = cq[2];
+                const KT cn1 = NCC == 11 ? cq[3] : (KT)0, c1n = NCC == 11 ? cq[4] : (KT)0;
+                const KT* n = cq + NCC - 6;
+                const KT bv[3] = {bx, by, bz};
+                KT CB[6][3];
 #pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                KT v[6] = {0, 0, 0, 0, 0, 0};
-                v[j] = bv[j];
-                if (j == 0) { v[3] = by; v[4] = bz; }
-                if (j == 1) { v[3] = bx; v[5] = bz; }
-                if (j == 2) { v[4] = bx; v[5] = by; }
-                const KT tr = bv[j];
-                KT ne = n[j] * bv[j];
+                for (int j = 0; j < 3; ++j) {
+                    KT v[6] = {0, 0, 0, 0, 0, 0};
+                    v[j] = bv[j];
+                    if (j == 0) { v[3] = by; v[4] = bz; }
+                    if (j == 1) { v[3] = bx; v[5] = bz; }
+                    if (j == 2) { v[4] = bx; v[5] = by; }
+                    const KT tr = bv[j];
+                    KT ne = n[j] * bv[j];
 #pragma unroll
-                for (int k = 3; k < 6; ++k) ne += n[k] * v[k];
-                const KT sn = cnn * ne + cn1 * tr, s1 = cA * tr + c1n * ne, t3 = tr * (KT)(1.0 / 3.0);
+                    for (int k = 3; k < 6; ++k) ne += n[k] * v[k];
+                    const KT sn = cnn * ne + cn1 * tr, s1 = cA * tr + c1n * ne, t3 = tr * (KT)(1.0 / 3.0);
 #pragma unroll
-                for (int i = 0; i < 3; ++i) CB[i][j] = s1 + cB * (v[i] - t3) + sn * n[i];
+                    for (int i = 0; i < 3; ++i) CB[i][j] = s1 + cB * (v[i] - t3) + sn * n[i];
 #pragma unroll
-                for (int k = 3; k < 6; ++k) CB[k][j] = (KT)0.5 * cB * v[k] + sn * n[k];
-            }
+                    for (int k = 3; k < 6; ++k) CB[k][j] = (KT)0.5 * cB * v[k] + sn * n[k];
+                }
 #pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                acc[0 + j] += ax * CB[0][j] + ay * CB[3][j] + az * CB[4][j];
-                acc[3 + j] += ay * CB[1][j] + ax * CB[3][j] + az * CB[5][j];
-                acc[6 + j] += az * CB[2][j] + ax * CB[4][j] + ay * CB[5][j];
+                for (int j = 0; j < 3; ++j) {
+                    acc[0 + j] += ax * CB[0][j] + ay * CB[3][j] + az * CB[4][j];
+                    acc[3 + j] += ay * CB[1][j] + ax * CB[3][j] + az * CB[5][j];
+                    acc[6 + j] += az * CB[2][j] + ax * CB[4][j] + ay * CB[5][j];
+                }
             }
         }
         if (MF) {  // K_aa element-major at the start of the scratch: [e][a][9]
-            store_block(Kscr + ((int64_t)e * NN + a) * kBlockPitch, acc);
+            if (e >= 0) store_block(Kscr + ((int64_t)e * NN + a) * kBlockPitch, acc);
             continue;
         }
-        const int32_t* cp = cpos + (int64_t)e * NN * NN;
-        store_block(Kscr + (int64_t)cp[a * NN + b] * kBlockPitch, acc);
-        if (SYM && a != b) {
-            const KT tr[9] = {acc[0], acc[3], acc[6], acc[1], acc[4], acc[7], acc[2], acc[5], acc[8]};
-            store_block(Kscr + (int64_t)cp[b * NN + a] * kBlockPitch, tr);
+        // the block at position cp[a][b] and, for a symmetric tangent with a != b, its
+        // transpose at cp[b][a]; staged kStage lanes at a time, then stored word by word
+        const int32_t* cp = cpos + (int64_t)(e >= 0 ? e : 0) * NN * NN;
+        const int p0 = e >= 0 ? cp[a * NN + b] : -1;
+        const int p1 = (e >= 0 && SYM && a != b) ? cp[b * NN + a] : -1;
+        const int nlive = ntask - t0 < 64 ? ntask - t0 : 64;
+        for (int s0 = 0; s0 < nlive; s0 += kStage) {
+            const int sl = lane - s0;
+            if (sl >= 0 && sl < kStage) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) stg_s[sl][k] = acc[k];
+                spos_s[sl] = p0;
+                spos_s[kStage + sl] = p1;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int nb = (nlive - s0 < kStage ? nlive - s0 : kStage);
+            for (int w = lane; w < (SYM ? 2 : 1) * kStage * 9; w += 64) {
+                const int bi = w / 9, v = w - bi * 9;
+                const int src = bi < kStage ? bi : bi - kStage;
+                const int pos = src < nb ? spos_s[bi] : -1;
+                if (pos >= 0) {
+                    const int vv = bi < kStage ? v : (v % 3) * 3 + v / 3;  // K_ba = K_ab^T
+                    Kscr[(int64_t)pos * kBlockPitch + v] = stg_s[src][vv];
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
 
