@@ -187,7 +187,8 @@ __global__ __launch_bounds__(64) void element_kernel(int E, int N, const double*
     // a residual-only pass (formK = 0) leaves the tangent blocks alone
     // Lanes take an element's blocks one after the other.  (Taking the EPW elements of one
     // block position side by side, so that the stores form runs of adjacent blocks, was
-    // measured 8 % slower at 64^3: the tangent reads lose their broadcast.)  The assembled
+    // measured 8 % slower at 64^3 with per-lane stores, as the tangent reads lose their
+    // broadcast, and no faster with the staged stores below.)  The assembled
     // passes write their blocks through LDS, kStage blocks (and their transposes) at a time:
     // the wavefront then stores consecutive words, about seven 72-byte blocks per store
     // instruction instead of one scattered block per lane.  Measured on the 64^3 hexa8 J2
