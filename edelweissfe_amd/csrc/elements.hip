@@ -36,7 +36,8 @@ struct MatState {
 };
 
 // MF (matrix-free form, KT = float): phase 1 also stores the compact tangent
-// w detJ [A, B, cnn, (cn1, c1n,) n] per QP for the matrix-free operator, and phase 2 forms
+// w detJ [A, B, cnn, (cn1, c1n,)] and the deviatoric n (five components) per QP for the
+// matrix-free operator, and phase 2 forms
 // only the diagonal blocks K_aa (for block Jacobi), stored element-major [e][a][9] at the
 // start of the element scratch.
 template <int NN, int Q, int MAT, typename KT, int EPW, bool MF>

@@ -1,7 +1,8 @@
 // Matrix-free FP32 operator for the mixed-precision form (EFE_FP32_MATFREE).
 //
 // The element pass stores, per element and QP, the compact tangent
-// w detJ [A, B, cnn, (cn1, c1n,) n0..n5] (9 floats; 11 for Drucker-Prager) of
+// w detJ [A, B, cnn, (cn1, c1n,)] and the deviatoric direction n as n_11, n_22, n_12, n_13,
+// n_23 (n_33 = -n_11 - n_22): 8 floats, 10 for Drucker-Prager, of
 // C = A 1(x)1 + B Idev + cnn n(x)n + cn1 n(x)1 + c1n 1(x)n, in [QP][field][element] order
 // so that consecutive lanes read consecutive elements.  y = K p is then, per element (one
 // thread):
@@ -17,7 +18,7 @@
 // rows return p (unit diagonal).  The same two kernels with the prescribed values as p and
 // no elimination give the right-hand-side correction K[:, dir] dir_values.
 //
-// Algorithmic bytes per application: 36..44 per element-QP pair (tangent), 4 per
+// Algorithmic bytes per application: 32..40 per element-QP pair (tangent), 4 per
 // element-node (connectivity) and 2 x 4 per element-node (slot index, force slot written and
 // read), plus the node vectors; against 9 x 4 + 4 bytes per 3x3 block of the assembled FP32
 // operator.  It also removes the FP32 assembly from every Newton iteration.
@@ -389,8 +390,9 @@ __global__ __launch_bounds__(256) void mf_apply_hex8_kernel(int E, int N, const 
     }
 }
 
-// Hexa8 product by workgroup bricks: workgroup g takes the 256 consecutive elements
-// 256 g .. 256 g + 255.  Its local nodes (the distinct nodes of those elements, ascending) are
+// Hexa8 product by workgroup bricks: workgroup g takes the run of up to 256 consecutive elements
+// e0s[g] .. e0s[g + 1] - 1 (every 256 elements; in batched contexts a run never crosses a
+// specimen boundary).  Its local nodes (the distinct nodes of those elements, ascending) are
 // staged once into LDS with coalesced loads: p (Dirichlet DOFs masked) and the FP64
 // coordinates.  Each thread takes its element's 8 nodes from LDS (local indices lidx), forms the
 // element forces exactly as mf_apply_hex8_kernel, and writes them to LDS; then each local node's
