@@ -106,6 +106,20 @@ __device__ __forceinline__ void gather_slots(int n0, int N, const int32_t* __res
 
 constexpr int kGatherLds = 3 * (kGatherChunk + kGatherChunk / 8 + 1);
 
+// gather_slots for short lists (brick partials, ~2.4 per node): each lane reads its node's
+// entries directly, in the same order, so the sums are bitwise those of gather_slots.
+__device__ __forceinline__ void gather_direct(int n, const int32_t* __restrict__ nptr, const float* __restrict__ Fe,
+                                              bool want, float (&y)[3]) {
+    y[0] = y[1] = y[2] = 0.0f;
+    if (!want) return;
+    const int b = nptr[n], e = nptr[n + 1];
+    for (int c = b; c < e; ++c) {
+        y[0] += Fe[3 * (int64_t)c];
+        y[1] += Fe[3 * (int64_t)c + 1];
+        y[2] += Fe[3 * (int64_t)c + 2];
+    }
+}
+
 // One thread per element: element force f_e = K_e p_e.  J is rebuilt from the node
 // coordinates relative to the element's first node (formed in FP64, so that FP32 keeps the
 // element's own scale), the compact tangent is read per QP, and f_a is written to the
@@ -571,6 +585,7 @@ __global__ __launch_bounds__(kBrick) void mfb_apply_hex8_kernel(
 
 // Batched solves: q = K p on the nodes of active specimens and the per-node p.q values
 // that the per-specimen reductions sum; nodes of frozen specimens are left alone (p.q 0).
+template <bool DIRECT>
 __global__ __launch_bounds__(256) void mf_gather_batched_kernel(int N, const int32_t* __restrict__ nptr,
                                                                 const float* __restrict__ Fe,
                                                                 const float* __restrict__ p,
@@ -578,12 +593,13 @@ __global__ __launch_bounds__(256) void mf_gather_batched_kernel(int N, const int
                                                                 const int32_t* __restrict__ spec,
                                                                 const int* __restrict__ active, float* __restrict__ q,
                                                                 double* __restrict__ npq) {
-    __shared__ float buf[kGatherLds];
+    __shared__ float buf[DIRECT ? 1 : kGatherLds];
     const int n0 = blockIdx.x * 256, n = n0 + threadIdx.x;
     const bool valid = n < N;
     const bool act = valid && active[spec[n]];
     float yv[3];
-    gather_slots<256>(n0, N, nptr, Fe, act, buf, yv);
+    if (DIRECT) gather_direct(n, nptr, Fe, act, yv);
+    else gather_slots<256>(n0, N, nptr, Fe, act, buf, yv);
     if (!valid) return;
     if (!act) {
         npq[n] = 0.0;
@@ -603,14 +619,14 @@ __global__ __launch_bounds__(256) void mf_gather_batched_kernel(int N, const int
 // MODE 0 (operator): y = sum of the node's element forces in element order on free DOFs,
 // y = p on Dirichlet DOFs, per-workgroup partials of p.y.  MODE 1 (correction): y = the
 // sum on free DOFs, 0 on Dirichlet DOFs.
-template <int BS, int MODE, typename TP, typename TY>
+template <int BS, int MODE, typename TP, typename TY, bool DIRECT = false>
 __global__ __launch_bounds__(BS) void mf_gather_kernel(int N, const int32_t* __restrict__ nptr,
                                                        const int32_t* __restrict__ nlist,
                                                        const float* __restrict__ Fe, const TP* __restrict__ p,
                                                        const uint8_t* __restrict__ dirflag,
                                                        TY* __restrict__ y, double* __restrict__ part,
                                                        const int* __restrict__ flags, int kr) {
-    __shared__ float buf[kGatherLds];
+    __shared__ float buf[DIRECT ? 1 : kGatherLds];
     __shared__ double sh[BS / 64];
     if (flags && flags[0]) return;
     const int blk = xcd_block_mf();
@@ -621,7 +637,8 @@ __global__ __launch_bounds__(BS) void mf_gather_kernel(int N, const int32_t* __r
         const int n = n0 + threadIdx.x;
         const bool valid = n < N;
         float yv[3];
-        gather_slots<BS>(n0, N, nptr, Fe, valid, buf, yv);
+        if (DIRECT) gather_direct(n, nptr, Fe, valid, yv);
+        else gather_slots<BS>(n0, N, nptr, Fe, valid, buf, yv);
         if (!valid) continue;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
@@ -741,9 +758,14 @@ inline const float* gather_forces(const efe_ctx* c) { return c->mfb ? c->mfbF : 
 template <int BS>
 void launch_gather_op(efe_ctx* c, hipStream_t st, const float* p, float* q, double* part, const int* flags,
                       int nwg, int kr) {
-    mf_gather_kernel<BS, 0, float, float><<<nwg, BS, 0, st>>>(c->N, gather_ptr(c), c->nlist,
-                                                              gather_forces(c), p, c->dirflag, q, part,
-                                                              flags, kr);
+    if (c->mfb)
+        mf_gather_kernel<BS, 0, float, float, true><<<nwg, BS, 0, st>>>(c->N, gather_ptr(c), c->nlist,
+                                                                        gather_forces(c), p, c->dirflag, q, part,
+                                                                        flags, kr);
+    else
+        mf_gather_kernel<BS, 0, float, float><<<nwg, BS, 0, st>>>(c->N, gather_ptr(c), c->nlist,
+                                                                  gather_forces(c), p, c->dirflag, q, part,
+                                                                  flags, kr);
 }
 
 }  // namespace
@@ -868,9 +890,14 @@ int efe_launch_mf_assembly(efe_ctx* c, double* P, double* F, const double* dir_v
     EFE_HIP(hipGetLastError());
     if (dir_values) {
         launch_apply<double>(c, c->stream, dir_values, 0, nullptr);
-        mf_gather_kernel<256, 1, double, double><<<round8_mf(grid), 256, 0, c->stream>>>(
-            c->N, gather_ptr(c), c->nlist, gather_forces(c), dir_values, c->dirflag, dir_correction,
-            nullptr, nullptr, 1);
+        if (c->mfb)
+            mf_gather_kernel<256, 1, double, double, true><<<round8_mf(grid), 256, 0, c->stream>>>(
+                c->N, gather_ptr(c), c->nlist, gather_forces(c), dir_values, c->dirflag, dir_correction,
+                nullptr, nullptr, 1);
+        else
+            mf_gather_kernel<256, 1, double, double><<<round8_mf(grid), 256, 0, c->stream>>>(
+                c->N, gather_ptr(c), c->nlist, gather_forces(c), dir_values, c->dirflag, dir_correction,
+                nullptr, nullptr, 1);
         EFE_HIP(hipGetLastError());
     }
     c->Kprec = EFE_FP32_MATFREE;
@@ -894,8 +921,12 @@ void efe_mf_launch_operator(efe_ctx* c, hipStream_t st, const float* p, float* q
 // y = K x for FP64 vectors with the matrix-free FP32 operator (Dirichlet eliminated).
 int efe_launch_spmv_mf(efe_ctx* c, const double* x, double* y) {
     launch_apply<double>(c, c->stream, x, 1, nullptr);
-    mf_gather_kernel<256, 0, double, double><<<round8_mf((c->N + 255) / 256), 256, 0, c->stream>>>(
-        c->N, gather_ptr(c), c->nlist, gather_forces(c), x, c->dirflag, y, nullptr, nullptr, 1);
+    if (c->mfb)
+        mf_gather_kernel<256, 0, double, double, true><<<round8_mf((c->N + 255) / 256), 256, 0, c->stream>>>(
+            c->N, gather_ptr(c), c->nlist, gather_forces(c), x, c->dirflag, y, nullptr, nullptr, 1);
+    else
+        mf_gather_kernel<256, 0, double, double><<<round8_mf((c->N + 255) / 256), 256, 0, c->stream>>>(
+            c->N, gather_ptr(c), c->nlist, gather_forces(c), x, c->dirflag, y, nullptr, nullptr, 1);
     EFE_HIP(hipGetLastError());
     return EFE_OK;
 }
@@ -904,8 +935,12 @@ int efe_launch_spmv_mf(efe_ctx* c, const double* x, double* y) {
 void efe_mf_launch_operator_batched(efe_ctx* c, hipStream_t st, const float* p, float* q, double* npq,
                                     const int* active) {
     launch_apply<float>(c, st, p, 1, nullptr, active);
-    mf_gather_batched_kernel<<<(c->N + 255) / 256, 256, 0, st>>>(c->N, gather_ptr(c), gather_forces(c), p, c->dirflag,
-                                                                 c->specOfNode, active, q, npq);
+    if (c->mfb)
+        mf_gather_batched_kernel<true><<<(c->N + 255) / 256, 256, 0, st>>>(
+            c->N, gather_ptr(c), gather_forces(c), p, c->dirflag, c->specOfNode, active, q, npq);
+    else
+        mf_gather_batched_kernel<false><<<(c->N + 255) / 256, 256, 0, st>>>(
+            c->N, gather_ptr(c), gather_forces(c), p, c->dirflag, c->specOfNode, active, q, npq);
 }
 
 void efe_mf_free(efe_ctx* c) {
