@@ -154,7 +154,9 @@ int efe_prepare_elements(efe_ctx* c, int precision) {
     if (precision == EFE_FP32 && !c->K32) {
         EFE_HIP(hipMalloc(&c->K32, nval * sizeof(float)));
         EFE_HIP(hipMemsetAsync(c->K32, 0, nval * sizeof(float), c->stream));
-        EFE_HIP(hipMalloc(&c->Minv32, (size_t)c->N * 9 * sizeof(float)));
+        // the matrix-free operator may have allocated it already: a PCG graph captured for
+        // that operator holds its address, so it must not move
+        if (!c->Minv32) EFE_HIP(hipMalloc(&c->Minv32, (size_t)c->N * 9 * sizeof(float)));
     }
     if (precision == EFE_FP32_MATFREE)
         if (int rc = efe_mf_alloc(c)) return rc;

@@ -17,9 +17,12 @@
 // Global scalars are never stored by one workgroup for the others: every
 // workgroup re-reduces the (few thousand) partials of the previous kernel in the
 // same fixed order, so all workgroups hold bitwise identical scalars and the
-// whole solve is deterministic.  The two r.z partial buffers alternate with the
-// iteration parity, so a batch of an even number of iterations is a fixed
-// sequence of launches that is captured once into a hipGraph and replayed.
+// whole solve is deterministic.  A runs on the product's grid, B and C on a smaller
+// grid of their own (their partials re-reduced by every workgroup cost less).  The two
+// r.z partial buffers alternate with the iteration parity, so a batch of an even number
+// of iterations is a fixed sequence of launches that is captured once into a hipGraph
+// and replayed.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -30,6 +33,11 @@ struct KrylovWork {
     // rows per thread: a workgroup owns kr * block consecutive rows; [0] FP64, [1] FP32
     int krp[2] = {1, 1}, nwgp[2] = {0, 0};
     int kr = 1;  // of the precision being solved
+    // the vector kernels (init, update, direction) on their own, smaller grid: krv rows per
+    // thread, nwgv workgroups; [0] FP64, [1] FP32
+    int krvp[2] = {1, 1}, nwgvp[2] = {0, 0};
+    int krv = 1, nwgv = 0;
+    int ps = 0;  // row stride of the partials array (the largest workgroup count)
     // FP64 and FP32 vector sets
     double *x64 = nullptr, *r64 = nullptr, *z64 = nullptr, *p64 = nullptr, *q64 = nullptr;
     float *x32 = nullptr, *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr;
@@ -110,6 +118,25 @@ __device__ __forceinline__ void reduce_partials_n(const double* __restrict__ par
     block_sums<BS, K>(out, sh);
 }
 
+// reduce_partials_n with a count per array; each array's sum is formed in the same order as
+// reduce_partials_n would form it.
+template <int BS, int K>
+__device__ __forceinline__ void reduce_partials_nv(const double* __restrict__ part, const int (&offs)[K],
+                                                   const int (&ns)[K], double (&out)[K], double* sh) {
+    int nmax = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        out[k] = 0.0;
+        nmax = ns[k] > nmax ? ns[k] : nmax;
+    }
+    for (int i = threadIdx.x; i < nmax; i += BS) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (i < ns[k]) out[k] += part[offs[k] + i];
+    }
+    block_sums<BS, K>(out, sh);
+}
+
 template <int BS>
 __device__ __forceinline__ double reduce_partials(const double* __restrict__ part, int n, double* sh) {
     const int offs[1] = {0};
@@ -168,7 +195,7 @@ __global__ __launch_bounds__(BS) void spmv_kernel(int N, const int32_t* __restri
 template <typename T, int BS>
 __global__ __launch_bounds__(BS) void pcg_init_kernel(int N, const double* __restrict__ b, const T* __restrict__ Minv,
                                                       T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
-                                                      T* __restrict__ p, double* __restrict__ part, int nwg,
+                                                      T* __restrict__ p, double* __restrict__ part, int ps,
                                                       int* __restrict__ flags, int kr) {
     __shared__ double sh[BS];
     const int blk = xcd_block();
@@ -197,8 +224,8 @@ __global__ __launch_bounds__(BS) void pcg_init_kernel(int N, const double* __res
     block_sums<BS, 2>(sums, sh);
     const double s1 = sums[0], s2 = sums[1];
     if (threadIdx.x == 0) {
-        part[1 * nwg + blk] = s1;
-        part[4 * nwg + blk] = s2;
+        part[1 * ps + blk] = s1;
+        part[4 * ps + blk] = s2;
         if (blk == 0) {
             flags[0] = 0;
             flags[1] = 0;
@@ -211,13 +238,15 @@ template <typename T, int BS>
 __global__ __launch_bounds__(BS) void pcg_update_kernel(int N, const T* __restrict__ Minv, T* __restrict__ x,
                                                         T* __restrict__ r, T* __restrict__ z,
                                                         const T* __restrict__ p, const T* __restrict__ q,
-                                                        double* __restrict__ part, int nwg, int par,
-                                                        const int* __restrict__ flags, int kr) {
+                                                        double* __restrict__ part, int ps, int nop, int nvec,
+                                                        int par, const int* __restrict__ flags, int kr) {
     __shared__ double sh[BS];
     if (flags[0]) return;
-    const int offs[2] = {0, (1 + par) * nwg};
+    // p.q: nop partials of the operator's grid; r.z: nvec partials of this grid
+    const int offs[2] = {0, (1 + par) * ps};
+    const int ns[2] = {nop, nvec};
     double sc[2];
-    reduce_partials_n<BS, 2>(part, offs, nwg, sc, sh);
+    reduce_partials_nv<BS, 2>(part, offs, ns, sc, sh);
     const double pq = sc[0], rz = sc[1];
     const T alpha = (T)(rz / pq);
     const int blk = xcd_block();
@@ -245,21 +274,21 @@ __global__ __launch_bounds__(BS) void pcg_update_kernel(int N, const T* __restri
     block_sums<BS, 2>(sums, sh);
     const double s1 = sums[0], s2 = sums[1];
     if (threadIdx.x == 0) {
-        part[(2 - par) * nwg + blk] = s1;
-        part[3 * nwg + blk] = s2;
+        part[(2 - par) * ps + blk] = s1;
+        part[3 * ps + blk] = s2;
     }
 }
 
 template <typename T, int BS>
 __global__ __launch_bounds__(BS) void pcg_direction_kernel(int N, const T* __restrict__ z, T* __restrict__ p,
-                                                           const double* __restrict__ part, int nwg, int par,
-                                                           int jlocal, const double* __restrict__ dtol2,
+                                                           const double* __restrict__ part, int ps, int nvec,
+                                                           int par, int jlocal, const double* __restrict__ dtol2,
                                                            int* __restrict__ flags, int kr) {
     __shared__ double sh[BS];
     if (flags[0]) return;
-    const int offs[3] = {(1 + par) * nwg, (2 - par) * nwg, 3 * nwg};
+    const int offs[3] = {(1 + par) * ps, (2 - par) * ps, 3 * ps};
     double sc[3];
-    reduce_partials_n<BS, 3>(part, offs, nwg, sc, sh);
+    reduce_partials_n<BS, 3>(part, offs, nvec, sc, sh);
     const double rzo = sc[0], rzn = sc[1], rr = sc[2];
     const bool bad = !(rr == rr) || !(rzn == rzn);
     const bool done = bad || rr <= dtol2[0];
@@ -463,8 +492,10 @@ void launch_iter(efe_ctx* c, KrylovWork* w, hipStream_t st, T* x, T* r, T* z, T*
                                                 kr);
     else  // matrix-free FP32 operator
         efe_mf_launch_operator(c, st, (const float*)p, (float*)q, w->part, w->flags, grid, BS, kr);
-    pcg_update_kernel<T, BS><<<grid, BS, 0, st>>>(c->N, Minv, x, r, z, p, q, w->part, grid, par, w->flags, kr);
-    pcg_direction_kernel<T, BS><<<grid, BS, 0, st>>>(c->N, z, p, w->part, grid, par, j, w->dscal, w->flags, kr);
+    const int gv = w->nwgv, krv = w->krv;
+    pcg_update_kernel<T, BS><<<gv, BS, 0, st>>>(c->N, Minv, x, r, z, p, q, w->part, w->ps, grid, gv, par, w->flags,
+                                                krv);
+    pcg_direction_kernel<T, BS><<<gv, BS, 0, st>>>(c->N, z, p, w->part, w->ps, gv, par, j, w->dscal, w->flags, krv);
 }
 
 template <typename T>
@@ -482,10 +513,10 @@ template <typename T>
 void launch_init_bs(efe_ctx* c, KrylovWork* w, hipStream_t st, const double* b, const T* Minv, T* x, T* r, T* z,
                     T* p) {
     switch (w->block) {
-        case 64: pcg_init_kernel<T, 64><<<w->nwg, 64, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->nwg, w->flags, w->kr); break;
-        case 128: pcg_init_kernel<T, 128><<<w->nwg, 128, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->nwg, w->flags, w->kr); break;
-        case 512: pcg_init_kernel<T, 512><<<w->nwg, 512, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->nwg, w->flags, w->kr); break;
-        default: pcg_init_kernel<T, 256><<<w->nwg, 256, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->nwg, w->flags, w->kr); break;
+        case 64: pcg_init_kernel<T, 64><<<w->nwgv, 64, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->ps, w->flags, w->krv); break;
+        case 128: pcg_init_kernel<T, 128><<<w->nwgv, 128, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->ps, w->flags, w->krv); break;
+        case 512: pcg_init_kernel<T, 512><<<w->nwgv, 512, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->ps, w->flags, w->krv); break;
+        default: pcg_init_kernel<T, 256><<<w->nwgv, 256, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->ps, w->flags, w->krv); break;
     }
 }
 
@@ -532,13 +563,25 @@ int ensure_work(efe_ctx* c) {
     // operators with at most ~1024; uncapped, the reductions dominate at 96^3.
     const int groups = (c->N + block - 1) / block;
     const int cap[2] = {2048, 1024};
+    // The vector kernels (init, update, direction) stream less per row than the product and
+    // re-reduce every partial of the previous kernel in every workgroup, so they take at most
+    // 512 workgroups of several rows per thread.  Measured at 64^3 (history Krylov iteration,
+    // FP64 / mixed): the product's grid 125.6 / 48.6 us, at most 512 123.7 / 47.4, 256 123.1 /
+    // 47.4, 128 125.9 / 49.7.
+    const int vcap = 512;
+    w->ps = 0;
     for (int i = 0; i < 2; ++i) {
         w->krp[i] = (groups + cap[i] - 1) / cap[i];
         w->nwgp[i] = round8((c->N + block * w->krp[i] - 1) / (block * w->krp[i]));
+        w->krvp[i] = (groups + vcap - 1) / vcap;
+        w->nwgvp[i] = round8((c->N + block * w->krvp[i] - 1) / (block * w->krvp[i]));
+        w->ps = std::max(w->ps, std::max(w->nwgp[i], w->nwgvp[i]));
     }
     w->kr = w->krp[0];
     w->nwg = w->nwgp[0];
-    EFE_HIP(hipMalloc(&w->part, (size_t)5 * (w->nwgp[0] > w->nwgp[1] ? w->nwgp[0] : w->nwgp[1]) * sizeof(double)));
+    w->krv = w->krvp[0];
+    w->nwgv = w->nwgvp[0];
+    EFE_HIP(hipMalloc(&w->part, (size_t)5 * w->ps * sizeof(double)));
     return EFE_OK;
 }
 
@@ -549,13 +592,15 @@ int pcg_solve(efe_ctx* c, const double* b, double* xout, const efe_solve_opts* o
     hipStream_t st = w->ks;
     w->kr = w->krp[sizeof(T) == 4 ? 1 : 0];
     w->nwg = w->nwgp[sizeof(T) == 4 ? 1 : 0];
+    w->krv = w->krvp[sizeof(T) == 4 ? 1 : 0];
+    w->nwgv = w->nwgvp[sizeof(T) == 4 ? 1 : 0];
     // order after the caller's stream
     EFE_HIP(hipEventRecord(w->evA, c->stream));
     EFE_HIP(hipStreamWaitEvent(st, w->evA, 0));
     launch_init_bs<T>(c, w, st, b, Minv, x, r, z, p);
     EFE_HIP(hipGetLastError());
     // b.b (host) to fix the absolute tolerance
-    sum_partials_kernel<<<1, 256, 0, st>>>(w->part + 4 * w->nwg, w->nwg, w->dscal + 1);
+    sum_partials_kernel<<<1, 256, 0, st>>>(w->part + 4 * w->ps, w->nwgv, w->dscal + 1);
     EFE_HIP(hipMemcpyAsync(w->hpart, w->dscal + 1, sizeof(double), hipMemcpyDeviceToHost, st));
     EFE_HIP(hipStreamSynchronize(st));
     const double bnorm = sqrt(w->hpart[0]);
@@ -614,7 +659,7 @@ int pcg_solve(efe_ctx* c, const double* b, double* xout, const efe_solve_opts* o
     EFE_HIP(hipEventRecord(w->evB, st));
     EFE_HIP(hipStreamWaitEvent(c->stream, w->evB, 0));
     // final residual of the recursion
-    sum_partials_kernel<<<1, 256, 0, st>>>(w->part + 3 * w->nwg, w->nwg, w->dscal + 1);
+    sum_partials_kernel<<<1, 256, 0, st>>>(w->part + 3 * w->ps, w->nwgv, w->dscal + 1);
     EFE_HIP(hipMemcpyAsync(w->hpart, w->dscal + 1, sizeof(double), hipMemcpyDeviceToHost, st));
     EFE_HIP(hipStreamSynchronize(st));
     info->residual = sqrt(w->hpart[0]);

@@ -512,6 +512,44 @@ def test_matfree_operator_matches_assembled(el_type, material, props, n):
     assert torch.equal(ym[d], x[d].float().double())
 
 
+def test_matfree_solve_after_an_assembled_fp32_pass_keeps_its_preconditioner():
+    """The matrix-free PCG is replayed from a captured graph that holds the address of the FP32
+    block-Jacobi inverse; a first assembled FP32 pass must not move that buffer.  A matrix-free
+    solve at a plastic state, after matrix-free and assembled FP32 passes and solves at another
+    state (the mixed operator cost probe's sequence), equals bit for bit the same solve on a
+    fresh context."""
+    from edelweissfe_amd import _lib
+
+    pm = device_model((4, 4, 3), (1.0, 1.0, 1.0), "C3D8", "vonmises", J2, jitter=0.1, seed=2)
+    dirs = np.r_[pm.dofsOfNodeSet("gen_bottom"), pm.dofsOfNodeSet("gen_top", [1])]
+    rng = np.random.default_rng(3)
+    dU = torch.as_tensor(rng.uniform(-1, 1, 3 * pm.nNodes) * 4e-3, device="cuda")
+    b0 = torch.as_tensor(rng.standard_normal(3 * pm.nNodes), device="cuda")
+
+    def solve(probe):
+        sysm = _system(pm)
+        sysm.set_dirichlet(dirs)
+        P, F, x = sysm.zeros(), sysm.zeros(), sysm.zeros()
+        if probe:
+            for prec in (_lib.EFE_FP32_MATFREE, _lib.EFE_FP32):
+                sysm.compute_elements(sysm.zeros(), P, F, precision=prec)
+                sysm.linsolve(b0, x, precision=prec, rtol=1e-30, max_iter=64, check_every=32)
+        sysm.compute_elements(dU, P, F, precision=_lib.EFE_FP32_MATFREE)
+        assert (sysm.qp_state(1)[..., 12] > 0).any()
+        b = P.clone()
+        b[torch.as_tensor(dirs, device="cuda")] = 0.0
+        x = sysm.zeros()
+        info = sysm.linsolve(b, x, precision=_lib.EFE_FP32_MATFREE, rtol=1e-6, check_every=32)
+        sysm.close()
+        assert info.status == 0
+        return info.iterations, x
+
+    its_a, xa = solve(False)
+    its_b, xb = solve(True)
+    assert its_a == its_b
+    assert torch.equal(xa, xb)
+
+
 @pytest.mark.parametrize("case", [
     ((4, 3, 5), "C3D8", "vonmises", J2, 3e-3),
     ((2, 2, 2), "C3D20R", "druckerprager", None, 3e-3),
