@@ -51,6 +51,8 @@ struct KrylovWork {
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};  // FP64, FP32 assembled, FP32 matrix-free
     int graph_batch[3] = {0, 0, 0};
     int graph_block[3] = {0, 0, 0};
+    // the operator buffers a batch graph was captured with (a graph keeps their addresses)
+    const void* graph_ops[3][4] = {};
 };
 
 namespace {
@@ -623,8 +625,11 @@ int pcg_solve(efe_ctx* c, const double* b, double* xout, const efe_solve_opts* o
     const int maxit = o->max_iter > 0 ? o->max_iter : 10000;
     w->hpart[1] = tol2;
     EFE_HIP(hipMemcpyAsync(w->dscal, w->hpart + 1, sizeof(double), hipMemcpyHostToDevice, st));
-    // (re)build the captured batch when its shape changed
-    if (!w->graph[gi] || w->graph_batch[gi] != batch || w->graph_block[gi] != w->block) {
+    // (re)build the captured batch when its shape or an operator buffer changed
+    const void* ops[4] = {K, Minv, c->mfC, c->mfbF};
+    bool same = true;
+    for (int i = 0; i < 4; ++i) same = same && w->graph_ops[gi][i] == ops[i];
+    if (!w->graph[gi] || w->graph_batch[gi] != batch || w->graph_block[gi] != w->block || !same) {
         if (w->graph[gi]) hipGraphExecDestroy(w->graph[gi]);
         w->graph[gi] = nullptr;
         hipGraph_t g;
@@ -635,6 +640,7 @@ int pcg_solve(efe_ctx* c, const double* b, double* xout, const efe_solve_opts* o
         EFE_HIP(hipGraphDestroy(g));
         w->graph_batch[gi] = batch;
         w->graph_block[gi] = w->block;
+        for (int i = 0; i < 4; ++i) w->graph_ops[gi][i] = ops[i];
     }
     int it = 0;
     EFE_HIP(hipEventRecord(w->t0, st));
