@@ -512,6 +512,42 @@ def test_matfree_operator_matches_assembled(el_type, material, props, n):
     assert torch.equal(ym[d], x[d].float().double())
 
 
+def test_matfree_operator_on_a_mesh_without_element_locality():
+    """Elements in random order: 256 consecutive elements touch more nodes than a workgroup
+    brick stages, so the hexa8 matrix-free operator keeps one force slot per (element, node)
+    (mf_apply_hex8_kernel and the slot gather).  Against the FP64 assembled operator of the same
+    context at a plastic state, as test_matfree_operator_matches_assembled does."""
+    from edelweissfe_amd import _lib
+    from edelweissfe_amd.device import DeviceSystem
+
+    pm = device_model((12, 12, 12), (1.0, 1.0, 1.0), "C3D8", "vonmises", J2, jitter=0.1, seed=2)
+    perm = np.random.default_rng(4).permutation(pm.connectivity.shape[0])
+    sysm = DeviceSystem(pm.coordinates, np.ascontiguousarray(pm.connectivity[perm]), "C3D8", "vonmises",
+                        pm.material["properties"])
+    dirs = np.r_[pm.dofsOfNodeSet("gen_bottom"), pm.dofsOfNodeSet("gen_top", [1])]
+    sysm.set_dirichlet(dirs)
+    rng = np.random.default_rng(11)
+    dU = torch.as_tensor(rng.uniform(-1, 1, sysm.n_dof) * 4e-3, device="cuda")
+    x = torch.as_tensor(rng.standard_normal(sysm.n_dof), device="cuda")
+    P64, F64 = sysm.zeros(), sysm.zeros()
+    sysm.compute_elements(dU, P64, F64)
+    assert (sysm.qp_state(1)[..., 12] > 0).any()
+    y64 = sysm.zeros()
+    sysm.spmv(x, y64)
+    Pm, Fm = sysm.zeros(), sysm.zeros()
+    sysm.compute_elements(dU, Pm, Fm, precision=_lib.EFE_FP32_MATFREE)
+    ym = sysm.zeros()
+    sysm.spmv(x, ym)
+    b = Pm.clone()
+    b[torch.as_tensor(dirs, device="cuda")] = 0.0
+    xm = sysm.zeros()
+    info = sysm.linsolve(b, xm, precision=_lib.EFE_FP32_MATFREE, rtol=1e-6)
+    sysm.close()
+    assert torch.equal(Pm, P64) and torch.equal(Fm, F64)
+    assert rel_err(ym.cpu().numpy(), y64.cpu().numpy()) < 3e-6
+    assert info.status == 0 and info.iterations > 0
+
+
 def test_matfree_solve_after_an_assembled_fp32_pass_keeps_its_preconditioner():
     """The matrix-free PCG is replayed from a captured graph that holds the address of the FP32
     block-Jacobi inverse; a first assembled FP32 pass must not move that buffer.  A matrix-free
