@@ -228,10 +228,13 @@ def test_specimen_alone_and_in_a_batch_are_bitwise_equal(precision):
     assert np.array_equal(alone.qpStates(ma)[0], batch.qpStates(mb)[1])
 
 
-def test_groups_by_element_and_material_on_the_device():
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_groups_by_element_and_material_on_the_device(precision):
     """NISTBatchedGroups: five interleaved specimens of three (element type, material)
     groups (hexa8 J2, hexa8 linear elastic, hexa20R J2), one batched context per group;
-    every specimen follows its own oracle NIST run."""
+    every specimen follows its own oracle NIST run (FP64: Newton counts equal, U and stress
+    to 1e-9; mixed, which runs the hexa20R group on the per-slot matrix-free operator with
+    frozen specimens skipped: U to 1e-8)."""
     from edelweissfe_amd.solvers import NISTBatchedGroups
     from test_batched_control import GROUPED
 
@@ -240,7 +243,7 @@ def test_groups_by_element_and_material_on_the_device():
         pm = device_model(n, (1.0, 1.0, 1.0), et, mat, props, jitter=0.1, seed=seed)
         models.append(pm)
         steps.append(device_step(pm, uniaxial_bcs(strain), maxInc=0.25, minInc=1e-6, maxNumInc=100, maxIter=15))
-    solver = NISTBatchedGroups(precision="fp64", autotune=False)
+    solver = NISTBatchedGroups(precision=precision, autotune=False)
     ok, _ = solver.solveSteps(models, steps)
     assert ok and [g[1] for g in solver.groups] == [[0, 3], [1, 4], [2]]
     states = solver.qpStates(models)
@@ -250,9 +253,12 @@ def test_groups_by_element_and_material_on_the_device():
         so.solve_step(om, oracle_step(uniaxial_bcs(strain), maxInc=0.25, minInc=1e-6, maxNumInc=100, maxIter=15))
         want = [np.inf if h[2] is None else h[2] for h in so.history]
         print(s, et, mat, solver.history[s], want)
-        assert solver.history[s] == want
-        assert rel_err(models[s].U, om.U) < 1e-9
-        assert rel_err(states[s][..., 0:6], om.state[..., 0:6]) < 1e-9
+        if precision == "fp64":
+            assert solver.history[s] == want
+            assert rel_err(models[s].U, om.U) < 1e-9
+            assert rel_err(states[s][..., 0:6], om.state[..., 0:6]) < 1e-9
+        else:
+            assert rel_err(models[s].U, om.U) < 1e-8
 
 
 def _run_specimens(specs, precision, newton_on_device, **kw):
