@@ -789,10 +789,11 @@ void launch_elem_t(efe_ctx* c, const double* dU) {
 
 template <int NN, int Q, int MAT, typename KT>
 int launch_elem_epw(efe_ctx* c, const double* dU) {
-    // default: hexa8 4 (assembled passes) or 8 (matrix-free pass: its phase 1 of one lane per
-    // element and QP then fills the wavefront, 0.34 against 0.51 ms at 64^3), hexa20R 2, hexa20 1
+    // default: hexa8 4 (assembled passes) or 8 (matrix-free and residual-only passes: phase 1,
+    // one lane per element and QP, then fills the wavefront; 64^3: matrix-free 0.34 against
+    // 0.51 ms, residual-only 0.27 against 0.34 ms), hexa20R 2, hexa20 1
     int epw = c->epw;
-    if (epw <= 0) epw = (NN == 8) ? (c->mfPass ? 8 : 4) : (Q == 8 ? 2 : 1);
+    if (epw <= 0) epw = (NN == 8) ? ((c->mfPass || !c->formK) ? 8 : 4) : (Q == 8 ? 2 : 1);
     switch (epw) {
         case 1: launch_elem_t<NN, Q, MAT, KT, 1>(c, dU); break;
         case 2: launch_elem_t<NN, Q, MAT, KT, 2>(c, dU); break;
