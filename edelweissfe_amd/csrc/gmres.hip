@@ -121,8 +121,13 @@ __global__ __launch_bounds__(BS) void ilu_copy_kernel(int N, const int32_t* __re
     const int ps = nodeSlot[n];
     const int s = ps / kSlice, l = ps - (ps / kSlice) * kSlice;
     const int64_t vb = (int64_t)slice_off[s] * 9 + l;
-    for (int k = 0, b = bptr[n]; b < bptr[n + 1]; ++k, ++b) {
-        T* dst = F + (int64_t)fidx[b] * 9;
+    const int b0 = bptr[n], nb = bptr[n + 1] - b0;
+    // several blocks in flight per lane; measured on the 48^3 hexa20 operator: FP64 2.25 / 1.98
+    // / 2.66 ms at 1 / 4 / 8 blocks, FP32 1.44 / 1.20 / 1.04 ms
+    constexpr int kUnroll = sizeof(T) == 8 ? 4 : 8;
+#pragma unroll kUnroll
+    for (int k = 0; k < nb; ++k) {
+        T* dst = F + (int64_t)fidx[b0 + k] * 9;
 #pragma unroll
         for (int v = 0; v < 9; ++v) dst[v] = K[vb + (int64_t)(k * 9 + v) * kSlice];
     }
