@@ -400,6 +400,7 @@ __global__ __launch_bounds__(256) void assemble_kernel(
     for (int v = 0; v < 9; ++v) Minv[(int64_t)n * 9 + v] = (KT)iv[v];
     if (P) {
         double p0 = 0, p1 = 0, p2 = 0, f0 = 0, f1 = 0, f2 = 0;
+#pragma unroll 4
         for (int c = nptr[n]; c < nptr[n + 1]; ++c) {
             const double* src = Pscr + (int64_t)nlist[c] * 3;
             p0 += src[0];
@@ -533,6 +534,7 @@ __global__ __launch_bounds__(256) void assemble_runs_kernel(
     for (int v = 0; v < 9; ++v) Minv[(int64_t)n * 9 + v] = (KT)iv[v];
     if (P) {
         double p0 = 0, p1 = 0, p2 = 0, f0 = 0, f1 = 0, f2 = 0;
+#pragma unroll 4
         for (int c = nptr[n]; c < nptr[n + 1]; ++c) {
             const double* src = Pscr + (int64_t)nlist[c] * 3;
             p0 += src[0];
@@ -717,6 +719,7 @@ __global__ void pf_gather_kernel(int N, const int32_t* __restrict__ nptr, const 
     if (n >= N) return;
     if (specActive && !specActive[specOfNode[n]]) return;
     double p0 = 0, p1 = 0, p2 = 0, f0 = 0, f1 = 0, f2 = 0;
+#pragma unroll 4
     for (int c = nptr[n]; c < nptr[n + 1]; ++c) {
         const double* src = Pscr + (int64_t)nlist[c] * 3;
         p0 += src[0];

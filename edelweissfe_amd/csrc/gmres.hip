@@ -548,6 +548,7 @@ __global__ __launch_bounds__(BS) void mdot_kernel(int D, const T* __restrict__ V
     const int blk = blockIdx.x;
     const T* vi = V + (int64_t)blockIdx.y * D;
     double s = 0.0;
+#pragma unroll 4
     for (int i = blk * BS + threadIdx.x; i < D; i += nwg * BS) s += (double)vi[i] * (double)w[i];
     s = bsum(s, sh);
     if (threadIdx.x == 0) part[(int64_t)blockIdx.y * nwg + blk] = s;
@@ -569,6 +570,7 @@ __global__ void combine_kernel(int D, int nv, const T* __restrict__ V, const dou
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= D) return;
     double acc = overwrite ? 0.0 : (double)w[i];
+#pragma unroll 4
     for (int k = 0; k < nv; ++k) acc += (overwrite ? 1.0 : -1.0) * h[k] * (double)V[(int64_t)k * D + i];
     w[i] = (T)acc;
 }

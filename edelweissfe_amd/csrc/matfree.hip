@@ -674,6 +674,7 @@ __global__ __launch_bounds__(256) void mf_diag_kernel(
     // K_nn = sum of the K_aa blocks [e][a][9] of the (element, local node) pairs of node n,
     // in element order (the order of the diagonal block's contributions in the assembly)
     float acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll 4
     for (int c = nptr[n]; c < nptr[n + 1]; ++c) add_block(Kscr + (int64_t)nlist[c] * kBlockPitch, acc);
     double dg[9];
 #pragma unroll
@@ -697,6 +698,7 @@ __global__ __launch_bounds__(256) void mf_diag_kernel(
 #pragma unroll
     for (int v = 0; v < 9; ++v) Minv[(int64_t)n * 9 + v] = (float)iv[v];
     double p0 = 0, p1 = 0, p2 = 0, f0 = 0, f1 = 0, f2 = 0;
+#pragma unroll 4
     for (int c = nptr[n]; c < nptr[n + 1]; ++c) {
         const double* src = Pscr + (int64_t)nlist[c] * 3;
         p0 += src[0];
