@@ -12,12 +12,16 @@ error against the FP64 run is reported per field.
 
 Prints one JSON line.  `value` is FP64 Newton iterations (linear solves) per
 second over the timed load histories; inputs (mesh, pattern, state) are
-resident in HBM before the timed region starts.
+resident in HBM before the timed region starts.  Unless --no-others is given,
+the default run then measures the three other workloads (96^3 lognormal J2, the
+512-specimen batch, the 48^3 hexa20R footing), one child process each after
+the headline's own timing, and carries their lines under "other_workloads".
 """
 
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -519,6 +523,38 @@ def bench_footing(args):
     print(json.dumps(result), flush=True)
 
 
+# The other workloads of the default run: a single history each (the batch with one warm-up
+# history; the 96^3 and footing histories take 20-30 s each and run without one, so their
+# figures include first-use costs: graph captures and the mixed-operator probe).
+OTHER_WORKLOADS = (
+    ("96cube_lognormal_j2", ["--workload", "lognormal", "--steps", "1", "--warmup", "0"]),
+    ("batched_512_specimens", ["--workload", "batched", "--steps", "1", "--warmup", "1"]),
+    ("footing_48cube_hexa20r_dp", ["--workload", "footing", "--steps", "1", "--warmup", "0"]),
+)
+_DROP = ("metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline", "data")
+
+
+def run_other_workloads(timeout_s=400):
+    """Each of OTHER_WORKLOADS in a child process (its progress lines pass through on
+    stderr); its JSON line without the keys it shares with the headline line, or an error
+    entry when the child fails, so that one workload's failure does not lose the others."""
+    out = {}
+    for name, extra in OTHER_WORKLOADS:
+        cmd = [sys.executable, "-u", os.path.abspath(__file__), "--no-cpu", "--no-others"] + extra
+        log("[bench] other workload %s: %s" % (name, " ".join(extra)))
+        try:
+            proc = subprocess.run(cmd, stdout=subprocess.PIPE, timeout=timeout_s, text=True)
+            lines = [ln for ln in proc.stdout.splitlines() if ln.startswith("{")]
+            if proc.returncode != 0 or not lines:
+                out[name] = {"error": "exit status %d" % proc.returncode}
+                continue
+            d = json.loads(lines[-1])
+            out[name] = {k: v for k, v in d.items() if k not in _DROP}
+        except Exception as e:  # reported in the line, never dropped silently
+            out[name] = {"error": repr(e)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -540,6 +576,8 @@ def main():
                          "footing: 48^3 C3D20R Drucker-Prager geostatic + rigid footing, GMRES + ILU(0)")
     ap.add_argument("--specimens", type=int, default=512)
     ap.add_argument("--batch-increments", type=int, default=10)
+    ap.add_argument("--no-others", action="store_true",
+                    help="cube workload: skip the other workloads' child runs after the headline")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the library runs the Newton step on one GPU (run with --gpus 1)",
@@ -667,6 +705,10 @@ def main():
                                                           budget_s=args.cpu_seconds, counts=counts, log=log)
         except Exception as e:  # reported, never silently replaced
             result["cpu_baseline"] = {"error": repr(e)}
+    if not args.no_others and not lognormal and args.n == 64:
+        system.close()  # the children build their own device systems
+        torch.cuda.empty_cache()
+        result["other_workloads"] = run_other_workloads()
     print(json.dumps(result), flush=True)
 
 
