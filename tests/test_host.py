@@ -146,3 +146,18 @@ def test_bench_refuses_more_than_one_gpu():
                        timeout=300)
     assert r.returncode == 2
     assert "one GPU only" in r.stderr
+
+
+def test_bench_other_workloads_report_a_failing_child_and_keep_the_rest(monkeypatch):
+    """run_other_workloads: a child that fails becomes an error entry in the line, and the
+    workloads after it still run (here every child is asked for two GPUs and refuses at once,
+    before any GPU work)."""
+    import sys
+
+    sys.path.insert(0, ROOT)
+    import bench
+
+    monkeypatch.setattr(bench, "OTHER_WORKLOADS", (("first", ["--gpus", "2"]), ("second", ["--gpus", "2"])))
+    out = bench.run_other_workloads(timeout_s=300)
+    assert set(out) == {"first", "second"}
+    assert out["first"] == {"error": "exit status 2"} and out["second"] == {"error": "exit status 2"}
