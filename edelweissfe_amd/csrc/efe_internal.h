@@ -218,6 +218,8 @@ int efe_launch_geostatic(efe_ctx* c, const double* def, const int32_t* elements,
 // matrix-free FP32 operator (matfree.hip)
 int efe_mf_alloc(efe_ctx* c);
 int efe_launch_mf_assembly(efe_ctx* c, double* P, double* F, const double* dir_values, double* dir_correction);
+// hexa8 bricks: the product's node partials (mfbF over mfbPptr) and per-brick partials of p.Kp
+void efe_mf_launch_apply_partials(efe_ctx* c, hipStream_t st, const float* p, double* part, const int* flags);
 void efe_mf_launch_operator(efe_ctx* c, hipStream_t st, const float* p, float* q, double* part, const int* flags,
                             int nwg, int bs, int kr);
 void efe_mf_free(efe_ctx* c);

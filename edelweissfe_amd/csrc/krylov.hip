@@ -22,6 +22,10 @@
 // r.z partial buffers alternate with the iteration parity, so a batch of an even number
 // of iterations is a fixed sequence of launches that is captured once into a hipGraph
 // and replayed.
+// With the hexa8 brick operator (matrix-free FP32), A is the brick product alone, its
+// partials the bricks' shares of p.Kp (Dirichlet DOFs masked), and B gathers each node's
+// brick partials itself (q is never stored): p.q = the bricks' sum + the sum of p^2 over the
+// Dirichlet DOFs, whose partials C (and the init) form with p.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -41,7 +45,7 @@ struct KrylovWork {
     // FP64 and FP32 vector sets
     double *x64 = nullptr, *r64 = nullptr, *z64 = nullptr, *p64 = nullptr, *q64 = nullptr;
     float *x32 = nullptr, *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr;
-    double* part = nullptr;  // [5][nwg]: pq, rz0, rz1, rr, bb
+    double* part = nullptr;  // [6][ps]: pq, rz0, rz1, rr, bb, p.p over the Dirichlet DOFs
     int* flags = nullptr;    // device: [done, nan, iters_local, pad]
     double* dscal = nullptr; // device: [tol2, scratch sum]
     int* hflags = nullptr;   // pinned host copy
@@ -194,14 +198,15 @@ __global__ __launch_bounds__(BS) void spmv_kernel(int N, const int32_t* __restri
 
 // x = 0, r = b (b_dir = b on Dirichlet DOFs is part of the system), z = Minv r, p = z,
 // partials r.z -> part[1], b.b -> part[4].
-template <typename T, int BS>
+template <typename T, int BS, bool DP2 = false>
 __global__ __launch_bounds__(BS) void pcg_init_kernel(int N, const double* __restrict__ b, const T* __restrict__ Minv,
                                                       T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
                                                       T* __restrict__ p, double* __restrict__ part, int ps,
-                                                      int* __restrict__ flags, int kr) {
+                                                      int* __restrict__ flags, int kr,
+                                                      const uint8_t* __restrict__ dirflag = nullptr) {
     __shared__ double sh[BS];
     const int blk = xcd_block();
-    double rz = 0, bb = 0;
+    double rz = 0, bb = 0, pd = 0;
     for (int it = 0; it < kr; ++it) {
         const int n = (blk * kr + it) * BS + threadIdx.x;
         if (n >= N) break;
@@ -220,14 +225,20 @@ __global__ __launch_bounds__(BS) void pcg_init_kernel(int N, const double* __res
             z[3 * n + i] = zi;
             p[3 * n + i] = zi;
             rz += (double)rv[i] * zi;
+            if (DP2 && dirflag[3 * n + i]) pd += (double)zi * zi;
         }
     }
-    double sums[2] = {rz, bb};
-    block_sums<BS, 2>(sums, sh);
+    constexpr int NS = DP2 ? 3 : 2;
+    double sums[NS];
+    sums[0] = rz;
+    sums[1] = bb;
+    if constexpr (DP2) sums[2] = pd;
+    block_sums<BS, NS>(sums, sh);
     const double s1 = sums[0], s2 = sums[1];
     if (threadIdx.x == 0) {
         part[1 * ps + blk] = s1;
         part[4 * ps + blk] = s2;
+        if constexpr (DP2) part[5 * ps + blk] = sums[NS - 1];
         if (blk == 0) {
             flags[0] = 0;
             flags[1] = 0;
@@ -281,11 +292,12 @@ __global__ __launch_bounds__(BS) void pcg_update_kernel(int N, const T* __restri
     }
 }
 
-template <typename T, int BS>
+template <typename T, int BS, bool DP2 = false>
 __global__ __launch_bounds__(BS) void pcg_direction_kernel(int N, const T* __restrict__ z, T* __restrict__ p,
-                                                           const double* __restrict__ part, int ps, int nvec,
+                                                           double* __restrict__ part, int ps, int nvec,
                                                            int par, int jlocal, const double* __restrict__ dtol2,
-                                                           int* __restrict__ flags, int kr) {
+                                                           int* __restrict__ flags, int kr,
+                                                           const uint8_t* __restrict__ dirflag = nullptr) {
     __shared__ double sh[BS];
     if (flags[0]) return;
     const int offs[3] = {(1 + par) * ps, (2 - par) * ps, 3 * ps};
@@ -304,11 +316,78 @@ __global__ __launch_bounds__(BS) void pcg_direction_kernel(int N, const T* __res
     }
     const T beta = (T)(rzn / rzo);
     const int blk = xcd_block();
+    double pd = 0;
     for (int it = 0; it < kr; ++it) {
         const int n = (blk * kr + it) * BS + threadIdx.x;
         if (n >= N) break;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) p[3 * n + i] = z[3 * n + i] + beta * p[3 * n + i];
+        for (int i = 0; i < 3; ++i) {
+            const T pv = z[3 * n + i] + beta * p[3 * n + i];
+            p[3 * n + i] = pv;
+            if (DP2 && dirflag[3 * n + i]) pd += (double)pv * pv;
+        }
+    }
+    if (DP2) {
+        const double s = block_sum<BS>(pd, sh);
+        if (threadIdx.x == 0) part[5 * ps + blk] = s;
+    }
+}
+
+// B for the hexa8 brick operator (FP32): p.q = the bricks' partials (nop of them) + the
+// Dirichlet p.p partials; q of node n = the sum of its brick partials (in brick order), p on
+// its Dirichlet DOFs, formed here and not stored.
+template <int BS>
+__global__ __launch_bounds__(BS) void pcg_update_mfb_kernel(int N, const float* __restrict__ Minv,
+                                                            float* __restrict__ x, float* __restrict__ r,
+                                                            float* __restrict__ z, const float* __restrict__ p,
+                                                            const int32_t* __restrict__ gptr,
+                                                            const float* __restrict__ gF,
+                                                            const uint8_t* __restrict__ dirflag,
+                                                            double* __restrict__ part, int ps, int nop, int nvec,
+                                                            int par, const int* __restrict__ flags, int kr) {
+    __shared__ double sh[BS];
+    if (flags[0]) return;
+    const int offs[3] = {0, 5 * ps, (1 + par) * ps};
+    const int ns[3] = {nop, nvec, nvec};
+    double sc[3];
+    reduce_partials_nv<BS, 3>(part, offs, ns, sc, sh);
+    const double pq = sc[0] + sc[1], rz = sc[2];
+    const float alpha = (float)(rz / pq);
+    const int blk = xcd_block();
+    double rzn = 0, rr = 0;
+    for (int it = 0; it < kr; ++it) {
+        const int n = (blk * kr + it) * BS + threadIdx.x;
+        if (n >= N) break;
+        float qv[3] = {0.0f, 0.0f, 0.0f};
+        const int g0 = gptr[n], g1 = gptr[n + 1];
+        for (int c = g0; c < g1; ++c) {
+            qv[0] += gF[3 * (int64_t)c];
+            qv[1] += gF[3 * (int64_t)c + 1];
+            qv[2] += gF[3 * (int64_t)c + 2];
+        }
+        float rv[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const float pv = p[3 * n + i];
+            const float q = dirflag[3 * n + i] ? pv : qv[i];
+            x[3 * n + i] += alpha * pv;
+            rv[i] = r[3 * n + i] - alpha * q;
+            r[3 * n + i] = rv[i];
+            rr += (double)rv[i] * rv[i];
+        }
+        const float* M = Minv + (int64_t)n * 9;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const float zi = M[3 * i] * rv[0] + M[3 * i + 1] * rv[1] + M[3 * i + 2] * rv[2];
+            z[3 * n + i] = zi;
+            rzn += (double)rv[i] * zi;
+        }
+    }
+    double sums[2] = {rzn, rr};
+    block_sums<BS, 2>(sums, sh);
+    if (threadIdx.x == 0) {
+        part[(2 - par) * ps + blk] = sums[0];
+        part[3 * ps + blk] = sums[1];
     }
 }
 
@@ -489,12 +568,22 @@ void launch_iter(efe_ctx* c, KrylovWork* w, hipStream_t st, T* x, T* r, T* z, T*
                  const T* Minv, int par, int j) {
     const int grid = w->nwg;
     const int kr = w->kr;
+    const int gv = w->nwgv, krv = w->krv;
+    if constexpr (sizeof(T) == 4) {
+        if (!K && c->mfb) {  // hexa8 bricks: the product's partials gathered by the update
+            efe_mf_launch_apply_partials(c, st, p, w->part, w->flags);
+            pcg_update_mfb_kernel<BS><<<gv, BS, 0, st>>>(c->N, Minv, x, r, z, p, c->mfbPptr, c->mfbF, c->dirflag,
+                                                         w->part, w->ps, round8(c->mfbG), gv, par, w->flags, krv);
+            pcg_direction_kernel<T, BS, true><<<gv, BS, 0, st>>>(c->N, z, p, w->part, w->ps, gv, par, j, w->dscal,
+                                                                 w->flags, krv, c->dirflag);
+            return;
+        }
+    }
     if (K)
         spmv_kernel<T, BS><<<grid, BS, 0, st>>>(c->N, c->slice_off, c->scol, c->slotNode, K, p, q, w->part, w->flags,
                                                 kr);
     else  // matrix-free FP32 operator
         efe_mf_launch_operator(c, st, (const float*)p, (float*)q, w->part, w->flags, grid, BS, kr);
-    const int gv = w->nwgv, krv = w->krv;
     pcg_update_kernel<T, BS><<<gv, BS, 0, st>>>(c->N, Minv, x, r, z, p, q, w->part, w->ps, grid, gv, par, w->flags,
                                                 krv);
     pcg_direction_kernel<T, BS><<<gv, BS, 0, st>>>(c->N, z, p, w->part, w->ps, gv, par, j, w->dscal, w->flags, krv);
@@ -513,7 +602,16 @@ void launch_iter_bs(efe_ctx* c, KrylovWork* w, hipStream_t st, T* x, T* r, T* z,
 
 template <typename T>
 void launch_init_bs(efe_ctx* c, KrylovWork* w, hipStream_t st, const double* b, const T* Minv, T* x, T* r, T* z,
-                    T* p) {
+                    T* p, bool dp2) {
+    if (dp2) {  // the brick operator's Dirichlet p.p partials
+        switch (w->block) {
+            case 64: pcg_init_kernel<T, 64, true><<<w->nwgv, 64, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->ps, w->flags, w->krv, c->dirflag); break;
+            case 128: pcg_init_kernel<T, 128, true><<<w->nwgv, 128, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->ps, w->flags, w->krv, c->dirflag); break;
+            case 512: pcg_init_kernel<T, 512, true><<<w->nwgv, 512, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->ps, w->flags, w->krv, c->dirflag); break;
+            default: pcg_init_kernel<T, 256, true><<<w->nwgv, 256, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->ps, w->flags, w->krv, c->dirflag); break;
+        }
+        return;
+    }
     switch (w->block) {
         case 64: pcg_init_kernel<T, 64><<<w->nwgv, 64, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->ps, w->flags, w->krv); break;
         case 128: pcg_init_kernel<T, 128><<<w->nwgv, 128, 0, st>>>(c->N, b, Minv, x, r, z, p, w->part, w->ps, w->flags, w->krv); break;
@@ -526,7 +624,9 @@ int ensure_work(efe_ctx* c) {
     int block = c->kblock;
     if (block != 64 && block != 128 && block != 256 && block != 512) block = 256;
     KrylovWork* w = c->kw;
-    if (w && w->block == block) return EFE_OK;
+    // the brick operator writes one p.Kp partial per brick into the partials array
+    const int need = c->mfb ? round8(c->mfbG) : 0;
+    if (w && w->block == block && w->ps >= need) return EFE_OK;
     if (!w) {
         w = new KrylovWork();
         c->kw = w;
@@ -579,11 +679,12 @@ int ensure_work(efe_ctx* c) {
         w->nwgvp[i] = round8((c->N + block * w->krvp[i] - 1) / (block * w->krvp[i]));
         w->ps = std::max(w->ps, std::max(w->nwgp[i], w->nwgvp[i]));
     }
+    w->ps = std::max(w->ps, need);
     w->kr = w->krp[0];
     w->nwg = w->nwgp[0];
     w->krv = w->krvp[0];
     w->nwgv = w->nwgvp[0];
-    EFE_HIP(hipMalloc(&w->part, (size_t)5 * w->ps * sizeof(double)));
+    EFE_HIP(hipMalloc(&w->part, (size_t)6 * w->ps * sizeof(double)));
     return EFE_OK;
 }
 
@@ -599,7 +700,7 @@ int pcg_solve(efe_ctx* c, const double* b, double* xout, const efe_solve_opts* o
     // order after the caller's stream
     EFE_HIP(hipEventRecord(w->evA, c->stream));
     EFE_HIP(hipStreamWaitEvent(st, w->evA, 0));
-    launch_init_bs<T>(c, w, st, b, Minv, x, r, z, p);
+    launch_init_bs<T>(c, w, st, b, Minv, x, r, z, p, !K && c->mfb);
     EFE_HIP(hipGetLastError());
     // b.b (host) to fix the absolute tolerance
     sum_partials_kernel<<<1, 256, 0, st>>>(w->part + 4 * w->ps, w->nwgv, w->dscal + 1);

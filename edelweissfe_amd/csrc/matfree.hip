@@ -425,16 +425,20 @@ __global__ __launch_bounds__(kBrick) void mfb_apply_hex8_kernel(
     const uint16_t* __restrict__ lidx, const double* __restrict__ coords, const float* __restrict__ mfC,
     const TP* __restrict__ p, const uint8_t* __restrict__ dirflag, int mask, const int* __restrict__ flags,
     float* __restrict__ Fpart, int G, const int32_t* __restrict__ e0s, const int* __restrict__ active,
-    const int32_t* __restrict__ specOfElem) {
+    const int32_t* __restrict__ specOfElem, double* __restrict__ part) {
     // LDS, reused: first the staged node data (p 3 x 768 floats, x 3 x 768 doubles), then the
     // element forces (256 x 8 x 3 floats)
     __shared__ double lds[3 * kBrickNodes + (3 * kBrickNodes + 1) / 2];
+    __shared__ double red[kBrick / 64];
     double* sx = lds;
     float* sp = (float*)(lds + 3 * kBrickNodes);
     float* sf = (float*)lds;
     if (flags && flags[0]) return;
     const int g = xcd_block_mf();
-    if (g >= G) return;  // the grid is rounded up to a multiple of 8
+    if (g >= G) {  // the grid is rounded up to a multiple of 8
+        if (part && threadIdx.x == 0) part[g] = 0.0;
+        return;
+    }
     const int eb = e0s[g], ee = e0s[g + 1];
     // batched solves: a brick of a frozen specimen leaves its partials alone (the gather skips
     // that specimen's nodes)
@@ -476,6 +480,9 @@ __global__ __launch_bounds__(kBrick) void mfb_apply_hex8_kernel(
         }
     }
     __syncthreads();  // the staging area becomes the force area
+    // with part given: the brick's sum of p_e . f_e (its share of p . K p), formed in the
+    // monomial basis: f = H^T fa / 8 and cp = H p / 8 with H H = 8 I give p . f = cp . fa
+    double en = 0.0;
     if (live) {
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
@@ -555,6 +562,11 @@ __global__ __launch_bounds__(kBrick) void mfb_apply_hex8_kernel(
                 fa[i][7] += m0 * yz + m1 * xz + m2 * xy;
             }
         }
+        if (part)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int k = 0; k < 8; ++k) en += (double)cp[i][k] * (double)fa[i][k];
 #pragma unroll
         for (int i = 0; i < 3; ++i) walsh3_t(fa[i]);
         // forces [i][t * 8 + a] (component-major: lanes t write 8 apart, then the node sums read
@@ -580,6 +592,10 @@ __global__ __launch_bounds__(kBrick) void mfb_apply_hex8_kernel(
         dst[0] = f0;
         dst[1] = f1;
         dst[2] = f2;
+    }
+    if (part) {
+        const double sum = block_sum_mf<kBrick>(en, red);
+        if (threadIdx.x == 0) part[g] = sum;
     }
 }
 
@@ -724,14 +740,14 @@ inline int round8_mf(int n) { return (n + 7) & ~7; }
 
 template <typename TP>
 void launch_apply(efe_ctx* c, hipStream_t st, const TP* p, int mask, const int* flags,
-                  const int* active = nullptr) {
+                  const int* active = nullptr, double* part = nullptr) {
     const int grid = round8_mf((c->E + 255) / 256);
     const bool dp = c->mfNC == 10;
     if (c->mfb) {  // bricks (hexa8); grid: the bricks, rounded up to a multiple of 8
         const int gb = round8_mf(c->mfbG);
 #define MFB_ARGS                                                                                                 \
     c->E, c->N, c->mfbLptr, c->mfbLnode, c->mfbLpos, c->mfbLincp, c->mfbLinc, c->mfbLidx, c->coords, c->mfC, p, \
-        c->dirflag, mask, flags, c->mfbF, c->mfbG, c->mfbE0, active, c->specOfElem
+        c->dirflag, mask, flags, c->mfbF, c->mfbG, c->mfbE0, active, c->specOfElem, part
         if (dp) mfb_apply_hex8_kernel<10, TP><<<gb, kBrick, 0, st>>>(MFB_ARGS);
         else mfb_apply_hex8_kernel<8, TP><<<gb, kBrick, 0, st>>>(MFB_ARGS);
 #undef MFB_ARGS
@@ -918,6 +934,13 @@ void efe_mf_launch_operator(efe_ctx* c, hipStream_t st, const float* p, float* q
         case 512: launch_gather_op<512>(c, st, p, q, part, flags, nwg, kr); break;
         default: launch_gather_op<256>(c, st, p, q, part, flags, nwg, kr); break;
     }
+}
+
+// The brick product alone (hexa8 bricks only): the node partials of K p for the caller's own
+// gather, and per brick the partial of p . K p (Dirichlet DOFs masked) at part[g], g below
+// round8(mfbG).
+void efe_mf_launch_apply_partials(efe_ctx* c, hipStream_t st, const float* p, double* part, const int* flags) {
+    launch_apply<float>(c, st, p, 1, flags, nullptr, part);
 }
 
 // y = K x for FP64 vectors with the matrix-free FP32 operator (Dirichlet eliminated).
